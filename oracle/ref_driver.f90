@@ -1,0 +1,241 @@
+! Driver for the reference's own Fortran routines -- TEST INFRASTRUCTURE ONLY.
+!
+! oracle/build_ref.sh compiles this file together with the reference modules where they
+! lie (/root/reference/Source: global, mod_minimg, mod_force_nonbond,
+! mod_force_correction, mod_force_dihed, mod_force_bond, mod_force_angle, mod_pbc) into
+! oracle/_ref/libsofia_ref.so.  The entry points below only copy caller arrays into
+! `module global`, call one reference subroutine unchanged, and copy its outputs back,
+! so the golden-vector script in tests/golden/ can compare the C oracle with the
+! reference itself.  Nothing here is part of the product.
+module sofia_ref_driver
+  use iso_c_binding
+  use global
+  use mod_force_nonbond
+  use mod_force_correction
+  use mod_force_dihed
+  use mod_force_bond
+  use mod_force_angle
+  use mod_pbc
+  implicit none
+contains
+
+  ! Scalars of module global (precisions as declared in Source/global.f90).
+  subroutine ref_set_scalars(c_box, c_ibox, c_rcut2, c_cut_coulsq, c_alpha, c_gewald, &
+       c_e_shift, c_f_shift, c_coulpre, c_ewald_f, c_ewald_p, c_a, c_pis, c_pis_inv, &
+       c_fudge12, c_fudge13, c_fudge14) bind(c, name="ref_set_scalars")
+    real(c_float), value :: c_box, c_ibox, c_coulpre
+    real(c_double), value :: c_rcut2, c_cut_coulsq, c_alpha, c_gewald, c_e_shift, c_f_shift
+    real(c_double), value :: c_ewald_f, c_ewald_p, c_pis, c_pis_inv
+    real(c_double), value :: c_fudge12, c_fudge13, c_fudge14
+    real(c_double), intent(in) :: c_a(5)
+    box = c_box; ibox = c_ibox; coulpre = c_coulpre
+    rcut2 = c_rcut2; cut_coulsq = c_cut_coulsq
+    alpha = c_alpha; gewald = c_gewald
+    e_shift = c_e_shift; f_shift = c_f_shift
+    EWALD_F = c_ewald_f; EWALD_P = c_ewald_p
+    A1 = c_a(1); A2 = c_a(2); A3 = c_a(3); A4 = c_a(4); A5 = c_a(5)
+    MY_PIS = c_pis; MY_PIS_INV = c_pis_inv
+    fudge_12 = c_fudge12; fudge_13 = c_fudge13; fudge_14_coul = c_fudge14
+  end subroutine ref_set_scalars
+
+  ! Atoms (x, y, z, type), charges and the LJ tables.
+  subroutine ref_set_atoms(n, nt, xyz, typ, qq, l1, l2, l3, l4, l141, l142, l143, l144) &
+       bind(c, name="ref_set_atoms")
+    integer(c_int), value :: n, nt
+    real(c_float), intent(in) :: xyz(3, n), qq(n), l1(nt*nt), l2(nt*nt), l3(nt*nt), l4(nt*nt)
+    integer(c_int), intent(in) :: typ(n)
+    real(c_double), intent(in) :: l141(nt*nt), l142(nt*nt), l143(nt*nt), l144(nt*nt)
+    integer :: i, a, b
+    np = n; numAtomType = nt
+    if (allocated(position)) deallocate(position, q, ff, ffh, ff_bond, ff_angle, ff_dihed)
+    if (allocated(lj1)) deallocate(lj1, lj2, lj3, lj4, lj14_1, lj14_2, lj14_3, lj14_4)
+    allocate(position(n), q(n), ff(n), ffh(n), ff_bond(n), ff_angle(n), ff_dihed(n))
+    allocate(lj1(nt*nt), lj2(nt*nt), lj3(nt*nt), lj4(nt*nt))
+    allocate(lj14_1(nt,nt), lj14_2(nt,nt), lj14_3(nt,nt), lj14_4(nt,nt))
+    do i = 1, n
+       position(i)%x = xyz(1, i); position(i)%y = xyz(2, i); position(i)%z = xyz(3, i)
+       position(i)%type = typ(i)
+       q(i) = qq(i)
+    end do
+    lj1 = l1; lj2 = l2; lj3 = l3; lj4 = l4
+    ! lj14 tables are (itype, jtype) arrays; the flat input is row-major (i-1)*nt + j
+    do a = 1, nt
+       do b = 1, nt
+          lj14_1(a, b) = l141((a-1)*nt + b); lj14_2(a, b) = l142((a-1)*nt + b)
+          lj14_3(a, b) = l143((a-1)*nt + b); lj14_4(a, b) = l144((a-1)*nt + b)
+       end do
+    end do
+  end subroutine ref_set_atoms
+
+  subroutine get_forces(src, out)
+    type(coords), intent(in) :: src(:)
+    real(c_float), intent(out) :: out(3, *)
+    integer :: i
+    do i = 1, np
+       out(1, i) = src(i)%x; out(2, i) = src(i)%y; out(3, i) = src(i)%z
+    end do
+  end subroutine get_forces
+
+  subroutine zero_forces()
+    integer :: i
+    do i = 1, np
+       ff(i) = coords(0.0, 0.0, 0.0); ffh(i) = coords(0.0, 0.0, 0.0)
+       ff_bond(i) = coords(0.0, 0.0, 0.0); ff_angle(i) = coords(0.0, 0.0, 0.0)
+       ff_dihed(i) = coords(0.0, 0.0, 0.0)
+    end do
+  end subroutine zero_forces
+
+  ! lj_cut_coul_dsf_nonewton_NOMIC (Source/mod_force_nonbond.f90:105-196) on a given
+  ! Verlet list (1-based ids, stride na).
+  subroutine ref_dsf(na, nl, nn, fout, epot, ecoul) bind(c, name="ref_dsf")
+    integer(c_int), value :: na
+    integer(c_int), intent(in) :: nl(na*np), nn(np)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: epot, ecoul
+    if (allocated(nlist)) deallocate(nlist, numneigh)
+    allocate(nlist(na*np), numneigh(np))
+    nlist = nl; numneigh = nn; neigh_alloc = na
+    call zero_forces()
+    potential = 0.0d0; e_coul = 0.0d0
+    call lj_cut_coul_dsf_nonewton_NOMIC(0)
+    call get_forces(ff, fout)
+    epot = potential; ecoul = e_coul
+  end subroutine ref_dsf
+
+  ! lj_cut_coul_long_nonewton_NOMIC (Source/mod_force_nonbond.f90:300-400).
+  subroutine ref_coul_long(na, nl, nn, fout, epot, ecoul, vir) bind(c, name="ref_coul_long")
+    integer(c_int), value :: na
+    integer(c_int), intent(in) :: nl(na*np), nn(np)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: epot, ecoul, vir(3)
+    if (allocated(nlist)) deallocate(nlist, numneigh)
+    allocate(nlist(na*np), numneigh(np))
+    nlist = nl; numneigh = nn; neigh_alloc = na
+    call zero_forces()
+    potential = 0.0d0; e_coul = 0.0d0
+    call lj_cut_coul_long_nonewton_NOMIC
+    call get_forces(ff, fout)
+    epot = potential; ecoul = e_coul
+    vir(1) = virialx; vir(2) = virialy; vir(3) = virialz
+  end subroutine ref_coul_long
+
+  ! lj_cut_coul_dsf_N2 (Source/mod_force_nonbond.f90:409-502) with the exclusion table.
+  subroutine ref_dsf_n2(n1, n2, n3, spec, fout, epot, ecoul) bind(c, name="ref_dsf_n2")
+    integer(c_int), intent(in) :: n1(np), n2(np), n3(np), spec(12, np)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: epot, ecoul
+    if (allocated(num1bond)) deallocate(num1bond, num2bond, num3bond, specbond)
+    allocate(num1bond(np), num2bond(np), num3bond(np), specbond(12, np))
+    num1bond = n1; num2bond = n2; num3bond = n3; specbond = spec
+    call zero_forces()
+    potential = 0.0d0; e_coul = 0.0d0
+    call lj_cut_coul_dsf_N2(0)
+    call get_forces(ff, fout)
+    epot = potential; ecoul = e_coul
+  end subroutine ref_dsf_n2
+
+  ! correct_12_bonds / correct_13_bonds / correct_14_bonds
+  ! (Source/mod_force_correction.f90:6-278).  kind = 12, 13 or 14; lst holds the
+  ! reference list (bondlist(3,n) / anglelist(4,n) / dihedlist(5,n)).
+  subroutine ref_correct(kind, n, width, lst, fout, ec, vir) bind(c, name="ref_correct")
+    integer(c_int), value :: kind, n, width
+    integer(c_int), intent(in) :: lst(width, n)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: ec, vir(3)
+    call zero_forces()
+    e_coul_bond = 0.0d0; e_coul_angle = 0.0d0; e_coul_dihed = 0.0d0
+    v_bondx = 0.0d0; v_bondy = 0.0d0; v_bondz = 0.0d0
+    v_anglex = 0.0d0; v_angley = 0.0d0; v_anglez = 0.0d0
+    v_dihedx = 0.0d0; v_dihedy = 0.0d0; v_dihedz = 0.0d0
+    if (kind == 12) then
+       if (allocated(bondlist)) deallocate(bondlist)
+       allocate(bondlist(3, n)); bondlist = lst; numbond = n
+       call correct_12_bonds
+       call get_forces(ff_bond, fout); ec = e_coul_bond
+       vir(1) = v_bondx; vir(2) = v_bondy; vir(3) = v_bondz
+    else if (kind == 13) then
+       if (allocated(anglelist)) deallocate(anglelist)
+       allocate(anglelist(4, n)); anglelist = lst; numangle = n
+       call correct_13_bonds
+       call get_forces(ff_angle, fout); ec = e_coul_angle
+       vir(1) = v_anglex; vir(2) = v_angley; vir(3) = v_anglez
+    else
+       if (allocated(dihedlist)) deallocate(dihedlist)
+       allocate(dihedlist(5, n)); dihedlist = lst; numdihed = n
+       call correct_14_bonds
+       call get_forces(ff_dihed, fout); ec = e_coul_dihed
+       vir(1) = v_dihedx; vir(2) = v_dihedy; vir(3) = v_dihedz
+    end if
+  end subroutine ref_correct
+
+  ! dihedral_opls_DSF (Source/mod_force_dihed.f90:816-1092) with every torsion
+  ! coefficient zero, which leaves exactly the scaled 1-4 DSF + LJ pair term
+  ! (lines 1003-1048) in ffh, e_coul_14 and pot_14.
+  subroutine ref_pair14(n, lst, fout, ec14, pe14) bind(c, name="ref_pair14")
+    integer(c_int), value :: n
+    integer(c_int), intent(in) :: lst(5, n)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: ec14, pe14
+    integer :: ntype
+    if (allocated(dihedlist)) deallocate(dihedlist)
+    allocate(dihedlist(5, n)); dihedlist = lst; numdihed = n
+    ntype = maxval(lst(5, :))
+    if (allocated(dihedcoeff)) deallocate(dihedcoeff)
+    allocate(dihedcoeff(4, ntype)); dihedcoeff = 0.0d0
+    call zero_forces()
+    e_coul_14 = 0.0d0; pot_14 = 0.0d0
+    call dihedral_opls_DSF(0)
+    call get_forces(ffh, fout)
+    ec14 = e_coul_14; pe14 = pot_14
+  end subroutine ref_pair14
+
+  ! bond_harmonic / angle_harmonic (Source/mod_force_bond.f90:6-75,
+  ! Source/mod_force_angle.f90:10-122).
+  subroutine ref_bond(n, lst, nbt, kb, r0, fout, eb) bind(c, name="ref_bond")
+    integer(c_int), value :: n, nbt
+    integer(c_int), intent(in) :: lst(3, n)
+    real(c_double), intent(in) :: kb(nbt), r0(nbt)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: eb
+    if (allocated(bondlist)) deallocate(bondlist)
+    allocate(bondlist(3, n)); bondlist = lst; numbond = n
+    if (allocated(bondcoeff)) deallocate(bondcoeff)
+    allocate(bondcoeff(2, nbt)); bondcoeff(1, :) = kb; bondcoeff(2, :) = r0
+    call zero_forces()
+    e_bond = 0.0d0
+    call bond_harmonic(0)
+    call get_forces(ff_bond, fout)
+    eb = e_bond
+  end subroutine ref_bond
+
+  subroutine ref_angle(n, lst, nat, k, th0, fout, ea) bind(c, name="ref_angle")
+    integer(c_int), value :: n, nat
+    integer(c_int), intent(in) :: lst(4, n)
+    real(c_double), intent(in) :: k(nat), th0(nat)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: ea
+    if (allocated(anglelist)) deallocate(anglelist)
+    allocate(anglelist(4, n)); anglelist = lst; numangle = n
+    if (allocated(anglecoeff)) deallocate(anglecoeff)
+    allocate(anglecoeff(2, nat)); anglecoeff(1, :) = k; anglecoeff(2, :) = th0
+    call zero_forces()
+    e_angle = 0.0d0
+    call angle_harmonic(0)
+    call get_forces(ff_angle, fout)
+    ea = e_angle
+  end subroutine ref_angle
+
+  ! pbc (Source/mod_pbc.f90:6-39) on xyz in place.
+  subroutine ref_pbc(xyz) bind(c, name="ref_pbc")
+    real(c_float), intent(inout) :: xyz(3, np)
+    integer :: i
+    do i = 1, np
+       position(i)%x = xyz(1, i); position(i)%y = xyz(2, i); position(i)%z = xyz(3, i)
+    end do
+    call pbc(0)
+    do i = 1, np
+       xyz(1, i) = position(i)%x; xyz(2, i) = position(i)%y; xyz(3, i) = position(i)%z
+    end do
+  end subroutine ref_pbc
+
+end module sofia_ref_driver
