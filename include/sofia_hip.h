@@ -1,0 +1,216 @@
+/*
+ * sofia_hip.h -- C ABI of the MI355X (gfx950) implementation of Sofia's hot path:
+ * cell binning + Verlet-list build (Source/mod_neighbor.f90), LJ + damped-shifted-force
+ * pair forces (Source/mod_force_nonbond.f90), exclusion corrections
+ * (Source/mod_force_correction.f90) and the scaled 1-4 pair of dihedral_opls_DSF
+ * (Source/mod_force_dihed.f90:1003-1048), with FP64 velocity Verlet, PBC, Langevin and
+ * SHAKE (Source/mod_ensemble.f90, mod_pbc.f90, mod_langevin.f90, mod_shake.f90).
+ *
+ * Plain pointers and sizes only.  Every function returns SOFIA_OK (0) or a negative
+ * status; sofia_last_error() gives the message of the last failure on this thread.
+ *
+ * Data conventions are the reference's (Source/global.f90): per-atom arrays are in the
+ * current (cell-sorted) order, atom ids stored in lists are 1-based, `sofia_ainfo` is
+ * `type ainfo` (x, y, z real*4 + integer type, 16 B), `sofia_coords` is `type coords`
+ * (3 x real*4), specbond is Fortran's column-major specbond(12, np).
+ *
+ * Two groups of entry points:
+ *  (1) drop-ins for the reference subroutines, operating on one system whose state the
+ *      host uploads (sofia_put_state) and reads back -- each cites the routine it replaces;
+ *  (2) the device-resident engine: many independent replicas (one system = a batch of
+ *      one) advanced in fused, hipGraph-captured MD segments, in FP64 or mixed precision.
+ * A context is used from one host thread; work is ordered on the context's stream.
+ */
+#ifndef SOFIA_HIP_H
+#define SOFIA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SOFIA_OK = 0,
+    SOFIA_E_NODEV = -1,      /* no HIP device */
+    SOFIA_E_OOM = -2,        /* device allocation failed */
+    SOFIA_E_ARG = -3,        /* invalid argument or size */
+    SOFIA_E_OVERFLOW = -4,   /* Verlet list overflow that could not be grown */
+    SOFIA_E_HIP = -5,        /* HIP runtime error */
+    SOFIA_E_SHAKE = -6,      /* SHAKE determinant <= 0 (the reference stops) */
+    SOFIA_E_STATE = -7       /* call out of order (e.g. no list yet) */
+};
+
+enum { SOFIA_PREC_FP64 = 0, SOFIA_PREC_MIXED = 1 };
+enum { SOFIA_COUL_DSF = 1, SOFIA_COUL_EWALD = 2 };   /* coul_flag, input_files/input:26 */
+
+typedef struct { float x, y, z; int32_t type; } sofia_ainfo;   /* Source/global.f90:82-85 */
+typedef struct { float x, y, z; } sofia_coords;                /* Source/global.f90:87-89 */
+
+/* Run parameters with the reference's precisions (Source/global.f90:69-137,
+ * derived as in Source/get_started.f90:25-117). */
+typedef struct {
+    int32_t ntypes;
+    float rcut, cut_coul, pad;          /* real*4 */
+    float coulpre;                      /* real*4 332.06371 */
+    float dt;                           /* fs, real*4 */
+    float kcal2amu;                     /* real*4 0.0004184 */
+    double rcut2, cut_coulsq;           /* from real*4 products */
+    double rv2, rvrc2;                  /* (cut_coul+pad)^2, pad^2 */
+    double alpha, gewald;
+    double e_shift, f_shift;            /* DSF shifts */
+    double ewald_p, ewald_f;
+    double a1, a2, a3, a4, a5;          /* A&S 7.1.26 */
+    double my_pis, my_pis_inv;
+    double amu2e;
+    double fudge_12, fudge_13, fudge_14_coul;
+    double shake_tol;                   /* 1e-4 (real*4 literal) */
+    int32_t shake_maxiter;              /* 25 */
+    int32_t precision;                  /* SOFIA_PREC_* (engine) */
+    int32_t coul_flag;                  /* SOFIA_COUL_* (engine) */
+    int32_t neigh_alloc0;               /* initial list stride (512, Source/mod_memory.f90:137) */
+} sofia_params;
+
+typedef struct sofia_ctx sofia_ctx;
+
+/* ---------------------------------------------------------------- lifecycle */
+int sofia_last_error(char *buf, size_t len);
+int sofia_device_count(int *count);
+/* One system of np atoms (a batch of one). */
+int sofia_ctx_create(const sofia_params *p, int32_t np, float box, sofia_ctx **out);
+/* nrep replicas; natoms[r] atoms and box[r] for replica r (atoms stored back to back). */
+int sofia_ctx_create_batch(const sofia_params *p, int32_t nrep, const int32_t *natoms,
+                           const float *box, sofia_ctx **out);
+int sofia_ctx_destroy(sofia_ctx *ctx);
+int sofia_synchronize(sofia_ctx *ctx);
+
+/* ---------------------------------------------------------------- force field / topology
+ * All ids are 1-based within a replica; arrays of a batch are concatenated replica by
+ * replica.  lj1..4 real*4 and lj14_1..4 real*8 are ntypes*ntypes, index
+ * (itype-1)*ntypes + jtype-1 (Source/mod_readfile.f90:595-615); mass is per type. */
+int sofia_set_forcefield(sofia_ctx *ctx, const float *lj1, const float *lj2, const float *lj3,
+                         const float *lj4, const double *lj14_1, const double *lj14_2,
+                         const double *lj14_3, const double *lj14_4, const double *mass);
+/* num1bond/num2bond/num3bond and specbond(12, np) (Source/mod_buildbond_lists.f90:80-183) */
+int sofia_set_exclusions(sofia_ctx *ctx, const int32_t *num1bond, const int32_t *num2bond,
+                         const int32_t *num3bond, const int32_t *specbond);
+/* bondlist(3,nb), anglelist(4,na), dihedlist(5,nd) as in Source/mod_buildbond_lists.f90;
+ * per-replica counts in nb_rep/na_rep/nd_rep (NULL for a single system). */
+int sofia_set_bonded(sofia_ctx *ctx, const int32_t *bondlist, int32_t nb, const int32_t *nb_rep,
+                     const int32_t *anglelist, int32_t na, const int32_t *na_rep,
+                     const int32_t *dihedlist, int32_t nd, const int32_t *nd_rep);
+/* Bonded coefficients for the engine: bond (k, r0), angle (k, theta0 rad), torsion
+ * (K, n, shift deg) per type; torsions(5, nt) in geometric order a-b-c-d. */
+int sofia_set_bonded_coeffs(sofia_ctx *ctx, int32_t nbt, const double *bond_k,
+                            const double *bond_r0, int32_t nat, const double *angle_k,
+                            const double *angle_th0, int32_t ntt, const double *tors_k,
+                            const double *tors_n, const double *tors_shift,
+                            const int32_t *torsions, int32_t ntor, const int32_t *ntor_rep);
+/* SHAKE clusters: num (1, 3, 4, -3), atoms(4, ns) (0 = unused), bond lengths(3, ns)
+ * (Source/global.f90:351-355 shake_info). */
+int sofia_set_shake(sofia_ctx *ctx, int32_t ns, const int32_t *ns_rep, const int32_t *num,
+                    const int32_t *atoms, const double *bond);
+
+/* ---------------------------------------------------------------- state transfer */
+int sofia_put_state(sofia_ctx *ctx, const sofia_ainfo *pos, const float *q,
+                    const sofia_coords *v, const int32_t *global_id);
+int sofia_get_state(sofia_ctx *ctx, sofia_ainfo *pos, sofia_coords *v, int32_t *global_id);
+/* FP64 positions/velocities of the engine (x, y, z interleaved per atom), may be NULL */
+int sofia_get_state_f64(sofia_ctx *ctx, double *x, double *v);
+int sofia_put_state_f64(sofia_ctx *ctx, const double *x, const double *v);
+/* drx/dry/drz displacement accumulators since the last list build */
+int sofia_get_displacement(sofia_ctx *ctx, float *drx, float *dry, float *drz);
+int sofia_put_displacement(sofia_ctx *ctx, const float *drx, const float *dry, const float *drz);
+
+/* ---------------------------------------------------------------- (1) drop-ins */
+/* neighbor_wrapper(step), Source/mod_neighbor.f90:8-86: check_allocation, new_nlist,
+ * sort_posit_charge (the permutation is applied to the device copies of position, q, v,
+ * global_id, num*bond, specbond and the bonded/SHAKE lists), then
+ * build_neighbor_nonewton with grow-and-redo (Source/mod_neighbor.f90:258-298).
+ * tt (optional, np) receives tt(old) = new (1-based) so the host can re-index its own
+ * arrays.  Zeroes drx/dry/drz. */
+int sofia_neighbor_wrapper(sofia_ctx *ctx, int32_t step, int32_t *tt);
+/* cell ranges of the last build: start/endposit (1-based, per cell 0..ncellT-1) and
+ * atombin (per atom); ncellT/ncellD/rn of replica 0 */
+int sofia_get_cells(sofia_ctx *ctx, int32_t *start, int32_t *endposit, int32_t *atombin,
+                    int32_t *ncellT, int32_t *ncellD, float *rn);
+/* nlist (np*neigh_alloc, 1-based ids, reference layout) and numneigh; NULL skips. */
+int sofia_get_nlist(sofia_ctx *ctx, int32_t *nlist, int32_t *numneigh, int32_t *neigh_alloc);
+/* check_neighbor(nflag, step), Source/mod_neighbor.f90:895-913 */
+int sofia_check_neighbor(sofia_ctx *ctx, int32_t *nflag);
+/* lj_cut_coul_dsf_nonewton_NOMIC/_MIC, Source/mod_force_nonbond.f90:105-196 / 8-101:
+ * FP64 pair math over the current list; ff (np, optional) overwritten with the real*4
+ * forces, ff64 (3*np, optional) with the FP64 sums; potential/e_coul are ADDED the
+ * halved sums; virial (optional, 3) receives 0.5*sum dx^2 F/r. */
+int sofia_lj_cut_coul_dsf(sofia_ctx *ctx, sofia_coords *ff, double *ff64, double *potential,
+                          double *e_coul, double *virial);
+/* Mixed-precision variant: FP32 pair math, int64 fixed-point force sums. */
+int sofia_lj_cut_coul_dsf_mixed(sofia_ctx *ctx, sofia_coords *ff, double *ff64,
+                                double *potential, double *e_coul, double *virial);
+/* lj_cut_coul_long_nonewton_NOMIC, Source/mod_force_nonbond.f90:300-400 (PME real space) */
+int sofia_lj_cut_coul_long(sofia_ctx *ctx, sofia_coords *ff, double *potential,
+                           double *e_coul, double *virial);
+/* correct_12_bonds / correct_13_bonds / correct_14_bonds,
+ * Source/mod_force_correction.f90:6-93 / 95-184 / 187-278; which = 12, 13 or 14.
+ * ff (np) receives this term's forces (ff_bond/ff_angle/ff_dihed), e_coul and virial
+ * (3) are ADDED to. */
+int sofia_correct_bonds(sofia_ctx *ctx, int32_t which, sofia_coords *ff, double *e_coul,
+                        double *virial);
+/* The 1-4 pair inside dihedral_opls_DSF, Source/mod_force_dihed.f90:1003-1048;
+ * ffh (np) receives the pair forces, e_coul_14/pot_14 are ADDED to. */
+int sofia_pair14_dsf(sofia_ctx *ctx, sofia_coords *ffh, double *e_coul_14, double *pot_14);
+/* nve_v(dt_partial) / nve_x / pbc(step), Source/mod_ensemble.f90:11-87,
+ * Source/mod_pbc.f90:6-39, on the real*4 state with force ff (np). */
+int sofia_nve_v(sofia_ctx *ctx, float dt_partial, const sofia_coords *ff, double *ke);
+int sofia_nve_x(sofia_ctx *ctx);
+int sofia_pbc(sofia_ctx *ctx);
+/* shake(iflag), Source/mod_shake.f90:8-85: adds constraint forces to ff (np, in/out) and
+ * to virial (3).  max_iter (optional) receives the largest iteration count. */
+int sofia_shake(sofia_ctx *ctx, int32_t iflag, sofia_coords *ff, double *virial,
+                int32_t *max_iter);
+/* langevin_debug_intel, Source/mod_langevin.f90:15-43, with a counter-based RNG:
+ * ff_lang = -fric*m*v + gconst*sqrt(m)*U(-1/2,1/2), U from Philox4x32-10 keyed by
+ * (seed, step, global_id). */
+int sofia_langevin(sofia_ctx *ctx, float fric, float gconst, uint64_t seed, uint64_t step,
+                   sofia_coords *ff_lang);
+
+/* ---------------------------------------------------------------- (2) engine */
+typedef struct {
+    double temp;        /* K */
+    double gamma_ps;    /* Langevin friction, 1/ps (0 = NVE) */
+    uint64_t seed;      /* RNG key */
+} sofia_thermostat;
+
+/* Per-replica thermostat (nrep entries); default is NVE. */
+int sofia_set_thermostat(sofia_ctx *ctx, const sofia_thermostat *t);
+/* Copy the uploaded real*4 state into the FP64 engine state, build the list and compute
+ * the starting forces (force_driver(0) + shake(1), Source/mod_integrate.f90:237-240). */
+int sofia_md_init(sofia_ctx *ctx);
+/* Advance every replica nsteps velocity-Verlet steps (Source/mod_integrate.f90:270-308):
+ * nve_v, nve_x, pbc, check_neighbor, list rebuild when flagged, forces, Langevin,
+ * SHAKE, nve_v.  use_graph != 0 captures one step in a hipGraph and replays it. */
+int sofia_md_run(sofia_ctx *ctx, int32_t nsteps, int32_t use_graph);
+/* Capped steepest descent on the engine state (no SHAKE, no thermostat): each atom
+ * moves along its force by min(gamma*|F|, max_disp) A per step; ends with a full force
+ * evaluation so sofia_md_run can continue.  For relaxing generated starting structures. */
+int sofia_minimize(sofia_ctx *ctx, int32_t nsteps, double gamma, double max_disp);
+/* Energies per replica after the last step, 8 doubles each: potential (LJ), e_coul,
+ * pot_14, e_coul_14, e_bond, e_angle, e_dihedral, ke. */
+int sofia_md_energies(sofia_ctx *ctx, double *out);
+/* Number of list builds per replica since md_init (nrep ints). */
+int sofia_md_rebuilds(sofia_ctx *ctx, int32_t *out);
+/* Tuning knobs (autotuner): lanes per atom in the pair kernel (4..64, power of two),
+ * threads per block of the pair kernel, waves per block in the list build. */
+int sofia_set_tuning(sofia_ctx *ctx, int32_t lanes_per_atom, int32_t block, int32_t list_waves);
+/* Union-find clusters of solute molecules joined by an O-N contact < rcontact (A):
+ * atoms whose type is in o_types/n_types, molecule id per atom in mol (0-based within a
+ * replica, -1 = solvent).  largest[r] = largest cluster (molecules) per replica. */
+int sofia_set_molecules(sofia_ctx *ctx, const int32_t *mol);
+int sofia_cluster_largest(sofia_ctx *ctx, float rcontact, int32_t n_o, const int32_t *o_types,
+                          int32_t n_n, const int32_t *n_types, int32_t *largest);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOFIA_HIP_H */

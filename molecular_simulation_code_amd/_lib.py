@@ -1,0 +1,140 @@
+"""ctypes binding of libsofia_hip.so (include/sofia_hip.h).
+
+The library is loaded from this package directory only; if it is missing the import of
+anything that needs it raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsofia_hip.so")
+
+SOFIA_OK = 0
+ERRORS = {-1: "E_NODEV", -2: "E_OOM", -3: "E_ARG", -4: "E_OVERFLOW", -5: "E_HIP",
+          -6: "E_SHAKE", -7: "E_STATE"}
+PREC_FP64, PREC_MIXED = 0, 1
+COUL_DSF, COUL_EWALD = 1, 2
+
+
+class SofiaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SofiaParams(C.Structure):
+    _fields_ = [("ntypes", C.c_int32), ("rcut", C.c_float), ("cut_coul", C.c_float),
+                ("pad", C.c_float), ("coulpre", C.c_float), ("dt", C.c_float),
+                ("kcal2amu", C.c_float), ("rcut2", C.c_double), ("cut_coulsq", C.c_double),
+                ("rv2", C.c_double), ("rvrc2", C.c_double), ("alpha", C.c_double),
+                ("gewald", C.c_double), ("e_shift", C.c_double), ("f_shift", C.c_double),
+                ("ewald_p", C.c_double), ("ewald_f", C.c_double), ("a1", C.c_double),
+                ("a2", C.c_double), ("a3", C.c_double), ("a4", C.c_double), ("a5", C.c_double),
+                ("my_pis", C.c_double), ("my_pis_inv", C.c_double), ("amu2e", C.c_double),
+                ("fudge_12", C.c_double), ("fudge_13", C.c_double),
+                ("fudge_14_coul", C.c_double), ("shake_tol", C.c_double),
+                ("shake_maxiter", C.c_int32), ("precision", C.c_int32),
+                ("coul_flag", C.c_int32), ("neigh_alloc0", C.c_int32)]
+
+
+class SofiaThermostat(C.Structure):
+    _fields_ = [("temp", C.c_double), ("gamma_ps", C.c_double), ("seed", C.c_uint64)]
+
+
+# exported entry points: name -> argtypes (all return int)
+_vp = C.c_void_p
+_i32 = C.c_int32
+_f = C.c_float
+_d = C.c_double
+_u64 = C.c_uint64
+SIGNATURES = {
+    "sofia_last_error": [C.c_char_p, C.c_size_t],
+    "sofia_device_count": [_vp],
+    "sofia_ctx_create": [_vp, _i32, _f, _vp],
+    "sofia_ctx_create_batch": [_vp, _i32, _vp, _vp, _vp],
+    "sofia_ctx_destroy": [_vp],
+    "sofia_synchronize": [_vp],
+    "sofia_set_forcefield": [_vp] * 10,
+    "sofia_set_exclusions": [_vp] * 5,
+    "sofia_set_bonded": [_vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp],
+    "sofia_set_bonded_coeffs": [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp,
+                                _i32, _vp],
+    "sofia_set_shake": [_vp, _i32, _vp, _vp, _vp, _vp],
+    "sofia_put_state": [_vp] * 5,
+    "sofia_get_state": [_vp] * 4,
+    "sofia_get_state_f64": [_vp] * 3,
+    "sofia_put_state_f64": [_vp] * 3,
+    "sofia_get_displacement": [_vp] * 4,
+    "sofia_put_displacement": [_vp] * 4,
+    "sofia_neighbor_wrapper": [_vp, _i32, _vp],
+    "sofia_get_cells": [_vp] * 7,
+    "sofia_get_nlist": [_vp] * 4,
+    "sofia_check_neighbor": [_vp, _vp],
+    "sofia_lj_cut_coul_dsf": [_vp] * 6,
+    "sofia_lj_cut_coul_dsf_mixed": [_vp] * 6,
+    "sofia_lj_cut_coul_long": [_vp] * 5,
+    "sofia_correct_bonds": [_vp, _i32, _vp, _vp, _vp],
+    "sofia_pair14_dsf": [_vp] * 4,
+    "sofia_nve_v": [_vp, _f, _vp, _vp],
+    "sofia_nve_x": [_vp],
+    "sofia_pbc": [_vp],
+    "sofia_shake": [_vp, _i32, _vp, _vp, _vp],
+    "sofia_langevin": [_vp, _f, _f, _u64, _u64, _vp],
+    "sofia_set_thermostat": [_vp, _vp],
+    "sofia_md_init": [_vp],
+    "sofia_md_run": [_vp, _i32, _i32],
+    "sofia_minimize": [_vp, _i32, _d, _d],
+    "sofia_md_energies": [_vp, _vp],
+    "sofia_md_rebuilds": [_vp, _vp],
+    "sofia_set_tuning": [_vp, _i32, _i32, _i32],
+    "sofia_set_molecules": [_vp, _vp],
+    "sofia_cluster_largest": [_vp, _f, _i32, _vp, _i32, _vp, _vp],
+}
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m molecular_simulation_code_amd.build`"
+                               " (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().sofia_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc != SOFIA_OK:
+        raise SofiaError(rc, last_error())
+
+
+def ptr(a) -> C.c_void_p | None:
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"], "arrays passed to the C ABI must be C-contiguous"
+        return C.c_void_p(a.ctypes.data)
+    return C.cast(C.pointer(a), C.c_void_p)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().sofia_device_count(C.byref(n)))
+    return n.value

@@ -1,0 +1,303 @@
+// Bonded-pair kernels: Ewald exclusion corrections, the scaled 1-4 DSF pair, and the
+// harmonic bond / harmonic angle / cosine torsion terms of the full MD step.
+//
+// One thread per list item; forces go to a per-atom int64 fixed-point accumulator with
+// atomics (2^-32 kcal/mol/A quanta), so totals are exact and independent of thread
+// order.  Energies go to per-replica int64 slots the same way.
+//
+// Drop-in variants read the real*4 positions and follow the reference's precision
+// (real*4 coordinate difference and charge product, double for the rest); the engine
+// variants read the FP64 positions of the device-resident state.
+#include "sofia_device.h"
+
+namespace sofia {
+namespace {
+
+__device__ inline double erfc_as(double t, const DevParams &P) {
+    return t * (P.a1 + t * (P.a2 + t * (P.a3 + t * (P.a4 + t * P.a5))));
+}
+
+// correct_12/13/14_bonds, Source/mod_force_correction.f90:23-86 (and 112-177, 205-272)
+__global__ void k_correct(int32_t items, int32_t width, int32_t colb, const int32_t *__restrict__ lst,
+                          const float4 *__restrict__ pos4, const float *__restrict__ q,
+                          const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
+                          DevParams P, double fudge, int32_t n, long long *__restrict__ fout,
+                          long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < items;
+    double e = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+    int r = 0;
+    if (live) {
+        const int i1 = lst[(size_t)m * width], i2 = lst[(size_t)m * width + colb];
+        r = rep[i1];
+        const double bx = reps[r].box, ib = reps[r].ibox;
+        const float4 p1 = pos4[i1], p2 = pos4[i2];
+        double dx = (double)(p1.x - p2.x), dy = (double)(p1.y - p2.y), dz = (double)(p1.z - p2.z);
+        dx -= bx * rint(dx * ib);
+        dy -= bx * rint(dy * ib);
+        dz -= bx * rint(dz * ib);
+        const double dr2 = dx * dx + dy * dy + dz * dz;
+        const double dr2i = 1.0 / dr2;
+        const double rr = sqrt(dr2);
+        double pref = (double)((P.coulpre * q[i1]) * q[i2]) / rr;
+        const double grij = P.gewald * rr;
+        const double expm2 = exp(-grij * grij);
+        const double t = 1.0 / (1.0 + P.ewald_p * grij);
+        const double erfc = erfc_as(t, P) * expm2;
+        if (dr2 > P.cut_coulsq) pref = 0.0;
+        const double fc = pref * (erfc + P.ewald_f * grij * expm2) - (1.0 - fudge) * pref;
+        e = pref * erfc - (1.0 - fudge) * pref;
+        const double force = fc * dr2i;
+        atomic_fix(&fout[i1], dx * force);
+        atomic_fix(&fout[(size_t)n + i1], dy * force);
+        atomic_fix(&fout[2 * (size_t)n + i1], dz * force);
+        atomic_fix(&fout[i2], -dx * force);
+        atomic_fix(&fout[(size_t)n + i2], -dy * force);
+        atomic_fix(&fout[2 * (size_t)n + i2], -dz * force);
+        vx = dx * dx * force;
+        vy = dy * dy * force;
+        vz = dz * dz * force;
+    }
+    wave_add(to_fix(e), live, r, eacc, E_CORR);
+    wave_add(to_fix(vx), live, r, eacc, V_X);
+    wave_add(to_fix(vy), live, r, eacc, V_Y);
+    wave_add(to_fix(vz), live, r, eacc, V_Z);
+}
+
+// The 1-4 pair of dihedral_opls_DSF, Source/mod_force_dihed.f90:887-895 and 1003-1048:
+// nint minimum image, 0.5-scaled DSF Coulomb inside cut_coulsq, lj14 tables inside rcut2.
+template <bool F64POS>
+__global__ void k_pair14(int32_t items, const int32_t *__restrict__ dl, const float4 *__restrict__ pos4,
+                         const double *__restrict__ xd, const float *__restrict__ q,
+                         const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
+                         const double4 *__restrict__ lj14, DevParams P, int32_t n,
+                         long long *__restrict__ fout, long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < items;
+    double ec = 0.0, el = 0.0;
+    int r = 0;
+    if (live) {
+        const int i1 = dl[(size_t)m * 5], i4 = dl[(size_t)m * 5 + 3];
+        r = rep[i1];
+        const float box = reps[r].box, ibox = reps[r].ibox;
+        double d[3];
+        if (F64POS) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double v = xd[(size_t)c * n + i1] - xd[(size_t)c * n + i4];
+                d[c] = v - (double)box * rint(v * (double)ibox);
+            }
+        } else {
+            const float4 p1 = pos4[i1], p4 = pos4[i4];
+            const float df[3] = {p1.x - p4.x, p1.y - p4.y, p1.z - p4.z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                // Source/mod_minimg.f90: dx - box*nint(ibox*dx), box*nint a real*4 product
+                const double v = (double)df[c];
+                const float shift = box * (float)(int)round((double)ibox * v);
+                d[c] = v - (double)shift;
+            }
+        }
+        const double dr2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const float4 pi1 = pos4[i1], pi4 = pos4[i4];
+        const int ti = __float_as_int(pi1.w) - 1, tj = __float_as_int(pi4.w) - 1;
+        double force = 0.0;
+        if (dr2 < P.cut_coulsq) {
+            const double rr = sqrt(dr2);
+            const double dr2i = 1.0 / dr2;
+            const double pref = (double)((P.coulpre * q[i1]) * q[i4]) / rr;
+            const double erfcd = exp(-P.alpha * P.alpha * rr * rr);
+            const double t = 1.0 / (1.0 + P.ewald_p * P.alpha * rr);
+            const double erfcc = erfc_as(t, P) * erfcd;
+            const double fc =
+                0.5 * pref * (erfcc / rr + 2.0 * P.alpha / P.my_pis * erfcd + rr * P.f_shift) * rr;
+            ec = 0.5 * pref * (erfcc - rr * P.e_shift - dr2 * P.f_shift);
+            double flj = 0.0;
+            if (dr2 < P.rcut2) {
+                const double dr6i = dr2i * dr2i * dr2i;
+                const double4 c = lj14[ti * P.ntypes + tj];
+                flj = dr6i * (c.x * dr6i - c.y);
+                el = dr6i * (dr6i * c.z - c.w);
+            }
+            force = (fc + flj) * dr2i;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            atomic_fix(&fout[(size_t)c * n + i1], d[c] * force);
+            atomic_fix(&fout[(size_t)c * n + i4], -d[c] * force);
+        }
+    }
+    wave_add(to_fix(ec), live, r, eacc, E_COUL14);
+    wave_add(to_fix(el), live, r, eacc, E_POT14);
+}
+
+__device__ inline void dvec64(const double *xd, int n, int a, int b, double box, double ibox, double *o) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double v = xd[(size_t)c * n + a] - xd[(size_t)c * n + b];
+        o[c] = v - box * rint(v * ibox);
+    }
+}
+
+// bond_harmonic, Source/mod_force_bond.f90:24-45: E = k (r - r0)^2
+__global__ void k_bond(int32_t items, const int32_t *__restrict__ bl, const double *__restrict__ xd,
+                       const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
+                       const double *__restrict__ kb, const double *__restrict__ r0, int32_t n,
+                       long long *__restrict__ fout, long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < items;
+    double e = 0.0;
+    int r = 0;
+    if (live) {
+        const int i1 = bl[(size_t)m * 3], i2 = bl[(size_t)m * 3 + 1], t = bl[(size_t)m * 3 + 2] - 1;
+        r = rep[i1];
+        double d[3];
+        dvec64(xd, n, i1, i2, reps[r].box, reps[r].ibox, d);
+        const double rr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double dr = rr - r0[t];
+        const double rk = kb[t] * dr;
+        e = rk * dr;
+        const double force = (rr > 0.0) ? -2.0 * rk / rr : 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            atomic_fix(&fout[(size_t)c * n + i1], d[c] * force);
+            atomic_fix(&fout[(size_t)c * n + i2], -d[c] * force);
+        }
+    }
+    wave_add(to_fix(e), live, r, eacc, E_BOND);
+}
+
+// angle_harmonic, Source/mod_force_angle.f90:29-90: E = k (theta - theta0)^2
+__global__ void k_angle(int32_t items, const int32_t *__restrict__ al, const double *__restrict__ xd,
+                        const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
+                        const double *__restrict__ ka, const double *__restrict__ th0, int32_t n,
+                        long long *__restrict__ fout, long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < items;
+    double e = 0.0;
+    int r = 0;
+    if (live) {
+        const int i1 = al[(size_t)m * 4], i2 = al[(size_t)m * 4 + 1], i3 = al[(size_t)m * 4 + 2];
+        const int t = al[(size_t)m * 4 + 3] - 1;
+        r = rep[i1];
+        double d1[3], d2[3];
+        dvec64(xd, n, i1, i2, reps[r].box, reps[r].ibox, d1);
+        dvec64(xd, n, i3, i2, reps[r].box, reps[r].ibox, d2);
+        const double rsq1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+        const double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+        const double r1 = sqrt(rsq1), r2 = sqrt(rsq2);
+        double c = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) / (r1 * r2);
+        c = fmin(1.0, fmax(-1.0, c));
+        double s = sqrt(1.0 - c * c);
+        if (s < 0.001) s = 0.001;
+        s = 1.0 / s;
+        const double dth = acos(c) - th0[t];
+        const double tk = ka[t] * dth;
+        e = tk * dth;
+        const double a = 2.0 * tk * s;
+        const double a11 = a * c / rsq1, a12 = -a / (r1 * r2), a22 = a * c / rsq2;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double v1 = a11 * d1[k] + a12 * d2[k];
+            const double v2 = a22 * d2[k] + a12 * d1[k];
+            atomic_fix(&fout[(size_t)k * n + i1], -v1);
+            atomic_fix(&fout[(size_t)k * n + i2], v1 + v2);
+            atomic_fix(&fout[(size_t)k * n + i3], -v2);
+        }
+    }
+    wave_add(to_fix(e), live, r, eacc, E_ANGLE);
+}
+
+// Torsion E = K (1 + cos(n phi - shift)) (the AMBER form of Source/mod_force_dihed_amber.f90)
+// with the Blondel-Karplus gradient of phi.
+__global__ void k_torsion(int32_t items, const int32_t *__restrict__ tl, const double *__restrict__ xd,
+                          const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
+                          const double *__restrict__ kk, const double *__restrict__ mult,
+                          const double *__restrict__ shift_deg, int32_t n,
+                          long long *__restrict__ fout, long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < items;
+    double e = 0.0;
+    int r = 0;
+    if (live) {
+        const int a = tl[(size_t)m * 5], b = tl[(size_t)m * 5 + 1], c = tl[(size_t)m * 5 + 2];
+        const int d = tl[(size_t)m * 5 + 3], t = tl[(size_t)m * 5 + 4] - 1;
+        r = rep[a];
+        const double bx = reps[r].box, ib = reps[r].ibox;
+        double F[3], G[3], H[3];
+        dvec64(xd, n, a, b, bx, ib, F);
+        dvec64(xd, n, b, c, bx, ib, G);
+        dvec64(xd, n, d, c, bx, ib, H);
+        const double A[3] = {F[1] * G[2] - F[2] * G[1], F[2] * G[0] - F[0] * G[2], F[0] * G[1] - F[1] * G[0]};
+        const double B[3] = {H[1] * G[2] - H[2] * G[1], H[2] * G[0] - H[0] * G[2], H[0] * G[1] - H[1] * G[0]};
+        const double A2 = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+        const double B2 = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+        const double G2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+        if (A2 > 0.0 && B2 > 0.0 && G2 > 0.0) {
+            const double Gn = sqrt(G2);
+            const double BxA[3] = {B[1] * A[2] - B[2] * A[1], B[2] * A[0] - B[0] * A[2], B[0] * A[1] - B[1] * A[0]};
+            const double phi = atan2((BxA[0] * G[0] + BxA[1] * G[1] + BxA[2] * G[2]) / Gn,
+                                     A[0] * B[0] + A[1] * B[1] + A[2] * B[2]);
+            const double nn = mult[t], dl = shift_deg[t] * (3.141592653589793 / 180.0);
+            e = kk[t] * (1.0 + cos(nn * phi - dl));
+            const double dEdphi = -kk[t] * nn * sin(nn * phi - dl);
+            const double FG = F[0] * G[0] + F[1] * G[1] + F[2] * G[2];
+            const double HG = H[0] * G[0] + H[1] * G[1] + H[2] * G[2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double dpa = -Gn / A2 * A[k];
+                const double dpd = Gn / B2 * B[k];
+                const double dpb = -dpa - (FG / G2) * dpa - (HG / G2) * dpd;
+                const double dpc = -dpd + (FG / G2) * dpa + (HG / G2) * dpd;
+                atomic_fix(&fout[(size_t)k * n + a], -dEdphi * dpa);
+                atomic_fix(&fout[(size_t)k * n + b], -dEdphi * dpb);
+                atomic_fix(&fout[(size_t)k * n + c], -dEdphi * dpc);
+                atomic_fix(&fout[(size_t)k * n + d], -dEdphi * dpd);
+            }
+        }
+    }
+    wave_add(to_fix(e), live, r, eacc, E_DIHED);
+}
+
+}  // namespace
+
+void launch_correct(sofia_ctx *c, int which, long long *fout) {
+    const int32_t *lst = which == 12 ? c->bondlist : which == 13 ? c->anglelist : c->dihedlist;
+    const int items = which == 12 ? c->nb : which == 13 ? c->na : c->nd;
+    const int width = which == 12 ? 3 : which == 13 ? 4 : 5;
+    const int colb = which == 12 ? 1 : which == 13 ? 2 : 3;
+    const double fudge = which == 12 ? c->dp.fudge12 : which == 13 ? c->dp.fudge13 : c->dp.fudge14;
+    if (!items) return;
+    hipLaunchKernelGGL(k_correct, dim3(nblocks(items, 256)), dim3(256), 0, c->stream, items, width,
+                       colb, lst, c->a.pos4, c->a.q, c->rep, c->reps_d, c->dp, fudge, c->N, fout,
+                       c->eacc);
+}
+
+void launch_pair14(sofia_ctx *c, long long *fout) {
+    if (!c->nd) return;
+    hipLaunchKernelGGL(k_pair14<false>, dim3(nblocks(c->nd, 256)), dim3(256), 0, c->stream, c->nd,
+                       c->dihedlist, c->a.pos4, c->a.xd, c->a.q, c->rep, c->reps_d, c->lj14, c->dp,
+                       c->N, fout, c->eacc);
+}
+
+void launch_bonded_engine(sofia_ctx *c) {
+    const int n = c->N;
+    if (c->nd)
+        hipLaunchKernelGGL(k_pair14<true>, dim3(nblocks(c->nd, 256)), dim3(256), 0, c->stream, c->nd,
+                           c->dihedlist, c->a.pos4, c->a.xd, c->a.q, c->rep, c->reps_d, c->lj14,
+                           c->dp, n, c->fix, c->eacc);
+    if (c->nb && c->bond_k)
+        hipLaunchKernelGGL(k_bond, dim3(nblocks(c->nb, 256)), dim3(256), 0, c->stream, c->nb,
+                           c->bondlist, c->a.xd, c->rep, c->reps_d, c->bond_k, c->bond_r0, n,
+                           c->fix, c->eacc);
+    if (c->na && c->angle_k)
+        hipLaunchKernelGGL(k_angle, dim3(nblocks(c->na, 256)), dim3(256), 0, c->stream, c->na,
+                           c->anglelist, c->a.xd, c->rep, c->reps_d, c->angle_k, c->angle_th0, n,
+                           c->fix, c->eacc);
+    if (c->ntor && c->tors_k)
+        hipLaunchKernelGGL(k_torsion, dim3(nblocks(c->ntor, 256)), dim3(256), 0, c->stream, c->ntor,
+                           c->torsions, c->a.xd, c->rep, c->reps_d, c->tors_k, c->tors_n,
+                           c->tors_shift, n, c->fix, c->eacc);
+}
+
+}  // namespace sofia

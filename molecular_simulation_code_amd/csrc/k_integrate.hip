@@ -1,0 +1,563 @@
+// Integration, wrapping, Langevin and SHAKE kernels.
+//
+// Two flavours:
+//  * drop-in ("32"): the reference's real*4 state (ainfo positions, coords velocities and
+//    forces) with its FP64 temporaries -- nve_v/nve_x (Source/mod_ensemble.f90:11-87),
+//    pbc (Source/mod_pbc.f90:6-39), shake (Source/mod_shake.f90:8-913),
+//    langevin_debug_intel (Source/mod_langevin.f90:15-43);
+//  * engine ("64"): the same algorithms on FP64 positions and velocities, with forces read
+//    as FP64 nonbond sums + int64 fixed-point sums of every other term.
+#include "sofia_device.h"
+
+namespace sofia {
+namespace {
+
+__global__ void k_zero(int32_t n3, long long *__restrict__ f, int32_t ne, long long *__restrict__ e) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) f[i] = 0;
+    if (e && i < ne) e[i] = 0;
+}
+
+__global__ void k_fix_to_aos(int32_t n, const long long *__restrict__ f, float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = (float)((double)f[(size_t)c * n + i] * FIX_INV);
+}
+
+__global__ void k_f64_to_aos(int32_t n, const double *__restrict__ f, float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = (float)f[(size_t)c * n + i];
+}
+
+// ------------------------------------------------------------------ drop-in (real*4 state)
+__global__ void k_nve_v32(int32_t n, float dt_partial, const float4 *__restrict__ pos4,
+                          const double *__restrict__ mass, float *__restrict__ v,
+                          const float *__restrict__ ff, const int32_t *__restrict__ rep,
+                          double amu2e, long long *__restrict__ eacc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    double k = 0.0;
+    int r = 0;
+    if (live) {
+        r = rep[i];
+        const double m = mass[__float_as_int(pos4[i].w) - 1];
+        const double dtm = (double)dt_partial / m;
+        float vv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            vv[c] = (float)((double)v[3 * (size_t)i + c] + dtm * (double)ff[3 * (size_t)i + c]);
+            v[3 * (size_t)i + c] = vv[c];
+        }
+        k = 0.5 * amu2e * m * (double)((vv[0] * vv[0] + vv[1] * vv[1]) + vv[2] * vv[2]);
+    }
+    wave_add(to_fix(k), live, r, eacc, E_KE);
+}
+
+__global__ void k_nve_x32(int32_t n, float dt, const float *__restrict__ v, float4 *__restrict__ pos4,
+                          float *__restrict__ dr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos4[i];
+    const float dx = dt * v[3 * (size_t)i], dy = dt * v[3 * (size_t)i + 1], dz = dt * v[3 * (size_t)i + 2];
+    dr[i] += dx;
+    dr[(size_t)n + i] += dy;
+    dr[2 * (size_t)n + i] += dz;
+    p.x += dx;
+    p.y += dy;
+    p.z += dz;
+    pos4[i] = p;
+}
+
+// while-loop wrap into (0, box]; bounded so a non-finite coordinate cannot spin forever
+__device__ inline float wrap32(float x, float box) {
+    for (int it = 0; it < 64 && x > box; ++it) x = x - box;
+    for (int it = 0; it < 64 && (double)x <= 0.0; ++it) x = x + box;
+    return x;
+}
+
+__global__ void k_pbc32(int32_t n, float4 *__restrict__ pos4, const int32_t *__restrict__ rep,
+                        const RepInfo *__restrict__ reps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float box = reps[rep[i]].box;
+    float4 p = pos4[i];
+    p.x = wrap32(p.x, box);
+    p.y = wrap32(p.y, box);
+    p.z = wrap32(p.z, box);
+    pos4[i] = p;
+}
+
+__global__ void k_langevin32(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ v,
+                             const int32_t *__restrict__ gid, const double *__restrict__ mass,
+                             float fric, float gconst, uint64_t seed, uint64_t step,
+                             float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = mass[__float_as_int(pos4[i].w) - 1];
+    const double g1 = -(double)fric * m, g2 = (double)gconst * sqrt(m);
+    float u[3];
+    uniform3(seed, step, (uint32_t)gid[i], u);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        out[3 * (size_t)i + c] = (float)(g1 * (double)v[3 * (size_t)i + c] + g2 * (double)u[c]);
+}
+
+// ------------------------------------------------------------------ SHAKE
+// Constraint k joins atoms (P[k], Q[k]) of the cluster: shake2 one bond, shake3 (0,1),(0,2),
+// shake4 (0,1),(0,2),(0,3), shake3angle (0,1),(0,2),(1,2).  The Lagrange multipliers solve
+// |s_k + sum_l cpl[k][l] lambda_l r_l|^2 = d_k^2 by the reference's iteration: linear
+// system with the quadratic terms lagged, stopped when every |delta lambda| <= tol.
+struct ShakeCluster {
+    int nc;          // constraints
+    int P[3], Q[3];
+    int idx[4];
+    double im[4];
+    double bl[3];
+};
+
+__device__ inline bool shake_solve(const ShakeCluster &S, const double (*r)[3], const double (*s)[3],
+                                   double tol, int maxiter, double lam[3], int *iters) {
+    if (S.nc == 1) {   // shake2: closed-form root of the quadratic (Source/mod_shake.f90:133-168)
+        const double r01sq = r[0][0] * r[0][0] + r[0][1] * r[0][1] + r[0][2] * r[0][2];
+        const double s01sq = s[0][0] * s[0][0] + s[0][1] * s[0][1] + s[0][2] * s[0][2];
+        const double ims = S.im[0] + S.im[1];
+        const double a = ims * ims * r01sq;
+        const double b = 2.0 * ims * (s[0][0] * r[0][0] + s[0][1] * r[0][1] + s[0][2] * r[0][2]);
+        const double cc = s01sq - S.bl[0] * S.bl[0];
+        const double det = b * b - 4.0 * a * cc;
+        if (det < 0.0) return false;
+        const double l1 = (-b + sqrt(det)) / (2.0 * a), l2 = (-b - sqrt(det)) / (2.0 * a);
+        lam[0] = (fabs(l1) <= fabs(l2)) ? l1 : l2;
+        *iters = 1;
+        return true;
+    }
+    const int nc = S.nc;
+    double cpl[3][3], A[3][3], G[3][3], ssq[3];
+    for (int k = 0; k < nc; ++k) {
+        ssq[k] = s[k][0] * s[k][0] + s[k][1] * s[k][1] + s[k][2] * s[k][2];
+        for (int l = 0; l < nc; ++l) {
+            double c = 0.0;
+            if (S.P[k] == S.P[l]) c += S.im[S.P[k]];
+            if (S.P[k] == S.Q[l]) c -= S.im[S.P[k]];
+            if (S.Q[k] == S.P[l]) c -= S.im[S.Q[k]];
+            if (S.Q[k] == S.Q[l]) c += S.im[S.Q[k]];
+            cpl[k][l] = c;
+            A[k][l] = 2.0 * c * (s[k][0] * r[l][0] + s[k][1] * r[l][1] + s[k][2] * r[l][2]);
+            G[k][l] = r[k][0] * r[l][0] + r[k][1] * r[l][1] + r[k][2] * r[l][2];
+        }
+    }
+    double Ai[3][3];
+    if (nc == 2) {
+        const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+        if (det == 0.0) return false;
+        const double di = 1.0 / det;
+        Ai[0][0] = A[1][1] * di; Ai[0][1] = -A[0][1] * di;
+        Ai[1][0] = -A[1][0] * di; Ai[1][1] = A[0][0] * di;
+    } else {
+        const double det = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] +
+                           A[0][2] * A[1][0] * A[2][1] - A[0][0] * A[1][2] * A[2][1] -
+                           A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
+        if (det == 0.0) return false;
+        const double di = 1.0 / det;
+        Ai[0][0] = di * (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
+        Ai[0][1] = -di * (A[0][1] * A[2][2] - A[0][2] * A[2][1]);
+        Ai[0][2] = di * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+        Ai[1][0] = -di * (A[1][0] * A[2][2] - A[1][2] * A[2][0]);
+        Ai[1][1] = di * (A[0][0] * A[2][2] - A[0][2] * A[2][0]);
+        Ai[1][2] = -di * (A[0][0] * A[1][2] - A[0][2] * A[1][0]);
+        Ai[2][0] = di * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+        Ai[2][1] = -di * (A[0][0] * A[2][1] - A[0][1] * A[2][0]);
+        Ai[2][2] = di * (A[0][0] * A[1][1] - A[0][1] * A[1][0]);
+    }
+    lam[0] = lam[1] = lam[2] = 0.0;
+    int it = 0;
+    bool done = false;
+    while (!done && it < maxiter) {
+        double bb[3], nl[3];
+        for (int k = 0; k < nc; ++k) {
+            double qd = 0.0;
+            for (int l = 0; l < nc; ++l)
+                for (int m = 0; m < nc; ++m) qd += cpl[k][l] * cpl[k][m] * G[l][m] * lam[l] * lam[m];
+            bb[k] = S.bl[k] * S.bl[k] - ssq[k] - qd;
+        }
+        done = true;
+        for (int k = 0; k < nc; ++k) {
+            double v = 0.0;
+            for (int l = 0; l < nc; ++l) v += Ai[k][l] * bb[l];
+            nl[k] = v;
+        }
+        for (int k = 0; k < nc; ++k) {
+            if (fabs(nl[k] - lam[k]) > tol) done = false;
+            lam[k] = nl[k];
+        }
+        ++it;
+    }
+    *iters = it;
+    return true;
+}
+
+__device__ inline ShakeCluster make_cluster(int num, const int32_t *at, const double *bd,
+                                            const double *mass, const float4 *pos4) {
+    ShakeCluster S;
+    for (int k = 0; k < 4; ++k) {
+        S.idx[k] = at[k];
+        S.im[k] = (at[k] >= 0) ? 1.0 / mass[__float_as_int(pos4[at[k]].w) - 1] : 0.0;
+    }
+    for (int k = 0; k < 3; ++k) S.bl[k] = bd[k];
+    if (num == 1) {
+        S.nc = 1; S.P[0] = 0; S.Q[0] = 1;
+    } else if (num == 3) {
+        S.nc = 2; S.P[0] = 0; S.Q[0] = 1; S.P[1] = 0; S.Q[1] = 2;
+    } else if (num == 4) {
+        S.nc = 3; S.P[0] = 0; S.Q[0] = 1; S.P[1] = 0; S.Q[1] = 2; S.P[2] = 0; S.Q[2] = 3;
+    } else {   // -3: rigid triangle, constraints 0-1, 0-2, 1-2
+        S.nc = 3; S.P[0] = 0; S.Q[0] = 1; S.P[1] = 0; S.Q[1] = 2; S.P[2] = 1; S.Q[2] = 2;
+    }
+    return S;
+}
+
+// drop-in: real*4 positions/velocities/forces, FP64 solve (Source/mod_shake.f90:8-85, 888-913)
+__global__ void k_shake32(int32_t ns, const int32_t *__restrict__ num, const int32_t *__restrict__ atoms,
+                          const double *__restrict__ bond, const float4 *__restrict__ pos4,
+                          const float *__restrict__ v, float *__restrict__ ff,
+                          const double *__restrict__ mass, const int32_t *__restrict__ rep,
+                          const RepInfo *__restrict__ reps, float dt, double dtsq, double tol,
+                          int maxiter, int32_t *__restrict__ err, int32_t *__restrict__ iters_max,
+                          long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < ns;
+    double vir[3] = {0.0, 0.0, 0.0};
+    int rr = 0;
+    if (live) {
+        const ShakeCluster S = make_cluster(num[m], atoms + 4 * (size_t)m, bond + 3 * (size_t)m, mass, pos4);
+        rr = rep[S.idx[0]];
+        const float box = reps[rr].box, ibox = reps[rr].ibox;
+        double xs[4][3];
+        for (int a = 0; a < 4; ++a) {
+            if (S.idx[a] < 0) continue;
+            const int i = S.idx[a];
+            const float4 p = pos4[i];
+            const float pp[3] = {p.x, p.y, p.z};
+            for (int c = 0; c < 3; ++c)
+                xs[a][c] = (double)(pp[c] + dt * v[3 * (size_t)i + c]) +
+                           dtsq * S.im[a] * (double)ff[3 * (size_t)i + c];
+        }
+        double r[3][3], s[3][3];
+        for (int k = 0; k < S.nc; ++k) {
+            const float4 pa = pos4[S.idx[S.P[k]]], pb = pos4[S.idx[S.Q[k]]];
+            const float df[3] = {pa.x - pb.x, pa.y - pb.y, pa.z - pb.z};
+            for (int c = 0; c < 3; ++c) {
+                const double d = (double)df[c];
+                r[k][c] = d - (double)(box * (float)(int)round((double)ibox * d));
+                const double e = xs[S.P[k]][c] - xs[S.Q[k]][c];
+                s[k][c] = e - (double)(box * (float)(int)round((double)ibox * e));
+            }
+        }
+        double lam[3];
+        int its = 0;
+        if (!shake_solve(S, r, s, tol, maxiter, lam, &its)) {
+            atomicOr(err, 1);
+        } else {
+            atomicMax(iters_max, its);
+            for (int k = 0; k < S.nc; ++k) {
+                const double l = lam[k] / dtsq;
+                const int a = S.idx[S.P[k]], b = S.idx[S.Q[k]];
+                for (int c = 0; c < 3; ++c) {
+                    ff[3 * (size_t)a + c] = (float)((double)ff[3 * (size_t)a + c] + l * r[k][c]);
+                    ff[3 * (size_t)b + c] = (float)((double)ff[3 * (size_t)b + c] - l * r[k][c]);
+                    vir[c] += r[k][c] * r[k][c] * l;
+                }
+            }
+        }
+    }
+    wave_add(to_fix(vir[0]), live, rr, eacc, V_X);
+    wave_add(to_fix(vir[1]), live, rr, eacc, V_Y);
+    wave_add(to_fix(vir[2]), live, rr, eacc, V_Z);
+}
+
+// ------------------------------------------------------------------ engine (FP64 state)
+__global__ void k_from32(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ v32,
+                         double *__restrict__ xd, double *__restrict__ vd, double *__restrict__ drd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos4[i];
+    xd[i] = p.x; xd[(size_t)n + i] = p.y; xd[2 * (size_t)n + i] = p.z;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        vd[(size_t)c * n + i] = v32[3 * (size_t)i + c];
+        drd[(size_t)c * n + i] = 0.0;
+    }
+}
+
+__global__ void k_to32(int32_t n, const double *__restrict__ xd, const double *__restrict__ vd,
+                       float4 *__restrict__ pos4, float *__restrict__ v32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos4[i];
+    p.x = (float)xd[i]; p.y = (float)xd[(size_t)n + i]; p.z = (float)xd[2 * (size_t)n + i];
+    pos4[i] = p;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v32[3 * (size_t)i + c] = (float)vd[(size_t)c * n + i];
+}
+
+// v += (dt/2 kcal2amu / m) F with F = nonbond + fixed-point rest; optional kinetic energy
+__global__ void k_nve_v64(int32_t n, double dthalf, const float4 *__restrict__ pos4,
+                          const double *__restrict__ mass, double *__restrict__ vd,
+                          const double *__restrict__ fnb, const long long *__restrict__ fix,
+                          const int32_t *__restrict__ rep, double amu2e, int ke,
+                          long long *__restrict__ eacc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    double k = 0.0;
+    int r = 0;
+    if (live) {
+        r = rep[i];
+        const double m = mass[__float_as_int(pos4[i].w) - 1];
+        const double dtm = dthalf / m;
+        double v2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t o = (size_t)c * n + i;
+            const double f = fnb[o] + (double)fix[o] * FIX_INV;
+            const double vv = vd[o] + dtm * f;
+            vd[o] = vv;
+            v2 += vv * vv;
+        }
+        k = 0.5 * amu2e * m * v2;
+    }
+    if (ke) wave_add(to_fix(k), live, r, eacc, E_KE);
+}
+
+// x += dt v, displacement += dt v, wrap into (0, box], refresh the float4 copy
+__global__ void k_nve_x64(int32_t n, double dt, double *__restrict__ xd, const double *__restrict__ vd,
+                          double *__restrict__ drd, float4 *__restrict__ pos4,
+                          const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double box = reps[rep[i]].box;
+    float4 p = pos4[i];
+    float pf[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = (size_t)c * n + i;
+        const double d = dt * vd[o];
+        drd[o] += d;
+        double x = xd[o] + d;
+        for (int it = 0; it < 64 && x > box; ++it) x -= box;
+        for (int it = 0; it < 64 && x <= 0.0; ++it) x += box;
+        xd[o] = x;
+        pf[c] = (float)x;
+    }
+    p.x = pf[0]; p.y = pf[1]; p.z = pf[2];
+    pos4[i] = p;
+}
+
+// Langevin force -fric m v + gconst sqrt(m) U(-1/2,1/2) added to the fixed-point sum
+__global__ void k_langevin64(int32_t n, const float4 *__restrict__ pos4, const double *__restrict__ vd,
+                             const int32_t *__restrict__ gid, const int32_t *__restrict__ rep,
+                             const RepInfo *__restrict__ reps, const double *__restrict__ mass,
+                             const uint64_t *__restrict__ step, long long *__restrict__ fix) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RepInfo &R = reps[rep[i]];
+    if (R.fric == 0.0 && R.gconst == 0.0) return;
+    const double m = mass[__float_as_int(pos4[i].w) - 1];
+    float u[3];
+    uniform3(R.seed, *step, (uint32_t)gid[i], u);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = (size_t)c * n + i;
+        fix[o] += to_fix(-R.fric * m * vd[o] + R.gconst * sqrt(m) * (double)u[c]);
+    }
+}
+
+__global__ void k_shake64(int32_t ns, const int32_t *__restrict__ num, const int32_t *__restrict__ atoms,
+                          const double *__restrict__ bond, const float4 *__restrict__ pos4,
+                          const double *__restrict__ xd, const double *__restrict__ vd,
+                          const double *__restrict__ fnb, long long *__restrict__ fix,
+                          const double *__restrict__ mass, const int32_t *__restrict__ rep,
+                          const RepInfo *__restrict__ reps, int32_t n, double dt, double dtsq,
+                          double tol, int maxiter, int32_t *__restrict__ err,
+                          int32_t *__restrict__ iters_max, long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < ns;
+    double vir[3] = {0.0, 0.0, 0.0};
+    int rr = 0;
+    if (live) {
+        const ShakeCluster S = make_cluster(num[m], atoms + 4 * (size_t)m, bond + 3 * (size_t)m, mass, pos4);
+        rr = rep[S.idx[0]];
+        const double box = reps[rr].box, ibox = reps[rr].ibox;
+        double xs[4][3];
+        for (int a = 0; a < 4; ++a) {
+            if (S.idx[a] < 0) continue;
+            const int i = S.idx[a];
+            for (int c = 0; c < 3; ++c) {
+                const size_t o = (size_t)c * n + i;
+                const double f = fnb[o] + (double)fix[o] * FIX_INV;
+                xs[a][c] = xd[o] + dt * vd[o] + dtsq * S.im[a] * f;
+            }
+        }
+        double r[3][3], s[3][3];
+        for (int k = 0; k < S.nc; ++k) {
+            const int a = S.idx[S.P[k]], b = S.idx[S.Q[k]];
+            for (int c = 0; c < 3; ++c) {
+                const double d = xd[(size_t)c * n + a] - xd[(size_t)c * n + b];
+                r[k][c] = d - box * rint(d * ibox);
+                const double e = xs[S.P[k]][c] - xs[S.Q[k]][c];
+                s[k][c] = e - box * rint(e * ibox);
+            }
+        }
+        double lam[3];
+        int its = 0;
+        if (!shake_solve(S, r, s, tol, maxiter, lam, &its)) {
+            atomicOr(err, 1);
+        } else {
+            atomicMax(iters_max, its);
+            for (int k = 0; k < S.nc; ++k) {
+                const double l = lam[k] / dtsq;
+                const int a = S.idx[S.P[k]], b = S.idx[S.Q[k]];
+                for (int c = 0; c < 3; ++c) {
+                    fix[(size_t)c * n + a] += to_fix(l * r[k][c]);
+                    fix[(size_t)c * n + b] -= to_fix(l * r[k][c]);
+                    vir[c] += r[k][c] * r[k][c] * l;
+                }
+            }
+        }
+    }
+    wave_add(to_fix(vir[0]), live, rr, eacc, V_X);
+    wave_add(to_fix(vir[1]), live, rr, eacc, V_Y);
+    wave_add(to_fix(vir[2]), live, rr, eacc, V_Z);
+}
+
+// Capped steepest descent: every atom moves along its force by min(gamma |F|, max_disp).
+__global__ void k_minimize_step(int32_t n, double gamma, double max_disp, double *__restrict__ xd,
+                                double *__restrict__ drd, const double *__restrict__ fnb,
+                                const long long *__restrict__ fix, float4 *__restrict__ pos4,
+                                const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double box = reps[rep[i]].box;
+    double f[3], f2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = (size_t)c * n + i;
+        f[c] = fnb[o] + (double)fix[o] * FIX_INV;
+        f2 += f[c] * f[c];
+    }
+    const double fn = sqrt(f2);
+    const double step = (fn > 0.0) ? fmin(gamma * fn, max_disp) / fn : 0.0;
+    float4 p = pos4[i];
+    float pf[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = (size_t)c * n + i;
+        const double d = isfinite(f[c]) ? step * f[c] : 0.0;
+        drd[o] += d;
+        double x = xd[o] + d;
+        for (int it = 0; it < 64 && x > box; ++it) x -= box;
+        for (int it = 0; it < 64 && x <= 0.0; ++it) x += box;
+        xd[o] = x;
+        pf[c] = (float)x;
+    }
+    p.x = pf[0]; p.y = pf[1]; p.z = pf[2];
+    pos4[i] = p;
+}
+
+__global__ void k_step_advance(uint64_t *step) { *step += 1; }
+
+}  // namespace
+
+void launch_zero_forces(sofia_ctx *c, long long *f, bool energies) {
+    const int n3 = 3 * c->N, ne = c->nrep * NE;
+    hipLaunchKernelGGL(k_zero, dim3(nblocks(n3 > ne ? n3 : ne, 256)), dim3(256), 0, c->stream, n3, f,
+                       ne, energies ? c->eacc : nullptr);
+}
+
+void launch_fix_to_coords(sofia_ctx *c, const long long *f, float *out) {
+    hipLaunchKernelGGL(k_fix_to_aos, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, f, out);
+}
+
+void launch_f64_to_coords(sofia_ctx *c, const double *f, float *out) {
+    hipLaunchKernelGGL(k_f64_to_aos, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, f, out);
+}
+
+void launch_nve_v32(sofia_ctx *c, float dt_partial, const float *ff) {
+    hipLaunchKernelGGL(k_nve_v32, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, dt_partial,
+                       c->a.pos4, c->mass, c->a.v32, ff, c->rep, c->dp.amu2e, c->eacc);
+}
+
+void launch_nve_x32(sofia_ctx *c) {
+    hipLaunchKernelGGL(k_nve_x32, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, c->dp.dt,
+                       c->a.v32, c->a.pos4, c->dr32);
+}
+
+void launch_pbc32(sofia_ctx *c) {
+    hipLaunchKernelGGL(k_pbc32, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, c->a.pos4,
+                       c->rep, c->reps_d);
+}
+
+void launch_shake32(sofia_ctx *c, int iflag, float *ff, double *) {
+    if (!c->ns) return;
+    const float dt = c->dp.dt, k2a = c->dp.kcal2amu;
+    const double dtsq = (iflag == 0) ? (double)(((0.5f * dt) * dt) * k2a) : (double)((dt * dt) * k2a);
+    hipLaunchKernelGGL(k_shake32, dim3(nblocks(c->ns, 128)), dim3(128), 0, c->stream, c->ns,
+                       c->shake_num, c->shake_atoms, c->shake_bond, c->a.pos4, c->a.v32, ff, c->mass,
+                       c->rep, c->reps_d, dt, dtsq, c->dp.shake_tol, c->dp.shake_maxiter,
+                       c->shake_err, c->shake_iters, c->eacc);
+}
+
+void launch_langevin32(sofia_ctx *c, float fric, float gconst, uint64_t seed, uint64_t step, float *out) {
+    hipLaunchKernelGGL(k_langevin32, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, c->a.pos4,
+                       c->a.v32, c->a.gid, c->mass, fric, gconst, seed, step, out);
+}
+
+void launch_engine_from32(sofia_ctx *c) {
+    hipLaunchKernelGGL(k_from32, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, c->a.pos4,
+                       c->a.v32, c->a.xd, c->a.vd, c->drd);
+}
+
+void launch_engine_to32(sofia_ctx *c) {
+    hipLaunchKernelGGL(k_to32, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, c->a.xd,
+                       c->a.vd, c->a.pos4, c->a.v32);
+}
+
+void launch_nve_v64(sofia_ctx *c, bool ke) {
+    const double dthalf = 0.5 * (double)c->dp.dt * (double)c->dp.kcal2amu;
+    hipLaunchKernelGGL(k_nve_v64, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, dthalf,
+                       c->a.pos4, c->mass, c->a.vd, c->fnb, c->fix, c->rep, c->dp.amu2e, ke ? 1 : 0,
+                       c->eacc);
+}
+
+void launch_nve_x64(sofia_ctx *c) {
+    hipLaunchKernelGGL(k_nve_x64, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N,
+                       (double)c->dp.dt, c->a.xd, c->a.vd, c->drd, c->a.pos4, c->rep, c->reps_d);
+}
+
+void launch_langevin64(sofia_ctx *c) {
+    hipLaunchKernelGGL(k_langevin64, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, c->a.pos4,
+                       c->a.vd, c->a.gid, c->rep, c->reps_d, c->mass, c->step_d, c->fix);
+}
+
+void launch_shake64(sofia_ctx *c) {
+    if (!c->ns) return;
+    const double dt = c->dp.dt;
+    const double dtsq = dt * dt * (double)c->dp.kcal2amu;
+    hipLaunchKernelGGL(k_shake64, dim3(nblocks(c->ns, 128)), dim3(128), 0, c->stream, c->ns,
+                       c->shake_num, c->shake_atoms, c->shake_bond, c->a.pos4, c->a.xd, c->a.vd,
+                       c->fnb, c->fix, c->mass, c->rep, c->reps_d, c->N, dt, dtsq, c->dp.shake_tol,
+                       c->dp.shake_maxiter, c->shake_err, c->shake_iters, c->eacc);
+}
+
+void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp) {
+    hipLaunchKernelGGL(k_minimize_step, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, gamma,
+                       max_disp, c->a.xd, c->drd, c->fnb, c->fix, c->a.pos4, c->rep, c->reps_d);
+}
+
+void launch_step_advance(sofia_ctx *c) {
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, c->stream, c->step_d);
+}
+
+}  // namespace sofia

@@ -1,0 +1,246 @@
+// Nonbonded pair kernels over the full ("nonewton") Verlet list, gfx950.
+//
+//   mode 0  DSF, FP64 pair arithmetic        lj_cut_coul_dsf_nonewton_NOMIC/_MIC
+//                                            (Source/mod_force_nonbond.f90:105-196 / 8-101)
+//   mode 1  DSF, FP32 pair arithmetic with per-pair int64 fixed-point force sums
+//   mode 2  Ewald real space, FP32 pair math lj_cut_coul_long_nonewton_NOMIC (:300-400)
+//
+// Pair expressions (Source/mod_force_nonbond.f90:153-183):
+//   dr6i = (1/r^2)^3 (0 beyond rcut2);  F_LJ*r = dr6i (lj1 dr6i - lj2);  E_LJ = dr6i (lj3 dr6i - lj4)
+//   pref = C qi qj / r (0 beyond cut_coulsq);  erfc(alpha r) by Abramowitz & Stegun 7.1.26
+//   F_C*r = pref (erfc/r + 2 alpha/sqrt(pi) exp(-alpha^2 r^2) + r f_shift) r
+//   E_C = pref (erfc - r e_shift - r^2 f_shift);  f_i += dx (F_C + F_LJ)/r^2
+// Energies are halved (full list).  The virial follows the PME kernel's convention,
+// 0.5 sum dx^2 F/r (the reference DSF kernel computes none).
+//
+// Layout: L lanes cooperate on one atom i (L = 4..64, chosen by the autotuner); lane s
+// walks list entries s, s+L, ... so each group reads a contiguous run of the atom's list,
+// and the L partial sums are combined with cross-lane shuffles.  Positions are the
+// cell-sorted float4 (x, y, z, type) array, so the j gathers of neighbouring atoms hit
+// the same cache lines.  No atomics on forces: every atom's sum is owned by one group.
+//
+// Mode 1 turns each pair force into an integer number of 2^-24 kcal/mol/A (truncation
+// of the exact float product; a 64-bit conversion when |f| >= 128), so per-atom sums are
+// exact and independent of L, of list order and of how replicas are batched.
+#include "sofia_device.h"
+
+namespace sofia {
+namespace {
+
+constexpr int LJ_MAX = 1024;   // ntypes^2 entries held in LDS
+
+template <int L, int MODE, bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ q,
+          const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
+          const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh, int32_t na,
+          const float4 *__restrict__ ljtab, DevParams P, double *__restrict__ fnb,
+          float *__restrict__ ff32, long long *__restrict__ eacc) {
+    __shared__ float4 s_lj[LJ_MAX];
+    const int nt2 = P.ntypes * P.ntypes;
+    for (int t = threadIdx.x; t < nt2; t += blockDim.x) s_lj[t] = ljtab[t];
+    __syncthreads();
+
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / L;
+    const int sub = gt % L;
+    const bool live = i < n;
+    const int ii = live ? i : n - 1;
+
+    const float4 xi = pos4[ii];
+    const int ti = __float_as_int(xi.w) - 1;
+    const int r = rep[ii];
+    const float box = reps[r].box, ibox = reps[r].ibox;
+    const int nn = live ? numneigh[ii] : 0;
+    const int32_t *lst = nlist + (size_t)ii * na;
+    const float4 *ljrow = s_lj + ti * P.ntypes - 1;   // index by jtype (1-based)
+
+    double fxd = 0.0, fyd = 0.0, fzd = 0.0;           // modes 0, 2
+    long long fxi = 0, fyi = 0, fzi = 0;              // mode 1
+    double pot = 0.0, ecl = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+
+    if (MODE == 0) {
+        const double x1 = xi.x, y1 = xi.y, z1 = xi.z;
+        const double bx = box, ib = ibox;
+        const double qc = (double)P.coulpre * (double)q[ii];
+        const double two_a_pis = (2.0 * P.alpha) * P.my_pis_inv;
+        const double a2 = P.alpha * P.alpha, pa = P.ewald_p * P.alpha;
+        for (int k = sub; k < nn; k += L) {
+            const int j = lst[k];
+            const float4 pj = pos4[j];
+            double dx = x1 - (double)pj.x, dy = y1 - (double)pj.y, dz = z1 - (double)pj.z;
+            dx -= bx * __builtin_rint(dx * ib);
+            dy -= bx * __builtin_rint(dy * ib);
+            dz -= bx * __builtin_rint(dz * ib);
+            const double dr2 = dx * dx + dy * dy + dz * dz;
+            const double dri = rsq_f64(dr2);
+            const double r = dr2 * dri;
+            const double dr2i = dri * dri;
+            double dr6i = dr2i * dr2i * dr2i;
+            if (dr2 > P.rcut2) dr6i = 0.0;
+            const float4 lj = ljrow[__float_as_int(pj.w)];
+            const double flj = dr6i * ((double)lj.x * dr6i - (double)lj.y);
+            double pref = qc * (double)q[j] * dri;
+            if (dr2 > P.cut_coulsq) pref = 0.0;
+            const double erfcd = exp(-a2 * dr2);
+            const double t = rcp_f64(1.0 + pa * r);
+            const double erfcc =
+                t * (P.a1 + t * (P.a2 + t * (P.a3 + t * (P.a4 + t * P.a5)))) * erfcd;
+            const double fc = pref * (erfcc * dri + two_a_pis * erfcd + r * P.f_shift) * r;
+            const double force = (fc + flj) * dr2i;
+            fxd += dx * force;
+            fyd += dy * force;
+            fzd += dz * force;
+            if (ENERGY) {
+                pot += dr6i * (dr6i * (double)lj.z - (double)lj.w);
+                ecl += pref * (erfcc - r * P.e_shift - dr2 * P.f_shift);
+                vx += dx * dx * force;
+                vy += dy * dy * force;
+                vz += dz * dz * force;
+            }
+        }
+    } else {
+        const float rcut2 = (float)P.rcut2, cut_coulsq = (float)P.cut_coulsq;
+        const float qc = P.coulpre * q[ii];
+        const float alpha = (float)(MODE == 1 ? P.alpha : P.gewald);
+        const float nal2e = -alpha * alpha * 1.4426950408889634f;   // exp(-a^2 r^2) = 2^(..)
+        const float pa = (float)P.ewald_p * alpha;
+        const float c2 = (float)(MODE == 1 ? (2.0 * P.alpha) * P.my_pis_inv : P.ewald_f);
+        const float A1 = (float)P.a1, A2 = (float)P.a2, A3 = (float)P.a3, A4 = (float)P.a4,
+                    A5 = (float)P.a5;
+        const float fsh = (float)P.f_shift, esh = (float)P.e_shift;
+        for (int k = sub; k < nn; k += L) {
+            const int j = lst[k];
+            const float4 pj = pos4[j];
+            float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
+            dx -= box * __builtin_rintf(dx * ibox);
+            dy -= box * __builtin_rintf(dy * ibox);
+            dz -= box * __builtin_rintf(dz * ibox);
+            const float dr2 = dx * dx + dy * dy + dz * dz;
+            const float dri = __builtin_amdgcn_rsqf(dr2);
+            const float r = dr2 * dri;
+            const float dr2i = dri * dri;
+            float dr6i = dr2i * dr2i * dr2i;
+            if (dr2 > rcut2) dr6i = 0.0f;
+            const float4 lj = ljrow[__float_as_int(pj.w)];
+            const float flj = dr6i * (lj.x * dr6i - lj.y);
+            float pref = qc * q[j] * dri;
+            if (dr2 > cut_coulsq) pref = 0.0f;
+            const float erfcd = __builtin_amdgcn_exp2f(nal2e * dr2);
+            const float t = __builtin_amdgcn_rcpf(1.0f + pa * r);
+            const float erfcc = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * erfcd;
+            float fc, ec;
+            if (MODE == 1) {
+                fc = pref * (erfcc * dri + c2 * erfcd + r * fsh) * r;
+                ec = pref * (erfcc - r * esh - dr2 * fsh);
+            } else {
+                fc = pref * (erfcc + c2 * (alpha * r) * erfcd);
+                ec = pref * erfcc;
+            }
+            const float force = (fc + flj) * dr2i;
+            if (MODE == 1) {
+                const float fs = force * PAIR_QUANT;
+                const float px = dx * fs, py = dy * fs, pz = dz * fs;
+                const float big = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
+                if (__builtin_expect(__any(big >= 2147483520.0f), 0)) {
+                    fxi += (long long)(double)px;
+                    fyi += (long long)(double)py;
+                    fzi += (long long)(double)pz;
+                } else {
+                    fxi += (long long)(int)px;
+                    fyi += (long long)(int)py;
+                    fzi += (long long)(int)pz;
+                }
+            } else {
+                fxd += (double)(dx * force);
+                fyd += (double)(dy * force);
+                fzd += (double)(dz * force);
+            }
+            if (ENERGY) {
+                pot += (double)(dr6i * (dr6i * lj.z - lj.w));
+                ecl += (double)ec;
+                vx += (double)(dx * dx * force);
+                vy += (double)(dy * dy * force);
+                vz += (double)(dz * dz * force);
+            }
+        }
+    }
+
+    double Fx, Fy, Fz;
+    if (MODE == 1) {
+        fxi = group_sum<L>(fxi);
+        fyi = group_sum<L>(fyi);
+        fzi = group_sum<L>(fzi);
+        const double qinv = 1.0 / (double)PAIR_QUANT;
+        Fx = (double)fxi * qinv;
+        Fy = (double)fyi * qinv;
+        Fz = (double)fzi * qinv;
+    } else {
+        Fx = group_sum<L>(fxd);
+        Fy = group_sum<L>(fyd);
+        Fz = group_sum<L>(fzd);
+    }
+    const bool owner = live && sub == 0;
+    if (owner) {
+        fnb[i] = Fx;
+        fnb[(size_t)n + i] = Fy;
+        fnb[2 * (size_t)n + i] = Fz;
+        if (ff32) {
+            ff32[3 * (size_t)i + 0] = (float)Fx;
+            ff32[3 * (size_t)i + 1] = (float)Fy;
+            ff32[3 * (size_t)i + 2] = (float)Fz;
+        }
+    }
+    if (ENERGY) {
+        pot = group_sum<L>(pot);
+        ecl = group_sum<L>(ecl);
+        vx = group_sum<L>(vx);
+        vy = group_sum<L>(vy);
+        vz = group_sum<L>(vz);
+        wave_add(to_fix(0.5 * pot), owner, r, eacc, E_LJ);
+        wave_add(to_fix(0.5 * ecl), owner, r, eacc, E_COUL);
+        wave_add(to_fix(0.5 * vx), owner, r, eacc, V_X);
+        wave_add(to_fix(0.5 * vy), owner, r, eacc, V_Y);
+        wave_add(to_fix(0.5 * vz), owner, r, eacc, V_Z);
+    }
+}
+
+template <int L, int MODE>
+void launch_t(sofia_ctx *c, bool energies, float *ff32) {
+    const int block = c->block;
+    const long long threads = (long long)c->N * L;
+    const int grid = (int)((threads + block - 1) / block);
+    if (energies)
+        hipLaunchKernelGGL((k_nonbond<L, MODE, true>), dim3(grid), dim3(block), 0, c->stream, c->N,
+                           c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
+                           c->neigh_alloc, c->ljtab, c->dp, c->fnb, ff32, c->eacc);
+    else
+        hipLaunchKernelGGL((k_nonbond<L, MODE, false>), dim3(grid), dim3(block), 0, c->stream, c->N,
+                           c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
+                           c->neigh_alloc, c->ljtab, c->dp, c->fnb, ff32, c->eacc);
+}
+
+template <int MODE>
+void launch_mode(sofia_ctx *c, bool energies, float *ff32) {
+    switch (c->lanes) {
+        case 4: launch_t<4, MODE>(c, energies, ff32); break;
+        case 8: launch_t<8, MODE>(c, energies, ff32); break;
+        case 32: launch_t<32, MODE>(c, energies, ff32); break;
+        case 64: launch_t<64, MODE>(c, energies, ff32); break;
+        default: launch_t<16, MODE>(c, energies, ff32); break;
+    }
+}
+
+}  // namespace
+
+void launch_nonbond_out(sofia_ctx *c, int mode, bool energies, float *ff32) {
+    if (mode == 0) launch_mode<0>(c, energies, ff32);
+    else if (mode == 1) launch_mode<1>(c, energies, ff32);
+    else launch_mode<2>(c, energies, ff32);
+}
+
+void launch_nonbond(sofia_ctx *c, int mode, bool energies) {
+    launch_nonbond_out(c, mode, energies, nullptr);
+}
+
+}  // namespace sofia

@@ -1,0 +1,1033 @@
+// C ABI of libsofia_hip.so: context management, topology upload, the drop-in entry
+// points that mirror the reference subroutines, and the device-resident MD engine.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sofia_device.h"
+
+namespace sofia {
+
+void launch_nonbond_out(sofia_ctx *c, int mode, bool energies, float *ff32);
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char *what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? SOFIA_E_OOM : SOFIA_E_HIP;
+}
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return check_hip(e, "hipMalloc");
+    return SOFIA_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+#define TRY(x)                          \
+    do {                                \
+        int rc_ = (x);                  \
+        if (rc_ != SOFIA_OK) return rc_; \
+    } while (0)
+
+int alloc_atoms(sofia::AtomArrays &A, int n) {
+    TRY(dalloc(&A.pos4, n));
+    TRY(dalloc(&A.q, n));
+    TRY(dalloc(&A.gid, n));
+    TRY(dalloc(&A.num1, n));
+    TRY(dalloc(&A.num2, n));
+    TRY(dalloc(&A.num3, n));
+    TRY(dalloc(&A.spec, (size_t)n * SPEC));
+    TRY(dalloc(&A.v32, (size_t)3 * n));
+    TRY(dalloc(&A.xd, (size_t)3 * n));
+    TRY(dalloc(&A.vd, (size_t)3 * n));
+    TRY(dalloc(&A.mol, n));
+    SOFIA_HIP(hipMemset(A.mol, 0xff, sizeof(int32_t) * n));
+    SOFIA_HIP(hipMemset(A.spec, 0xff, sizeof(int32_t) * (size_t)n * SPEC));
+    SOFIA_HIP(hipMemset(A.num1, 0, sizeof(int32_t) * n));
+    SOFIA_HIP(hipMemset(A.num2, 0, sizeof(int32_t) * n));
+    SOFIA_HIP(hipMemset(A.num3, 0, sizeof(int32_t) * n));
+    SOFIA_HIP(hipMemset(A.v32, 0, sizeof(float) * 3 * (size_t)n));
+    return SOFIA_OK;
+}
+
+void free_atoms(sofia::AtomArrays &A) {
+    dfree(A.pos4); dfree(A.q); dfree(A.gid); dfree(A.num1); dfree(A.num2); dfree(A.num3);
+    dfree(A.spec); dfree(A.v32); dfree(A.xd); dfree(A.vd); dfree(A.mol);
+}
+
+// check_allocation's grid, Source/mod_neighbor.f90:1446-1451 (real*4), vol = box**3
+void cell_grid(float box, float rcut, float cut_coul, float *rn, int *ncellT, int *ncellD) {
+    const float rc = (cut_coul > rcut) ? cut_coul : rcut;
+    const int nn = (int)(box / rc);
+    const float r = box / (float)nn;
+    const float vol = (box * box) * box;
+    const float r3 = (r * r) * r;
+    *rn = r;
+    *ncellT = (int)std::lround((double)(vol / r3));
+    *ncellD = (int)std::lround(std::pow((double)*ncellT, 1.0 / 3.0));
+}
+
+// neighcell_nonewton, Source/mod_neighbor.f90:1309-1378, offset by the replica's first cell
+void stencil(int ncellT, int ncellD, int cell0, int32_t *cnum) {
+    int count = 0;
+    for (int ic = 0; ic < ncellT; ++ic) {
+        cnum[count] = cell0 + ic;
+        const int icz = ic / (ncellD * ncellD);
+        const int icy = (ic - icz * ncellD * ncellD) / ncellD;
+        const int icx = ic - icy * ncellD - icz * ncellD * ncellD;
+        for (int iz = -1; iz <= 1; ++iz)
+            for (int iy = -1; iy <= 1; ++iy)
+                for (int ix = -1; ix <= 1; ++ix) {
+                    int zc = icz + iz, yc = icy + iy, xc = icx + ix;
+                    if (zc < 0) zc += ncellD; else if (zc >= ncellD) zc -= ncellD;
+                    if (yc < 0) yc += ncellD; else if (yc >= ncellD) yc -= ncellD;
+                    if (xc < 0) xc += ncellD; else if (xc >= ncellD) xc -= ncellD;
+                    if (iz != 0 || iy != 0 || ix != 0) {
+                        ++count;
+                        cnum[count] = cell0 + xc + yc * ncellD + zc * ncellD * ncellD;
+                    }
+                }
+        ++count;
+    }
+}
+
+int grow_alloc(int mx) {   // Source/mod_neighbor.f90:260-263
+    int na = (int)((float)mx * 1.2f);
+    while (na % 16) ++na;
+    return na;
+}
+
+int ensure_list(sofia_ctx *c, int na) {
+    const size_t need = (size_t)c->N * (size_t)na;
+    if (need > c->nlist_cap) {
+        dfree(c->nlist);
+        TRY(dalloc(&c->nlist, need));
+        c->nlist_cap = need;
+    }
+    c->neigh_alloc = na;
+    c->dp.neigh_alloc = na;
+    return SOFIA_OK;
+}
+
+// Stage host data through a device scratch buffer.
+int upload(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return SOFIA_OK;
+    SOFIA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SOFIA_OK;
+}
+int download(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return SOFIA_OK;
+    SOFIA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SOFIA_OK;
+}
+
+std::vector<int32_t> atom_offsets(const sofia_ctx *c) {
+    std::vector<int32_t> off(c->nrep + 1, 0);
+    for (int r = 0; r < c->nrep; ++r) off[r + 1] = off[r] + c->reps_h[r].natom;
+    return off;
+}
+
+// 1-based per-replica ids in `items` rows of `width` ints (first `nidx` are atom ids, 0
+// = unused) -> global 0-based ids (-1 = unused), replica by replica.
+int to_global_list(const sofia_ctx *c, const int32_t *src, int items, int width, int nidx,
+                   const int32_t *per_rep, std::vector<int32_t> &out) {
+    out.assign((size_t)items * width, 0);
+    std::vector<int32_t> counts;
+    if (per_rep) counts.assign(per_rep, per_rep + c->nrep);
+    else if (c->nrep == 1) counts = {items};
+    else return fail(SOFIA_E_ARG, "per-replica item counts are required for a batch");
+    long long tot = 0;
+    for (int v : counts) tot += v;
+    if (tot != items) return fail(SOFIA_E_ARG, "per-replica item counts do not add up");
+    int m = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        const int a0 = c->reps_h[r].atom0, na = c->reps_h[r].natom;
+        for (int k = 0; k < counts[r]; ++k, ++m) {
+            for (int w = 0; w < width; ++w) {
+                const int v = src[(size_t)m * width + w];
+                if (w < nidx) {
+                    if (v == 0) { out[(size_t)m * width + w] = -1; continue; }
+                    if (v < 1 || v > na) return fail(SOFIA_E_ARG, "atom id out of range in a bonded list");
+                    out[(size_t)m * width + w] = a0 + v - 1;
+                } else {
+                    out[(size_t)m * width + w] = v;
+                }
+            }
+        }
+    }
+    return SOFIA_OK;
+}
+
+void fill_devparams(sofia_ctx *c) {
+    const sofia_params &p = c->p;
+    sofia::DevParams &d = c->dp;
+    d.coulpre = p.coulpre; d.dt = p.dt; d.kcal2amu = p.kcal2amu; d.ntypes = p.ntypes;
+    d.rcut2 = p.rcut2; d.cut_coulsq = p.cut_coulsq; d.rv2 = p.rv2; d.rvrc2 = p.rvrc2;
+    d.alpha = p.alpha; d.gewald = p.gewald; d.e_shift = p.e_shift; d.f_shift = p.f_shift;
+    d.ewald_p = p.ewald_p; d.ewald_f = p.ewald_f;
+    d.a1 = p.a1; d.a2 = p.a2; d.a3 = p.a3; d.a4 = p.a4; d.a5 = p.a5;
+    d.my_pis = p.my_pis; d.my_pis_inv = p.my_pis_inv; d.amu2e = p.amu2e;
+    d.fudge12 = p.fudge_12; d.fudge13 = p.fudge_13; d.fudge14 = p.fudge_14_coul;
+    d.shake_tol = p.shake_tol; d.shake_maxiter = p.shake_maxiter;
+    d.nrep = c->nrep; d.natom = c->N; d.neigh_alloc = c->neigh_alloc;
+}
+
+int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *box, sofia_ctx **out) {
+    if (!p || !natoms || !box || !out || nrep < 1) return fail(SOFIA_E_ARG, "null argument or nrep < 1");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SOFIA_E_NODEV, "no HIP device");
+    if (p->ntypes < 1 || p->ntypes * p->ntypes > 1024) return fail(SOFIA_E_ARG, "ntypes must be 1..32");
+    std::unique_ptr<sofia_ctx> c(new sofia_ctx());
+    c->p = *p;
+    c->nrep = nrep;
+    SOFIA_HIP(hipGetDevice(&c->device));
+    SOFIA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int atom0 = 0, cell0 = 0;
+    for (int r = 0; r < nrep; ++r) {
+        if (natoms[r] < 1) return fail(SOFIA_E_ARG, "replica with no atoms");
+        sofia::RepInfo R{};
+        R.box = box[r];
+        R.ibox = (float)(1.0 / (double)box[r]);      // ibox = 1.0d0/box, Source/get_started.f90:66
+        cell_grid(R.box, p->rcut, p->cut_coul, &R.rn, &R.ncellT, &R.ncellD);
+        if (R.ncellD < 3)
+            return fail(SOFIA_E_ARG, "box must hold at least 3 cells of max(rcut, cut_coul) per side");
+        if (R.ncellD * R.ncellD * R.ncellD != R.ncellT)
+            return fail(SOFIA_E_ARG, "cell grid is not cubic (ncellD^3 != ncellT)");
+        R.atom0 = atom0; R.natom = natoms[r];
+        R.cell0 = cell0;
+        atom0 += natoms[r];
+        cell0 += R.ncellT;
+        c->reps_h.push_back(R);
+    }
+    c->N = atom0;
+    c->ncell_total = cell0;
+    c->last_e.assign((size_t)nrep * 8, 0.0);
+    const int n = c->N;
+    TRY(alloc_atoms(c->a, n));
+    TRY(alloc_atoms(c->b, n));
+    TRY(dalloc(&c->rep, n));
+    TRY(dalloc(&c->dr32, (size_t)3 * n));
+    TRY(dalloc(&c->drd, (size_t)3 * n));
+    TRY(dalloc(&c->fnb, (size_t)3 * n));
+    TRY(dalloc(&c->fix, (size_t)3 * n));
+    TRY(dalloc(&c->fscratch, (size_t)3 * n));
+    TRY(dalloc(&c->stage32, (size_t)3 * n));
+    TRY(dalloc(&c->ccount, c->ncell_total));
+    TRY(dalloc(&c->cstart, c->ncell_total));
+    TRY(dalloc(&c->cnum, (size_t)27 * c->ncell_total));
+    TRY(dalloc(&c->cell_rep, c->ncell_total));
+    TRY(dalloc(&c->acell, n));
+    TRY(dalloc(&c->aslot, n));
+    TRY(dalloc(&c->perm, n));
+    TRY(dalloc(&c->tt, n));
+    TRY(dalloc(&c->atombin, n));
+    TRY(dalloc(&c->sorted_ids, n));
+    TRY(dalloc(&c->numneigh, n));
+    TRY(dalloc(&c->maxcnt, 1));
+    TRY(dalloc(&c->rebuild, nrep));
+    TRY(dalloc(&c->nbuilds, nrep));
+    TRY(dalloc(&c->shake_err, 1));
+    TRY(dalloc(&c->shake_iters, 1));
+    TRY(dalloc(&c->eacc, (size_t)nrep * NE));
+    TRY(dalloc(&c->step_d, 1));
+    TRY(dalloc(&c->reps_d, nrep));
+    SOFIA_HIP(hipMemset(c->dr32, 0, sizeof(float) * 3 * (size_t)n));
+    SOFIA_HIP(hipMemset(c->drd, 0, sizeof(double) * 3 * (size_t)n));
+    SOFIA_HIP(hipMemset(c->fix, 0, sizeof(long long) * 3 * (size_t)n));
+    SOFIA_HIP(hipMemset(c->fnb, 0, sizeof(double) * 3 * (size_t)n));
+    SOFIA_HIP(hipMemset(c->rebuild, 0, sizeof(int32_t) * nrep));
+    SOFIA_HIP(hipMemset(c->nbuilds, 0, sizeof(int32_t) * nrep));
+    SOFIA_HIP(hipMemset(c->numneigh, 0, sizeof(int32_t) * n));
+    SOFIA_HIP(hipMemset(c->step_d, 0, sizeof(uint64_t)));
+    std::vector<int32_t> rep(n), cnum((size_t)27 * c->ncell_total), crep(c->ncell_total);
+    for (int r = 0; r < nrep; ++r) {
+        const sofia::RepInfo &R = c->reps_h[r];
+        for (int k = 0; k < R.natom; ++k) rep[R.atom0 + k] = r;
+        stencil(R.ncellT, R.ncellD, R.cell0, cnum.data() + (size_t)27 * R.cell0);
+        for (int k = 0; k < R.ncellT; ++k) crep[R.cell0 + k] = r;
+    }
+    TRY(upload(c->rep, rep.data(), sizeof(int32_t) * n));
+    TRY(upload(c->cnum, cnum.data(), sizeof(int32_t) * cnum.size()));
+    TRY(upload(c->cell_rep, crep.data(), sizeof(int32_t) * crep.size()));
+    TRY(upload(c->reps_d, c->reps_h.data(), sizeof(sofia::RepInfo) * nrep));
+    // global ids default to 1..natom per replica
+    std::vector<int32_t> gid(n);
+    for (int r = 0; r < nrep; ++r)
+        for (int k = 0; k < c->reps_h[r].natom; ++k) gid[c->reps_h[r].atom0 + k] = k + 1;
+    TRY(upload(c->a.gid, gid.data(), sizeof(int32_t) * n));
+    TRY(ensure_list(c.get(), p->neigh_alloc0 > 0 ? p->neigh_alloc0 : 512));
+    fill_devparams(c.get());
+    *out = c.release();
+    return SOFIA_OK;
+}
+
+float *staging_f32(sofia_ctx *c) { return c->stage32; }
+
+int sync_check(sofia_ctx *c) {
+    SOFIA_HIP(hipStreamSynchronize(c->stream));
+    SOFIA_HIP(hipGetLastError());
+    return SOFIA_OK;
+}
+
+int read_energies(sofia_ctx *c, std::vector<long long> &e) {
+    e.resize((size_t)c->nrep * NE);
+    SOFIA_HIP(hipMemcpyAsync(e.data(), c->eacc, sizeof(long long) * e.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    return sync_check(c);
+}
+
+double E(const std::vector<long long> &e, int r, int slot) { return (double)e[(size_t)r * NE + slot] * FIX_INV; }
+
+// Full sort + list build of every flagged replica; grow-and-redo on overflow.
+int rebuild_now(sofia_ctx *c, bool engine, bool reference_growth) {
+    launch_cell_reset(c, true);
+    launch_bin(c);
+    launch_cell_scan(c);
+    launch_scatter_sort(c);
+    launch_gather(c, engine);
+    launch_remap_topology(c);
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        SOFIA_HIP(hipMemsetAsync(c->maxcnt, 0, sizeof(int32_t), c->stream));
+        launch_build_list(c);
+        int32_t mx = 0;
+        SOFIA_HIP(hipMemcpyAsync(&mx, c->maxcnt, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        TRY(sync_check(c));
+        if (mx <= c->neigh_alloc) {
+            launch_finish_rebuild(c, engine);
+            c->have_list = true;
+            return sync_check(c);
+        }
+        int na = reference_growth ? grow_alloc(mx) : ((mx * 3 / 2 + 15) / 16) * 16;
+        TRY(ensure_list(c, na));
+    }
+    return fail(SOFIA_E_OVERFLOW, "Verlet list kept overflowing");
+}
+
+}  // namespace
+}  // namespace sofia
+
+using namespace sofia;
+
+extern "C" {
+
+int sofia_last_error(char *buf, size_t len) {
+    if (!buf || !len) return SOFIA_E_ARG;
+    std::snprintf(buf, len, "%s", g_last_error.c_str());
+    return SOFIA_OK;
+}
+
+int sofia_device_count(int *count) {
+    if (!count) return fail(SOFIA_E_ARG, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return SOFIA_OK;
+}
+
+int sofia_ctx_create(const sofia_params *p, int32_t np, float box, sofia_ctx **out) {
+    return create(p, 1, &np, &box, out);
+}
+
+int sofia_ctx_create_batch(const sofia_params *p, int32_t nrep, const int32_t *natoms,
+                           const float *box, sofia_ctx **out) {
+    return create(p, nrep, natoms, box, out);
+}
+
+int sofia_ctx_destroy(sofia_ctx *c) {
+    if (!c) return SOFIA_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graph_raw) (void)hipGraphDestroy(c->graph_raw);
+    free_atoms(c->a);
+    free_atoms(c->b);
+    dfree(c->rep); dfree(c->dr32); dfree(c->drd); dfree(c->fnb); dfree(c->fix); dfree(c->fscratch); dfree(c->stage32);
+    dfree(c->mol_rep); dfree(c->uf_parent); dfree(c->uf_size); dfree(c->uf_largest);
+    dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep); dfree(c->acell);
+    dfree(c->aslot); dfree(c->perm); dfree(c->tt); dfree(c->atombin); dfree(c->sorted_ids);
+    dfree(c->nlist); dfree(c->numneigh); dfree(c->maxcnt); dfree(c->rebuild); dfree(c->nbuilds);
+    dfree(c->shake_err); dfree(c->shake_iters); dfree(c->bondlist); dfree(c->anglelist);
+    dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms);
+    dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
+    dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
+    dfree(c->tors_shift); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SOFIA_OK;
+}
+
+int sofia_synchronize(sofia_ctx *c) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    return sync_check(c);
+}
+
+int sofia_set_forcefield(sofia_ctx *c, const float *lj1, const float *lj2, const float *lj3,
+                         const float *lj4, const double *l141, const double *l142,
+                         const double *l143, const double *l144, const double *mass) {
+    if (!c || !lj1 || !lj2 || !lj3 || !lj4 || !l141 || !l142 || !l143 || !l144 || !mass)
+        return fail(SOFIA_E_ARG, "null argument");
+    const int nt2 = c->p.ntypes * c->p.ntypes;
+    std::vector<float4> lj(nt2);
+    std::vector<double4> l14(nt2);
+    for (int k = 0; k < nt2; ++k) {
+        lj[k] = make_float4(lj1[k], lj2[k], lj3[k], lj4[k]);
+        l14[k] = make_double4(l141[k], l142[k], l143[k], l144[k]);
+    }
+    if (!c->ljtab) {
+        TRY(dalloc(&c->ljtab, nt2));
+        TRY(dalloc(&c->lj14, nt2));
+        TRY(dalloc(&c->mass, c->p.ntypes));
+    }
+    TRY(upload(c->ljtab, lj.data(), sizeof(float4) * nt2));
+    TRY(upload(c->lj14, l14.data(), sizeof(double4) * nt2));
+    TRY(upload(c->mass, mass, sizeof(double) * c->p.ntypes));
+    c->have_ff = true;
+    return SOFIA_OK;
+}
+
+int sofia_set_exclusions(sofia_ctx *c, const int32_t *n1, const int32_t *n2, const int32_t *n3,
+                         const int32_t *spec) {
+    if (!c || !n1 || !n2 || !n3 || !spec) return fail(SOFIA_E_ARG, "null argument");
+    const int n = c->N;
+    std::vector<int32_t> g((size_t)n * SPEC, -1);
+    for (int r = 0; r < c->nrep; ++r) {
+        const auto &R = c->reps_h[r];
+        for (int k = 0; k < R.natom; ++k) {
+            const int i = R.atom0 + k;
+            if (n3[i] < 0 || n3[i] > SPEC || n1[i] > n2[i] || n2[i] > n3[i])
+                return fail(SOFIA_E_ARG, "bad num1bond/num2bond/num3bond");
+            for (int l = 0; l < n3[i]; ++l) {
+                const int v = spec[(size_t)i * SPEC + l];
+                if (v < 1 || v > R.natom) return fail(SOFIA_E_ARG, "specbond id out of range");
+                g[(size_t)i * SPEC + l] = R.atom0 + v - 1;
+            }
+        }
+    }
+    TRY(upload(c->a.num1, n1, sizeof(int32_t) * n));
+    TRY(upload(c->a.num2, n2, sizeof(int32_t) * n));
+    TRY(upload(c->a.num3, n3, sizeof(int32_t) * n));
+    TRY(upload(c->a.spec, g.data(), sizeof(int32_t) * g.size()));
+    return SOFIA_OK;
+}
+
+int sofia_set_bonded(sofia_ctx *c, const int32_t *bl, int32_t nb, const int32_t *nb_rep,
+                     const int32_t *al, int32_t na, const int32_t *na_rep, const int32_t *dl,
+                     int32_t nd, const int32_t *nd_rep) {
+    if (!c || nb < 0 || na < 0 || nd < 0) return fail(SOFIA_E_ARG, "bad argument");
+    std::vector<int32_t> g;
+    dfree(c->bondlist); dfree(c->anglelist); dfree(c->dihedlist);
+    c->nb = c->na = c->nd = 0;
+    if (nb) {
+        TRY(to_global_list(c, bl, nb, 3, 2, nb_rep, g));
+        TRY(dalloc(&c->bondlist, g.size()));
+        TRY(upload(c->bondlist, g.data(), sizeof(int32_t) * g.size()));
+        c->nb = nb;
+    }
+    if (na) {
+        TRY(to_global_list(c, al, na, 4, 3, na_rep, g));
+        TRY(dalloc(&c->anglelist, g.size()));
+        TRY(upload(c->anglelist, g.data(), sizeof(int32_t) * g.size()));
+        c->na = na;
+    }
+    if (nd) {
+        TRY(to_global_list(c, dl, nd, 5, 4, nd_rep, g));
+        TRY(dalloc(&c->dihedlist, g.size()));
+        TRY(upload(c->dihedlist, g.data(), sizeof(int32_t) * g.size()));
+        c->nd = nd;
+    }
+    return SOFIA_OK;
+}
+
+int sofia_set_bonded_coeffs(sofia_ctx *c, int32_t nbt, const double *bk, const double *br0,
+                            int32_t nat, const double *ak, const double *ath, int32_t ntt,
+                            const double *tk, const double *tn, const double *ts,
+                            const int32_t *tors, int32_t ntor, const int32_t *ntor_rep) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    dfree(c->bond_k); dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0);
+    dfree(c->tors_k); dfree(c->tors_n); dfree(c->tors_shift); dfree(c->torsions);
+    c->ntor = 0;
+    if (nbt > 0) {
+        TRY(dalloc(&c->bond_k, nbt)); TRY(dalloc(&c->bond_r0, nbt));
+        TRY(upload(c->bond_k, bk, sizeof(double) * nbt)); TRY(upload(c->bond_r0, br0, sizeof(double) * nbt));
+    }
+    if (nat > 0) {
+        TRY(dalloc(&c->angle_k, nat)); TRY(dalloc(&c->angle_th0, nat));
+        TRY(upload(c->angle_k, ak, sizeof(double) * nat)); TRY(upload(c->angle_th0, ath, sizeof(double) * nat));
+    }
+    if (ntt > 0 && ntor > 0) {
+        TRY(dalloc(&c->tors_k, ntt)); TRY(dalloc(&c->tors_n, ntt)); TRY(dalloc(&c->tors_shift, ntt));
+        TRY(upload(c->tors_k, tk, sizeof(double) * ntt)); TRY(upload(c->tors_n, tn, sizeof(double) * ntt));
+        TRY(upload(c->tors_shift, ts, sizeof(double) * ntt));
+        std::vector<int32_t> g;
+        TRY(to_global_list(c, tors, ntor, 5, 4, ntor_rep, g));
+        TRY(dalloc(&c->torsions, g.size()));
+        TRY(upload(c->torsions, g.data(), sizeof(int32_t) * g.size()));
+        c->ntor = ntor;
+    }
+    c->nbt = nbt; c->nat = nat; c->ntt = ntt;
+    return SOFIA_OK;
+}
+
+int sofia_set_shake(sofia_ctx *c, int32_t ns, const int32_t *ns_rep, const int32_t *num,
+                    const int32_t *atoms, const double *bond) {
+    if (!c || ns < 0) return fail(SOFIA_E_ARG, "bad argument");
+    dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_bond);
+    c->ns = 0;
+    if (!ns) return SOFIA_OK;
+    for (int k = 0; k < ns; ++k)
+        if (num[k] != 1 && num[k] != 3 && num[k] != 4 && num[k] != -3)
+            return fail(SOFIA_E_ARG, "SHAKE cluster type must be 1, 3, 4 or -3");
+    std::vector<int32_t> g;
+    TRY(to_global_list(c, atoms, ns, 4, 4, ns_rep, g));
+    for (int k = 0; k < ns; ++k) {
+        const int need = num[k] == 1 ? 2 : (num[k] == 4 ? 4 : 3);
+        for (int a = 0; a < need; ++a)
+            if (g[(size_t)k * 4 + a] < 0) return fail(SOFIA_E_ARG, "SHAKE cluster is missing an atom");
+    }
+    TRY(dalloc(&c->shake_num, ns));
+    TRY(dalloc(&c->shake_atoms, g.size()));
+    TRY(dalloc(&c->shake_bond, (size_t)3 * ns));
+    TRY(upload(c->shake_num, num, sizeof(int32_t) * ns));
+    TRY(upload(c->shake_atoms, g.data(), sizeof(int32_t) * g.size()));
+    TRY(upload(c->shake_bond, bond, sizeof(double) * 3 * ns));
+    c->ns = ns;
+    return SOFIA_OK;
+}
+
+int sofia_put_state(sofia_ctx *c, const sofia_ainfo *pos, const float *q, const sofia_coords *v,
+                    const int32_t *gid) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    const int n = c->N;
+    if (pos) {
+        for (int i = 0; i < n; ++i)
+            if (pos[i].type < 1 || pos[i].type > c->p.ntypes) return fail(SOFIA_E_ARG, "atom type out of range");
+        TRY(upload(c->a.pos4, pos, sizeof(float4) * n));
+    }
+    if (q) TRY(upload(c->a.q, q, sizeof(float) * n));
+    if (v) TRY(upload(c->a.v32, v, sizeof(float) * 3 * (size_t)n));
+    if (gid) TRY(upload(c->a.gid, gid, sizeof(int32_t) * n));
+    return SOFIA_OK;
+}
+
+int sofia_get_state(sofia_ctx *c, sofia_ainfo *pos, sofia_coords *v, int32_t *gid) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    TRY(sync_check(c));
+    const int n = c->N;
+    if (pos) TRY(download(pos, c->a.pos4, sizeof(float4) * n));
+    if (v) TRY(download(v, c->a.v32, sizeof(float) * 3 * (size_t)n));
+    if (gid) TRY(download(gid, c->a.gid, sizeof(int32_t) * n));
+    return SOFIA_OK;
+}
+
+int sofia_get_state_f64(sofia_ctx *c, double *x, double *v) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    TRY(sync_check(c));
+    const int n = c->N;
+    std::vector<double> t((size_t)3 * n);
+    for (int w = 0; w < 2; ++w) {
+        double *dst = w ? v : x;
+        if (!dst) continue;
+        TRY(download(t.data(), w ? c->a.vd : c->a.xd, sizeof(double) * t.size()));
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) dst[3 * (size_t)i + d] = t[(size_t)d * n + i];
+    }
+    return SOFIA_OK;
+}
+
+int sofia_put_state_f64(sofia_ctx *c, const double *x, const double *v) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    const int n = c->N;
+    std::vector<double> t((size_t)3 * n);
+    for (int w = 0; w < 2; ++w) {
+        const double *src = w ? v : x;
+        if (!src) continue;
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) t[(size_t)d * n + i] = src[3 * (size_t)i + d];
+        TRY(upload(w ? c->a.vd : c->a.xd, t.data(), sizeof(double) * t.size()));
+    }
+    return SOFIA_OK;
+}
+
+int sofia_get_displacement(sofia_ctx *c, float *dx, float *dy, float *dz) {
+    if (!c || !dx || !dy || !dz) return fail(SOFIA_E_ARG, "null argument");
+    TRY(sync_check(c));
+    const int n = c->N;
+    TRY(download(dx, c->dr32, sizeof(float) * n));
+    TRY(download(dy, c->dr32 + n, sizeof(float) * n));
+    TRY(download(dz, c->dr32 + 2 * (size_t)n, sizeof(float) * n));
+    return SOFIA_OK;
+}
+
+int sofia_put_displacement(sofia_ctx *c, const float *dx, const float *dy, const float *dz) {
+    if (!c || !dx || !dy || !dz) return fail(SOFIA_E_ARG, "null argument");
+    const int n = c->N;
+    TRY(upload(c->dr32, dx, sizeof(float) * n));
+    TRY(upload(c->dr32 + n, dy, sizeof(float) * n));
+    TRY(upload(c->dr32 + 2 * (size_t)n, dz, sizeof(float) * n));
+    return SOFIA_OK;
+}
+
+// ------------------------------------------------------------------ drop-ins
+int sofia_neighbor_wrapper(sofia_ctx *c, int32_t, int32_t *tt) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    launch_set_rebuild(c, 1);
+    TRY(rebuild_now(c, c->engine_ready, true));
+    if (tt) {
+        std::vector<int32_t> t(c->N);
+        TRY(download(t.data(), c->tt, sizeof(int32_t) * c->N));
+        for (int r = 0; r < c->nrep; ++r) {
+            const auto &R = c->reps_h[r];
+            for (int k = 0; k < R.natom; ++k) tt[R.atom0 + k] = t[R.atom0 + k] - R.atom0 + 1;
+        }
+    }
+    return SOFIA_OK;
+}
+
+int sofia_get_cells(sofia_ctx *c, int32_t *start, int32_t *endposit, int32_t *atombin,
+                    int32_t *ncellT, int32_t *ncellD, float *rn) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no list built yet");
+    TRY(sync_check(c));
+    std::vector<int32_t> cs(c->ncell_total), cc(c->ncell_total), ab(c->N);
+    TRY(download(cs.data(), c->cstart, sizeof(int32_t) * cs.size()));
+    TRY(download(cc.data(), c->ccount, sizeof(int32_t) * cc.size()));
+    TRY(download(ab.data(), c->atombin, sizeof(int32_t) * ab.size()));
+    for (int r = 0; r < c->nrep; ++r) {
+        const auto &R = c->reps_h[r];
+        for (int k = 0; k < R.ncellT; ++k) {
+            const int g = R.cell0 + k;
+            if (start) start[g] = cs[g] - R.atom0 + 1;
+            if (endposit) endposit[g] = cs[g] - R.atom0 + cc[g];
+        }
+        if (atombin)
+            for (int k = 0; k < R.natom; ++k) atombin[R.atom0 + k] = ab[R.atom0 + k] - R.cell0;
+    }
+    if (ncellT) *ncellT = c->reps_h[0].ncellT;
+    if (ncellD) *ncellD = c->reps_h[0].ncellD;
+    if (rn) *rn = c->reps_h[0].rn;
+    return SOFIA_OK;
+}
+
+int sofia_get_nlist(sofia_ctx *c, int32_t *nlist, int32_t *numneigh, int32_t *na) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no list built yet");
+    TRY(sync_check(c));
+    if (na) *na = c->neigh_alloc;
+    std::vector<int32_t> nn(c->N);
+    TRY(download(nn.data(), c->numneigh, sizeof(int32_t) * c->N));
+    if (numneigh) std::memcpy(numneigh, nn.data(), sizeof(int32_t) * c->N);
+    if (nlist) {
+        const size_t total = (size_t)c->N * c->neigh_alloc;
+        TRY(download(nlist, c->nlist, sizeof(int32_t) * total));
+        for (int r = 0; r < c->nrep; ++r) {
+            const auto &R = c->reps_h[r];
+            for (int k = 0; k < R.natom; ++k) {
+                const int i = R.atom0 + k;
+                int32_t *row = nlist + (size_t)i * c->neigh_alloc;
+                for (int m = 0; m < nn[i]; ++m) row[m] = row[m] - R.atom0 + 1;
+                for (int m = nn[i]; m < c->neigh_alloc; ++m) row[m] = 0;
+            }
+        }
+    }
+    return SOFIA_OK;
+}
+
+int sofia_check_neighbor(sofia_ctx *c, int32_t *nflag) {
+    if (!c || !nflag) return fail(SOFIA_E_ARG, "null argument");
+    launch_set_rebuild(c, 0);
+    launch_check_neighbor(c, false);
+    std::vector<int32_t> f(c->nrep);
+    SOFIA_HIP(hipMemcpyAsync(f.data(), c->rebuild, sizeof(int32_t) * c->nrep, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_check(c));
+    *nflag = 0;
+    for (int v : f) *nflag |= v;
+    launch_set_rebuild(c, 0);
+    return sync_check(c);
+}
+
+static int nonbond_dropin(sofia_ctx *c, int mode, sofia_coords *ff, double *ff64, double *pot,
+                          double *ecoul, double *vir) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no Verlet list: call sofia_neighbor_wrapper first");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    launch_zero_forces(c, c->fscratch, true);
+    launch_nonbond_out(c, mode, true, staging_f32(c));
+    std::vector<long long> e;
+    TRY(read_energies(c, e));
+    const int n = c->N;
+    if (ff) TRY(download(ff, staging_f32(c), sizeof(float) * 3 * (size_t)n));
+    if (ff64) {
+        std::vector<double> t((size_t)3 * n);
+        TRY(download(t.data(), c->fnb, sizeof(double) * t.size()));
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) ff64[3 * (size_t)i + d] = t[(size_t)d * n + i];
+    }
+    double sp = 0, se = 0, vx = 0, vy = 0, vz = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        sp += E(e, r, E_LJ); se += E(e, r, E_COUL);
+        vx += E(e, r, V_X); vy += E(e, r, V_Y); vz += E(e, r, V_Z);
+    }
+    if (pot) *pot += sp;
+    if (ecoul) *ecoul += se;
+    if (vir) { vir[0] = vx; vir[1] = vy; vir[2] = vz; }
+    return SOFIA_OK;
+}
+
+int sofia_lj_cut_coul_dsf(sofia_ctx *c, sofia_coords *ff, double *ff64, double *pot, double *ec,
+                          double *vir) {
+    return nonbond_dropin(c, 0, ff, ff64, pot, ec, vir);
+}
+
+int sofia_lj_cut_coul_dsf_mixed(sofia_ctx *c, sofia_coords *ff, double *ff64, double *pot,
+                                double *ec, double *vir) {
+    return nonbond_dropin(c, 1, ff, ff64, pot, ec, vir);
+}
+
+int sofia_lj_cut_coul_long(sofia_ctx *c, sofia_coords *ff, double *pot, double *ec, double *vir) {
+    return nonbond_dropin(c, 2, ff, nullptr, pot, ec, vir);
+}
+
+int sofia_correct_bonds(sofia_ctx *c, int32_t which, sofia_coords *ff, double *ec, double *vir) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (which != 12 && which != 13 && which != 14) return fail(SOFIA_E_ARG, "which must be 12, 13 or 14");
+    launch_zero_forces(c, c->fscratch, true);
+    launch_correct(c, which, c->fscratch);
+    float *st = staging_f32(c);
+    launch_fix_to_coords(c, c->fscratch, st);
+    std::vector<long long> e;
+    TRY(read_energies(c, e));
+    if (ff) TRY(download(ff, st, sizeof(float) * 3 * (size_t)c->N));
+    double s = 0, vx = 0, vy = 0, vz = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        s += E(e, r, E_CORR); vx += E(e, r, V_X); vy += E(e, r, V_Y); vz += E(e, r, V_Z);
+    }
+    if (ec) *ec += s;
+    if (vir) { vir[0] += vx; vir[1] += vy; vir[2] += vz; }
+    return SOFIA_OK;
+}
+
+int sofia_pair14_dsf(sofia_ctx *c, sofia_coords *ffh, double *ec14, double *pot14) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    launch_zero_forces(c, c->fscratch, true);
+    launch_pair14(c, c->fscratch);
+    float *st = staging_f32(c);
+    launch_fix_to_coords(c, c->fscratch, st);
+    std::vector<long long> e;
+    TRY(read_energies(c, e));
+    if (ffh) TRY(download(ffh, st, sizeof(float) * 3 * (size_t)c->N));
+    double a = 0, b = 0;
+    for (int r = 0; r < c->nrep; ++r) { a += E(e, r, E_COUL14); b += E(e, r, E_POT14); }
+    if (ec14) *ec14 += a;
+    if (pot14) *pot14 += b;
+    return SOFIA_OK;
+}
+
+int sofia_nve_v(sofia_ctx *c, float dt_partial, const sofia_coords *ff, double *ke) {
+    if (!c || !ff) return fail(SOFIA_E_ARG, "null argument");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    float *st = staging_f32(c);
+    TRY(upload(st, ff, sizeof(float) * 3 * (size_t)c->N));
+    SOFIA_HIP(hipMemsetAsync(c->eacc, 0, sizeof(long long) * c->nrep * NE, c->stream));
+    launch_nve_v32(c, dt_partial, st);
+    std::vector<long long> e;
+    TRY(read_energies(c, e));
+    if (ke) {
+        double s = 0;
+        for (int r = 0; r < c->nrep; ++r) s += E(e, r, E_KE);
+        *ke = s;
+    }
+    return SOFIA_OK;
+}
+
+int sofia_nve_x(sofia_ctx *c) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    launch_nve_x32(c);
+    return sync_check(c);
+}
+
+int sofia_pbc(sofia_ctx *c) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    launch_pbc32(c);
+    return sync_check(c);
+}
+
+int sofia_shake(sofia_ctx *c, int32_t iflag, sofia_coords *ff, double *vir, int32_t *max_iter) {
+    if (!c || !ff) return fail(SOFIA_E_ARG, "null argument");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    float *st = staging_f32(c);
+    TRY(upload(st, ff, sizeof(float) * 3 * (size_t)c->N));
+    SOFIA_HIP(hipMemsetAsync(c->eacc, 0, sizeof(long long) * c->nrep * NE, c->stream));
+    SOFIA_HIP(hipMemsetAsync(c->shake_err, 0, sizeof(int32_t), c->stream));
+    SOFIA_HIP(hipMemsetAsync(c->shake_iters, 0, sizeof(int32_t), c->stream));
+    launch_shake32(c, iflag, st, nullptr);
+    std::vector<long long> e;
+    TRY(read_energies(c, e));
+    int32_t err = 0, its = 0;
+    TRY(download(&err, c->shake_err, sizeof(int32_t)));
+    TRY(download(&its, c->shake_iters, sizeof(int32_t)));
+    TRY(download(ff, st, sizeof(float) * 3 * (size_t)c->N));
+    if (vir)
+        for (int r = 0; r < c->nrep; ++r) {
+            vir[0] += E(e, r, V_X); vir[1] += E(e, r, V_Y); vir[2] += E(e, r, V_Z);
+        }
+    if (max_iter) *max_iter = its;
+    if (err) return fail(SOFIA_E_SHAKE, "SHAKE determinant <= 0");
+    return SOFIA_OK;
+}
+
+int sofia_langevin(sofia_ctx *c, float fric, float gconst, uint64_t seed, uint64_t step,
+                   sofia_coords *ff_lang) {
+    if (!c || !ff_lang) return fail(SOFIA_E_ARG, "null argument");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    float *st = staging_f32(c);
+    launch_langevin32(c, fric, gconst, seed, step, st);
+    TRY(sync_check(c));
+    return download(ff_lang, st, sizeof(float) * 3 * (size_t)c->N);
+}
+
+// ------------------------------------------------------------------ engine
+int sofia_set_thermostat(sofia_ctx *c, const sofia_thermostat *t) {
+    if (!c || !t) return fail(SOFIA_E_ARG, "null argument");
+    bool any = false;
+    for (int r = 0; r < c->nrep; ++r) {
+        auto &R = c->reps_h[r];
+        const double g = t[r].gamma_ps * 1e-3;   // 1/fs
+        // Source/get_started.f90:81-84 with the friction as a parameter:
+        // fric = gamma*amu2e, gconst = sqrt(12 kB kcal2amu T gamma / (dt/2)) * amu2e
+        const double kB = (double)0.001987204181f;
+        R.fric = g * c->p.amu2e;
+        R.gconst = (g > 0.0) ? std::sqrt(12.0 * kB * (double)c->p.kcal2amu * t[r].temp * g /
+                                         (0.5 * (double)c->p.dt)) * c->p.amu2e
+                             : 0.0;
+        R.seed = t[r].seed;
+        any = any || g > 0.0;
+    }
+    c->has_thermostat = any;
+    TRY(upload(c->reps_d, c->reps_h.data(), sizeof(sofia::RepInfo) * c->nrep));
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return SOFIA_OK;
+}
+
+namespace {
+
+int nb_mode(const sofia_ctx *c) {
+    if (c->p.coul_flag == SOFIA_COUL_EWALD) return 2;
+    return c->p.precision == SOFIA_PREC_MIXED ? 1 : 0;
+}
+
+void force_sequence(sofia_ctx *c, bool energies) {
+    launch_zero_forces(c, c->fix, energies);
+    launch_nonbond(c, nb_mode(c), energies);
+    launch_bonded_engine(c);
+    if (c->has_thermostat) launch_langevin64(c);
+    launch_shake64(c);
+}
+
+void step_sequence(sofia_ctx *c, bool energies) {
+    launch_nve_v64(c, false);
+    launch_nve_x64(c);
+    launch_check_neighbor(c, true);
+    launch_cell_reset(c, false);
+    launch_bin(c);
+    launch_cell_scan(c);
+    launch_scatter_sort(c);
+    launch_gather(c, true);
+    launch_remap_topology(c);
+    launch_build_list(c);
+    launch_finish_rebuild(c, true);
+    force_sequence(c, energies);
+    launch_nve_v64(c, true);
+    launch_step_advance(c);
+}
+
+int collect_energies(sofia_ctx *c) {
+    std::vector<long long> e;
+    TRY(read_energies(c, e));
+    for (int r = 0; r < c->nrep; ++r) {
+        double *o = c->last_e.data() + (size_t)r * 8;
+        o[0] = E(e, r, E_LJ); o[1] = E(e, r, E_COUL); o[2] = E(e, r, E_POT14);
+        o[3] = E(e, r, E_COUL14); o[4] = E(e, r, E_BOND); o[5] = E(e, r, E_ANGLE);
+        o[6] = E(e, r, E_DIHED); o[7] = E(e, r, E_KE);
+    }
+    return SOFIA_OK;
+}
+
+int check_engine_flags(sofia_ctx *c) {
+    int32_t err = 0, mx = 0;
+    SOFIA_HIP(hipMemcpyAsync(&err, c->shake_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    SOFIA_HIP(hipMemcpyAsync(&mx, c->maxcnt, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_check(c));
+    if (mx > c->neigh_alloc) return fail(SOFIA_E_OVERFLOW, "Verlet list overflow during an MD segment");
+    if (err) return fail(SOFIA_E_SHAKE, "SHAKE determinant <= 0 during an MD segment");
+    return SOFIA_OK;
+}
+
+}  // namespace
+
+int sofia_md_init(sofia_ctx *c) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    launch_engine_from32(c);
+    c->engine_ready = true;
+    launch_set_rebuild(c, 1);
+    // engine lists get 50 % headroom so a segment never has to regrow them
+    TRY(rebuild_now(c, true, false));
+    int32_t mx = 0;
+    {
+        std::vector<int32_t> nn(c->N);
+        TRY(download(nn.data(), c->numneigh, sizeof(int32_t) * c->N));
+        for (int v : nn) mx = std::max(mx, v);
+        const int want = std::max(64, ((mx * 3 / 2 + 15) / 16) * 16);
+        if (want > c->neigh_alloc) {
+            TRY(ensure_list(c, want));
+            launch_set_rebuild(c, 1);
+            TRY(rebuild_now(c, true, false));
+        }
+    }
+    SOFIA_HIP(hipMemsetAsync(c->nbuilds, 0, sizeof(int32_t) * c->nrep, c->stream));
+    SOFIA_HIP(hipMemsetAsync(c->shake_err, 0, sizeof(int32_t), c->stream));
+    SOFIA_HIP(hipMemsetAsync(c->maxcnt, 0, sizeof(int32_t), c->stream));
+    force_sequence(c, true);
+    TRY(collect_energies(c));
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return check_engine_flags(c);
+}
+
+int sofia_md_run(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
+    if (nsteps <= 0) return SOFIA_OK;
+    if (use_graph && !c->graph) {
+        hipGraph_t g = nullptr;
+        SOFIA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        step_sequence(c, false);
+        SOFIA_HIP(hipStreamEndCapture(c->stream, &g));
+        SOFIA_HIP(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+        c->graph_raw = g;
+    }
+    for (int s = 0; s < nsteps - 1; ++s) {
+        if (use_graph) {
+            SOFIA_HIP(hipGraphLaunch(c->graph, c->stream));
+        } else {
+            step_sequence(c, false);
+        }
+    }
+    step_sequence(c, true);   // last step also reduces energies
+    c->step_h += nsteps;
+    TRY(collect_energies(c));
+    return check_engine_flags(c);
+}
+
+int sofia_minimize(sofia_ctx *c, int32_t nsteps, double gamma, double max_disp) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
+    if (nsteps < 0 || !(gamma > 0.0) || !(max_disp > 0.0)) return fail(SOFIA_E_ARG, "bad minimizer arguments");
+    const bool thermo = c->has_thermostat;
+    c->has_thermostat = false;
+    for (int s = 0; s < nsteps; ++s) {
+        launch_zero_forces(c, c->fix, false);
+        launch_nonbond(c, nb_mode(c), false);
+        launch_bonded_engine(c);
+        launch_minimize_step(c, gamma, max_disp);
+        launch_check_neighbor(c, true);
+        launch_cell_reset(c, false);
+        launch_bin(c);
+        launch_cell_scan(c);
+        launch_scatter_sort(c);
+        launch_gather(c, true);
+        launch_remap_topology(c);
+        launch_build_list(c);
+        launch_finish_rebuild(c, true);
+    }
+    c->has_thermostat = thermo;
+    force_sequence(c, true);
+    TRY(collect_energies(c));
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return check_engine_flags(c);
+}
+
+int sofia_md_energies(sofia_ctx *c, double *out) {
+    if (!c || !out) return fail(SOFIA_E_ARG, "null argument");
+    std::memcpy(out, c->last_e.data(), sizeof(double) * c->last_e.size());
+    return SOFIA_OK;
+}
+
+int sofia_md_rebuilds(sofia_ctx *c, int32_t *out) {
+    if (!c || !out) return fail(SOFIA_E_ARG, "null argument");
+    TRY(sync_check(c));
+    return download(out, c->nbuilds, sizeof(int32_t) * c->nrep);
+}
+
+int sofia_set_tuning(sofia_ctx *c, int32_t lanes, int32_t block, int32_t list_waves) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+        return fail(SOFIA_E_ARG, "lanes_per_atom must be 4, 8, 16, 32 or 64");
+    if (block < 64 || block > 256 || block % 64) return fail(SOFIA_E_ARG, "block must be 64, 128, 192 or 256");
+    if (list_waves != 2 && list_waves != 4 && list_waves != 8 && list_waves != 16)
+        return fail(SOFIA_E_ARG, "list_waves must be 2, 4, 8 or 16");
+    c->lanes = lanes;
+    c->block = block;
+    c->list_waves = list_waves;
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return SOFIA_OK;
+}
+
+int sofia_set_molecules(sofia_ctx *c, const int32_t *mol) {
+    if (!c || !mol) return fail(SOFIA_E_ARG, "null argument");
+    // per-replica molecule ids -> one global index space for the union-find
+    std::vector<int32_t> g(c->N, -1), mrep;
+    int base = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        const auto &R = c->reps_h[r];
+        int mx = -1;
+        for (int k = 0; k < R.natom; ++k) {
+            const int v = mol[R.atom0 + k];
+            if (v >= 0) {
+                g[R.atom0 + k] = base + v;
+                mx = std::max(mx, v);
+            }
+        }
+        for (int k = 0; k <= mx; ++k) mrep.push_back(r);
+        base += mx + 1;
+    }
+    dfree(c->mol_rep); dfree(c->uf_parent); dfree(c->uf_size); dfree(c->uf_largest);
+    c->nmol = base;
+    TRY(dalloc(&c->mol_rep, std::max(1, base)));
+    TRY(dalloc(&c->uf_parent, std::max(1, base)));
+    TRY(dalloc(&c->uf_size, std::max(1, base)));
+    TRY(dalloc(&c->uf_largest, c->nrep));
+    if (base) TRY(upload(c->mol_rep, mrep.data(), sizeof(int32_t) * base));
+    return upload(c->a.mol, g.data(), sizeof(int32_t) * c->N);
+}
+
+int sofia_cluster_largest(sofia_ctx *c, float rcontact, int32_t n_o, const int32_t *o_types,
+                          int32_t n_n, const int32_t *n_types, int32_t *largest) {
+    if (!c || !largest || n_o < 0 || n_n < 0) return fail(SOFIA_E_ARG, "bad argument");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no Verlet list");
+    if (!c->uf_parent) return fail(SOFIA_E_STATE, "call sofia_set_molecules first");
+    std::vector<int> ot(o_types, o_types + n_o), nt(n_types, n_types + n_n);
+    return cluster_largest(c, rcontact, ot, nt, largest);
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// Internal declarations shared by the HIP translation units of libsofia_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sofia_hip.h"
+
+namespace sofia {
+
+constexpr int SPEC = 12;                     // specbond slots per atom
+constexpr double FIX_SCALE = 4294967296.0;   // 2^32 units per kcal/mol/A and per kcal/mol
+constexpr double FIX_INV = 1.0 / FIX_SCALE;
+constexpr float PAIR_QUANT = 16777216.0f;    // 2^24: per-pair quantum of the mixed kernel
+constexpr int PAIR_SHIFT = 8;                // 2^32 / 2^24
+
+// energy / virial accumulator slots (int64 fixed point, per replica)
+enum {
+    E_LJ = 0, E_COUL, E_POT14, E_COUL14, E_BOND, E_ANGLE, E_DIHED, E_KE,
+    V_X, V_Y, V_Z, E_CORR, NE
+};
+
+// Per-replica geometry and thermostat (device table).
+struct RepInfo {
+    float box, ibox, rn;
+    int32_t ncellD, ncellT;
+    int32_t atom0, natom;
+    int32_t cell0;
+    double fric;      // friction * amu2e (kcal/mol per amu*A/fs)
+    double gconst;    // noise amplitude * amu2e
+    uint64_t seed;
+};
+
+// Scalars passed by value to every kernel.
+struct DevParams {
+    float coulpre, dt, kcal2amu;
+    int32_t ntypes;
+    double rcut2, cut_coulsq, rv2, rvrc2;
+    double alpha, gewald, e_shift, f_shift, ewald_p, ewald_f;
+    double a1, a2, a3, a4, a5, my_pis, my_pis_inv, amu2e;
+    double fudge12, fudge13, fudge14;
+    double shake_tol;
+    int32_t shake_maxiter;
+    int32_t nrep;
+    int32_t natom;
+    int32_t neigh_alloc;
+};
+
+// Per-atom arrays that the cell sort permutes (one set; the context keeps two).
+struct AtomArrays {
+    float4 *pos4 = nullptr;     // x, y, z, type (int bits)
+    float *q = nullptr;
+    int32_t *gid = nullptr;
+    int32_t *num1 = nullptr, *num2 = nullptr, *num3 = nullptr;
+    int32_t *spec = nullptr;    // [N][12] global 0-based, -1 = empty
+    float *v32 = nullptr;       // AoS coords (3N)
+    double *xd = nullptr;       // SoA x | y | z (3N)
+    double *vd = nullptr;       // SoA (3N)
+    int32_t *mol = nullptr;     // molecule id (cluster analysis), -1 = solvent
+};
+
+}  // namespace sofia
+
+struct sofia_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    sofia_params p{};
+    sofia::DevParams dp{};
+    int32_t nrep = 0, N = 0;
+    std::vector<sofia::RepInfo> reps_h;
+    sofia::RepInfo *reps_d = nullptr;
+    int32_t ncell_total = 0, ncell_cap = 0;
+
+    sofia::AtomArrays a, b;     // a = current, b = gather target
+    int32_t *rep = nullptr;     // replica of each atom (invariant under the sort)
+
+    // reference-format and engine state that is not permuted as a set
+    float *dr32 = nullptr;      // drx | dry | drz (3N), real*4
+    double *drd = nullptr;      // FP64 displacement since the last build (engine)
+    double *fnb = nullptr;      // nonbond forces, SoA (3N)
+    long long *fix = nullptr;   // fixed-point accumulator of every other force, SoA (3N)
+    long long *fscratch = nullptr;   // fixed-point scratch for drop-in outputs (3N)
+    float *stage32 = nullptr;        // AoS real*4 staging for drop-in inputs/outputs (3N)
+
+    // cluster analysis: global solute-molecule index per atom lives in AtomArrays::mol
+    int32_t nmol = 0;
+    int32_t *mol_rep = nullptr, *uf_parent = nullptr, *uf_size = nullptr, *uf_largest = nullptr;
+
+    // cells
+    int32_t *ccount = nullptr, *cstart = nullptr, *cnum = nullptr, *cell_rep = nullptr;
+    int32_t *acell = nullptr, *aslot = nullptr, *perm = nullptr, *tt = nullptr;
+    int32_t *atombin = nullptr, *sorted_ids = nullptr;
+
+    // Verlet list
+    int32_t *nlist = nullptr, *numneigh = nullptr;
+    int32_t neigh_alloc = 0;
+    size_t nlist_cap = 0;
+    int32_t *maxcnt = nullptr;  // device scalar
+    bool have_list = false;
+
+    // flags
+    int32_t *rebuild = nullptr;     // per replica
+    int32_t *nbuilds = nullptr;     // per replica
+    int32_t *shake_err = nullptr;   // device scalar
+    int32_t *shake_iters = nullptr; // device scalar
+
+    // topology (global 0-based indices)
+    int32_t *bondlist = nullptr, nb = 0;
+    int32_t *anglelist = nullptr, na = 0;
+    int32_t *dihedlist = nullptr, nd = 0;
+    int32_t *torsions = nullptr, ntor = 0;
+    int32_t *shake_num = nullptr, *shake_atoms = nullptr, ns = 0;
+    double *shake_bond = nullptr;
+
+    // force field
+    float4 *ljtab = nullptr;    // (lj1, lj2, lj3, lj4) per type pair
+    double4 *lj14 = nullptr;    // (lj14_1..4) per type pair
+    double *mass = nullptr;     // per type
+    double *bond_k = nullptr, *bond_r0 = nullptr, *angle_k = nullptr, *angle_th0 = nullptr;
+    double *tors_k = nullptr, *tors_n = nullptr, *tors_shift = nullptr;
+    int32_t nbt = 0, nat = 0, ntt = 0;
+    bool have_ff = false;
+
+    // energies (int64 fixed point, per replica x NE)
+    long long *eacc = nullptr;
+    std::vector<double> last_e;
+
+    // engine
+    bool engine_ready = false;
+    uint64_t *step_d = nullptr;  // device step counter (Langevin key)
+    uint64_t step_h = 0;
+    int lanes = 16, block = 256, list_waves = 4;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_raw = nullptr;
+    bool has_thermostat = false;
+
+    // host staging
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+namespace sofia {
+
+// ------------------------------------------------------------------ kernel launchers
+// neighbor build (k_neighbor.hip)
+void launch_cell_reset(sofia_ctx *c, bool all);
+void launch_bin(sofia_ctx *c);
+void launch_cell_scan(sofia_ctx *c);
+void launch_scatter_sort(sofia_ctx *c);
+void launch_gather(sofia_ctx *c, bool engine);
+void launch_remap_topology(sofia_ctx *c);
+void launch_build_list(sofia_ctx *c);
+void launch_check_neighbor(sofia_ctx *c, bool engine);
+void launch_set_rebuild(sofia_ctx *c, int32_t value);
+void launch_finish_rebuild(sofia_ctx *c, bool engine);
+
+// nonbond (k_nonbond.hip): mode 0 = DSF FP64, 1 = DSF mixed, 2 = Ewald real space (FP32)
+void launch_nonbond(sofia_ctx *c, int mode, bool energies);
+
+// bonded (k_bonded.hip)
+void launch_correct(sofia_ctx *c, int which, long long *fout);
+void launch_pair14(sofia_ctx *c, long long *fout);
+void launch_bonded_engine(sofia_ctx *c);
+
+// integration (k_integrate.hip)
+void launch_zero_forces(sofia_ctx *c, long long *f, bool energies);
+void launch_fix_to_coords(sofia_ctx *c, const long long *f, float *out_aos);
+void launch_f64_to_coords(sofia_ctx *c, const double *f, float *out_aos);
+void launch_nve_v32(sofia_ctx *c, float dt_partial, const float *ff_aos);
+void launch_nve_x32(sofia_ctx *c);
+void launch_pbc32(sofia_ctx *c);
+void launch_shake32(sofia_ctx *c, int iflag, float *ff_aos, double *vir_dev);
+void launch_langevin32(sofia_ctx *c, float fric, float gconst, uint64_t seed, uint64_t step,
+                       float *out_aos);
+void launch_engine_from32(sofia_ctx *c);
+void launch_engine_to32(sofia_ctx *c);
+void launch_nve_v64(sofia_ctx *c, bool ke);
+void launch_nve_x64(sofia_ctx *c);
+void launch_langevin64(sofia_ctx *c);
+void launch_shake64(sofia_ctx *c);
+void launch_step_advance(sofia_ctx *c);
+void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp);
+
+// cluster analysis (k_cluster.hip)
+int cluster_largest(sofia_ctx *c, float rcontact, const std::vector<int> &o_types,
+                    const std::vector<int> &n_types, int32_t *largest);
+
+// error helpers (sofia_api.cpp)
+int fail(int code, const std::string &msg);
+int check_hip(hipError_t e, const char *what);
+
+}  // namespace sofia
+
+#define SOFIA_HIP(call)                                                   \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return sofia::check_hip(e_, #call);         \
+    } while (0)

@@ -1,0 +1,452 @@
+"""Host side of the MI355X hot path, mirroring the reference's interface.
+
+``SofiaGPU`` holds the state the reference keeps in ``module global``
+(Source/global.f90) for one system and exposes the hot-path subroutines under their
+reference names -- ``neighbor_wrapper``, ``check_neighbor``,
+``lj_cut_coul_dsf_nonewton``, ``lj_cut_coul_long_nonewton``, ``correct_12_bonds`` ...,
+``nve_v``, ``nve_x``, ``pbc``, ``shake``, ``langevin`` -- each a thin call through the
+C ABI of libsofia_hip.so.  Like the Fortran routines they read and update the object's
+arrays (position, v, ff, energies) in the reference's layout and precision, including the
+permanent cell-sort permutation of neighbor_wrapper.
+
+``Engine`` is the device-resident path: one or many replicas advanced in fused MD segments
+in FP64 or mixed precision, with the state kept on the GPU between calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import system as S
+from ._lib import (COUL_DSF, COUL_EWALD, PREC_FP64, PREC_MIXED, SofiaParams, SofiaThermostat,
+                   check, lib, ptr)
+
+f32, f64, i32 = np.float32, np.float64, np.int32
+
+
+def make_params(d: dict, precision: int = PREC_FP64, coul_flag: int = COUL_DSF,
+                params: S.Params | None = None, neigh_alloc0: int = S.NEIGH_ALLOC0) -> SofiaParams:
+    """sofia_params from the derived constants of ``system.Params.derived``."""
+    p = params or S.Params()
+    return SofiaParams(ntypes=S.NTYPES, rcut=float(d["rcut"]), cut_coul=float(d["cut_coul"]),
+                       pad=float(d["pad"]), coulpre=float(S.COULPRE), dt=float(d["dt"]),
+                       kcal2amu=float(S.KCAL2AMU), rcut2=float(d["rcut2"]),
+                       cut_coulsq=float(d["cut_coulsq"]), rv2=float(d["rv2"]),
+                       rvrc2=float(d["rvrc2"]), alpha=float(d["alpha"]), gewald=float(d["gewald"]),
+                       e_shift=float(d["e_shift"]), f_shift=float(d["f_shift"]),
+                       ewald_p=float(S.EWALD_P), ewald_f=float(S.EWALD_F), a1=float(S.A1),
+                       a2=float(S.A2), a3=float(S.A3), a4=float(S.A4), a5=float(S.A5),
+                       my_pis=float(S.MY_PIS), my_pis_inv=float(S.MY_PIS_INV),
+                       amu2e=float(S.AMU2E), fudge_12=float(p.fudge_12), fudge_13=float(p.fudge_13),
+                       fudge_14_coul=float(p.fudge_14_coul), shake_tol=float(S.SHAKE_TOL),
+                       shake_maxiter=S.SHAKE_MAXITER, precision=precision, coul_flag=coul_flag,
+                       neigh_alloc0=neigh_alloc0)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _forcefield(ctx):
+    lj = S.lj_tables()
+    arrs = [_c(x, f32) for x in lj[:4]] + [_c(x, f64) for x in lj[4:]] + [_c(S.type_mass(), f64)]
+    check(lib().sofia_set_forcefield(ctx, *[ptr(a) for a in arrs]))
+    return arrs
+
+
+def torsion_list(s: S.SofiaSystem) -> np.ndarray:
+    """Torsions in geometric order a-b-c-d (1-based), one per topology dihedral."""
+    rows = []
+    base = 0
+    for _ in range(s.n_gly):
+        for a, b, c, d, t in S.GLYCINE.diheds:
+            rows.append((base + a, base + b, base + c, base + d, t))
+        base += 10
+    return np.array(rows, i32).reshape(-1, 5)
+
+
+def _bonded_coeffs(ctx, tors: np.ndarray, ntor_rep=None):
+    bk = _c([c[0] for c in S.BOND_COEFF], f64); br = _c([c[1] for c in S.BOND_COEFF], f64)
+    ak = _c([c[0] for c in S.ANGLE_COEFF], f64)
+    ath = _c(np.radians([c[1] for c in S.ANGLE_COEFF]), f64)
+    tk = _c([c[0] for c in S.DIHED_COEFF], f64); tn = _c([c[1] for c in S.DIHED_COEFF], f64)
+    ts = _c([c[2] for c in S.DIHED_COEFF], f64)
+    tors = _c(tors, i32)
+    keep = [bk, br, ak, ath, tk, tn, ts, tors]
+    nr = None if ntor_rep is None else _c(ntor_rep, i32)
+    check(lib().sofia_set_bonded_coeffs(ctx, len(bk), ptr(bk), ptr(br), len(ak), ptr(ak), ptr(ath),
+                                        len(tk), ptr(tk), ptr(tn), ptr(ts), ptr(tors), len(tors),
+                                        ptr(nr)))
+    return keep
+
+
+class SofiaGPU:
+    """One system with the reference's module-global state and subroutine names."""
+
+    def __init__(self, system: S.SofiaSystem, precision: int = PREC_FP64):
+        self.sys = system
+        self.d = system.derived()
+        self.np = system.np
+        self.params = make_params(self.d, precision, params=system.params)
+        h = C.c_void_p()
+        check(lib().sofia_ctx_create(C.byref(self.params), self.np, float(self.d["box"]), C.byref(h)))
+        self.ctx = h
+        self._keep = _forcefield(h)
+        # module-global state (reference layout and precision)
+        self.position = system.pos.astype(f32).copy()
+        self.type = system.types.astype(i32).copy()
+        self.q = system.q.astype(f32).copy()
+        self.v = system.v.astype(f32).copy()
+        self.global_id = system.global_id.astype(i32).copy()
+        self.mol = system.mol.astype(i32).copy()
+        self.num1bond = system.num1bond.copy(); self.num2bond = system.num2bond.copy()
+        self.num3bond = system.num3bond.copy(); self.specbond = system.specbond.copy()
+        self.bondlist = system.bondlist.copy(); self.anglelist = system.anglelist.copy()
+        self.dihedlist = system.dihedlist.copy()
+        self.shake_num = system.shake_num.copy(); self.shake_atoms = system.shake_atoms.copy()
+        self.shake_bond = system.shake_bond.copy()
+        self.ff = np.zeros((self.np, 3), f32)
+        self.drx = np.zeros(self.np, f32); self.dry = np.zeros(self.np, f32); self.drz = np.zeros(self.np, f32)
+        self.potential = 0.0; self.e_coul = 0.0; self.e_coul_14 = 0.0; self.pot_14 = 0.0
+        self.e_coul_bond = 0.0; self.e_coul_angle = 0.0; self.e_coul_dihed = 0.0
+        self.virial = np.zeros(3); self.ke = 0.0
+        self.neigh_alloc = S.NEIGH_ALLOC0
+        self.nn_builds = 0
+        self._upload_all()
+
+    # ------------------------------------------------------------------ plumbing
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().sofia_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ainfo(self):
+        a = np.empty((self.np, 4), f32)
+        a[:, :3] = self.position
+        a[:, 3] = self.type.view(f32)
+        return a
+
+    def _upload_all(self):
+        a = self.ainfo()
+        check(lib().sofia_put_state(self.ctx, ptr(a), ptr(_c(self.q, f32)), ptr(_c(self.v, f32)),
+                                    ptr(_c(self.global_id, i32))))
+        check(lib().sofia_set_exclusions(self.ctx, ptr(_c(self.num1bond, i32)), ptr(_c(self.num2bond, i32)),
+                                         ptr(_c(self.num3bond, i32)), ptr(_c(self.specbond, i32))))
+        check(lib().sofia_set_bonded(self.ctx, ptr(_c(self.bondlist, i32)), len(self.bondlist), None,
+                                     ptr(_c(self.anglelist, i32)), len(self.anglelist), None,
+                                     ptr(_c(self.dihedlist, i32)), len(self.dihedlist), None))
+        check(lib().sofia_set_shake(self.ctx, len(self.shake_num), None, ptr(_c(self.shake_num, i32)),
+                                    ptr(_c(self.shake_atoms, i32)), ptr(_c(self.shake_bond, f64))))
+        self._put_dr()
+
+    def _put_pos(self):
+        a = self.ainfo()
+        check(lib().sofia_put_state(self.ctx, ptr(a), None, ptr(_c(self.v, f32)), None))
+
+    def _get_pos(self):
+        a = np.empty((self.np, 4), f32)
+        v = np.empty((self.np, 3), f32)
+        gid = np.empty(self.np, i32)
+        check(lib().sofia_get_state(self.ctx, ptr(a), ptr(v), ptr(gid)))
+        self.position = a[:, :3].copy()
+        self.type = a[:, 3].view(i32).copy()
+        self.v = v
+        self.global_id = gid
+
+    def _put_dr(self):
+        check(lib().sofia_put_displacement(self.ctx, ptr(_c(self.drx, f32)), ptr(_c(self.dry, f32)),
+                                           ptr(_c(self.drz, f32))))
+
+    def _get_dr(self):
+        check(lib().sofia_get_displacement(self.ctx, ptr(self.drx), ptr(self.dry), ptr(self.drz)))
+
+    # ------------------------------------------------------------------ reference routines
+    def neighbor_wrapper(self, step: int = 0) -> np.ndarray:
+        """Source/mod_neighbor.f90:8-86.  Returns tt (old -> new, 1-based) and re-indexes
+        every host array exactly as sort_posit_charge/sort_* do."""
+        self._put_pos()
+        tt = np.empty(self.np, i32)
+        check(lib().sofia_neighbor_wrapper(self.ctx, step, ptr(tt)))
+        self.nn_builds += 1
+        perm = np.empty(self.np, i32)
+        perm[tt - 1] = np.arange(1, self.np + 1, dtype=i32)
+        inv = perm - 1
+        self._get_pos()
+        self.q = self.q[inv]
+        self.mol = self.mol[inv]
+        self.num1bond = self.num1bond[inv]; self.num2bond = self.num2bond[inv]
+        self.num3bond = self.num3bond[inv]
+        sp = self.specbond[inv]
+        self.specbond = np.where(sp > 0, tt[np.maximum(sp, 1) - 1], 0).astype(i32)
+        self.bondlist[:, :2] = tt[self.bondlist[:, :2] - 1]
+        self.anglelist[:, :3] = tt[self.anglelist[:, :3] - 1]
+        self.dihedlist[:, :4] = tt[self.dihedlist[:, :4] - 1]
+        sa = self.shake_atoms
+        self.shake_atoms = np.where(sa > 0, tt[np.maximum(sa, 1) - 1], 0).astype(i32)
+        self.drx[:] = 0; self.dry[:] = 0; self.drz[:] = 0
+        na = C.c_int32()
+        check(lib().sofia_get_nlist(self.ctx, None, None, C.byref(na)))
+        self.neigh_alloc = na.value
+        return tt
+
+    def get_nlist(self):
+        na = C.c_int32()
+        check(lib().sofia_get_nlist(self.ctx, None, None, C.byref(na)))
+        nl = np.empty(self.np * na.value, i32)
+        nn = np.empty(self.np, i32)
+        check(lib().sofia_get_nlist(self.ctx, ptr(nl), ptr(nn), C.byref(na)))
+        return nl, nn, na.value
+
+    def get_cells(self):
+        from . import system as _s
+        rn, nT, nD = _s.cell_geometry(float(self.d["box"]), float(self.d["rcut"]), float(self.d["cut_coul"]))
+        st = np.empty(nT, i32); en = np.empty(nT, i32); ab = np.empty(self.np, i32)
+        a, b, r = C.c_int32(), C.c_int32(), C.c_float()
+        check(lib().sofia_get_cells(self.ctx, ptr(st), ptr(en), ptr(ab), C.byref(a), C.byref(b), C.byref(r)))
+        return st, en, ab, (np.float32(r.value), a.value, b.value)
+
+    def check_neighbor(self, step: int = 0) -> int:
+        """Source/mod_neighbor.f90:895-913 on drx/dry/drz."""
+        self._put_dr()
+        f = C.c_int32()
+        check(lib().sofia_check_neighbor(self.ctx, C.byref(f)))
+        return f.value
+
+    def _nonbond(self, fn, with64=True):
+        self._put_pos()
+        ff = np.empty((self.np, 3), f32)
+        ff64 = np.empty((self.np, 3), f64) if with64 else None
+        pot, ec = C.c_double(0.0), C.c_double(0.0)
+        vir = np.zeros(3, f64)
+        if with64:
+            check(fn(self.ctx, ptr(ff), ptr(ff64), C.byref(pot), C.byref(ec), ptr(vir)))
+        else:
+            check(fn(self.ctx, ptr(ff), C.byref(pot), C.byref(ec), ptr(vir)))
+        self.ff = ff
+        self.ff64 = ff64
+        self.potential += pot.value
+        self.e_coul += ec.value
+        self.virial = vir
+        return ff
+
+    def lj_cut_coul_dsf_nonewton(self, step: int = 0):
+        """lj_cut_coul_dsf_nonewton_NOMIC, Source/mod_force_nonbond.f90:105-196 (FP64)."""
+        return self._nonbond(lib().sofia_lj_cut_coul_dsf)
+
+    def lj_cut_coul_dsf_nonewton_mixed(self, step: int = 0):
+        """FP32 pair arithmetic with int64 fixed-point force sums."""
+        return self._nonbond(lib().sofia_lj_cut_coul_dsf_mixed)
+
+    def lj_cut_coul_long_nonewton(self, step: int = 0):
+        """lj_cut_coul_long_nonewton_NOMIC, Source/mod_force_nonbond.f90:300-400."""
+        return self._nonbond(lib().sofia_lj_cut_coul_long, with64=False)
+
+    def _correct(self, which):
+        self._put_pos()
+        ff = np.empty((self.np, 3), f32)
+        ec = C.c_double(0.0)
+        vir = np.zeros(3, f64)
+        check(lib().sofia_correct_bonds(self.ctx, which, ptr(ff), C.byref(ec), ptr(vir)))
+        return ff, ec.value, vir
+
+    def correct_12_bonds(self):
+        """Source/mod_force_correction.f90:6-93 -> (ff_bond, e_coul_bond, v_bond)."""
+        ff, e, v = self._correct(12)
+        self.ff_bond, self.e_coul_bond, self.v_bond = ff, self.e_coul_bond + e, v
+        return ff
+
+    def correct_13_bonds(self):
+        """Source/mod_force_correction.f90:95-184."""
+        ff, e, v = self._correct(13)
+        self.ff_angle, self.e_coul_angle, self.v_angle = ff, self.e_coul_angle + e, v
+        return ff
+
+    def correct_14_bonds(self):
+        """Source/mod_force_correction.f90:187-278."""
+        ff, e, v = self._correct(14)
+        self.ff_dihed, self.e_coul_dihed, self.v_dihed = ff, self.e_coul_dihed + e, v
+        return ff
+
+    def pair14_dsf(self):
+        """1-4 pair of dihedral_opls_DSF, Source/mod_force_dihed.f90:1003-1048 -> ffh."""
+        self._put_pos()
+        ff = np.empty((self.np, 3), f32)
+        e1, e2 = C.c_double(0.0), C.c_double(0.0)
+        check(lib().sofia_pair14_dsf(self.ctx, ptr(ff), C.byref(e1), C.byref(e2)))
+        self.ffh = ff
+        self.e_coul_14 += e1.value
+        self.pot_14 += e2.value
+        return ff
+
+    def nve_v(self, dt_partial: float):
+        """Source/mod_ensemble.f90:11-40 with the current ff."""
+        self._put_pos()
+        ke = C.c_double()
+        check(lib().sofia_nve_v(self.ctx, float(np.float32(dt_partial)), ptr(_c(self.ff, f32)), C.byref(ke)))
+        self._get_pos()
+        self.ke = ke.value
+
+    def nve_x(self):
+        """Source/mod_ensemble.f90:48-87 (position and drx/dry/drz)."""
+        self._put_pos()
+        self._put_dr()
+        check(lib().sofia_nve_x(self.ctx))
+        self._get_pos()
+        self._get_dr()
+
+    def pbc(self, step: int = 0):
+        """Source/mod_pbc.f90:6-39."""
+        self._put_pos()
+        check(lib().sofia_pbc(self.ctx))
+        self._get_pos()
+
+    def shake(self, iflag: int = 1):
+        """Source/mod_shake.f90:8-85: constraint forces added to ff."""
+        self._put_pos()
+        ff = _c(self.ff, f32).copy()
+        vir = np.zeros(3, f64)
+        mit = C.c_int32()
+        check(lib().sofia_shake(self.ctx, iflag, ptr(ff), ptr(vir), C.byref(mit)))
+        self.ff = ff
+        self.virial = self.virial + vir
+        self.shake_iters = mit.value
+        return ff
+
+    def langevin(self, fric: float, gconst: float, seed: int, step: int):
+        """Source/mod_langevin.f90:15-43 with a counter-based generator."""
+        self._put_pos()
+        out = np.empty((self.np, 3), f32)
+        check(lib().sofia_langevin(self.ctx, float(fric), float(gconst), seed, step, ptr(out)))
+        self.ff_lang = out
+        return out
+
+
+@dataclass
+class ReplicaSpec:
+    system: S.SofiaSystem
+    temp: float = 300.0
+    gamma_ps: float = 0.0
+    seed: int = 1
+
+
+class Engine:
+    """Device-resident MD for one or many replicas (a single system is a batch of one)."""
+
+    def __init__(self, replicas: list, precision: int = PREC_MIXED, tuning=None,
+                 with_bonded: bool = True, coul_flag: int = COUL_DSF):
+        if not replicas:
+            raise ValueError("no replicas")
+        reps = [r if isinstance(r, ReplicaSpec) else ReplicaSpec(r) for r in replicas]
+        self.reps = reps
+        s0 = reps[0].system
+        self.d = s0.derived()
+        self.params = make_params(self.d, precision, coul_flag, params=s0.params)
+        n = np.array([r.system.np for r in reps], i32)
+        boxes = np.array([float(r.system.box) for r in reps], f32)
+        self.natoms = n
+        self.N = int(n.sum())
+        h = C.c_void_p()
+        check(lib().sofia_ctx_create_batch(C.byref(self.params), len(reps), ptr(n), ptr(boxes), C.byref(h)))
+        self.ctx = h
+        self._keep = _forcefield(h)
+        cat = lambda f: np.concatenate([f(r.system) for r in reps])
+        a = np.empty((self.N, 4), f32)
+        a[:, :3] = cat(lambda s: s.pos)
+        a[:, 3] = cat(lambda s: s.types).astype(i32).view(f32)
+        check(lib().sofia_put_state(h, ptr(a), ptr(_c(cat(lambda s: s.q), f32)),
+                                    ptr(_c(cat(lambda s: s.v), f32)),
+                                    ptr(_c(cat(lambda s: s.global_id), i32))))
+        check(lib().sofia_set_exclusions(h, ptr(_c(cat(lambda s: s.num1bond), i32)),
+                                         ptr(_c(cat(lambda s: s.num2bond), i32)),
+                                         ptr(_c(cat(lambda s: s.num3bond), i32)),
+                                         ptr(_c(cat(lambda s: s.specbond), i32))))
+        per = lambda f: _c([len(f(r.system)) for r in reps], i32)
+        bl = _c(cat(lambda s: s.bondlist), i32); al = _c(cat(lambda s: s.anglelist), i32)
+        dl = _c(cat(lambda s: s.dihedlist), i32)
+        nb, na, nd = per(lambda s: s.bondlist), per(lambda s: s.anglelist), per(lambda s: s.dihedlist)
+        check(lib().sofia_set_bonded(h, ptr(bl), len(bl), ptr(nb), ptr(al), len(al), ptr(na),
+                                     ptr(dl), len(dl), ptr(nd)))
+        if with_bonded:
+            tl = [torsion_list(r.system) for r in reps]
+            self._keep += _bonded_coeffs(h, np.concatenate(tl) if tl else np.zeros((0, 5), i32),
+                                         [len(t) for t in tl])
+        sn = _c(cat(lambda s: s.shake_num), i32); sa = _c(cat(lambda s: s.shake_atoms), i32)
+        sb = _c(cat(lambda s: s.shake_bond), f64); ns = per(lambda s: s.shake_num)
+        check(lib().sofia_set_shake(h, len(sn), ptr(ns), ptr(sn), ptr(sa), ptr(sb)))
+        mol = cat(lambda s: np.where(s.moltype == 1, s.mol, -1).astype(i32))
+        check(lib().sofia_set_molecules(h, ptr(_c(mol, i32))))
+        th = (SofiaThermostat * len(reps))(*[SofiaThermostat(r.temp, r.gamma_ps, r.seed) for r in reps])
+        check(lib().sofia_set_thermostat(h, th))
+        if tuning:
+            self.set_tuning(*tuning)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().sofia_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, lanes: int, block: int, list_waves: int):
+        check(lib().sofia_set_tuning(self.ctx, lanes, block, list_waves))
+
+    def init(self):
+        check(lib().sofia_md_init(self.ctx))
+
+    def minimize(self, nsteps: int = 200, gamma: float = 0.002, max_disp: float = 0.05):
+        check(lib().sofia_minimize(self.ctx, nsteps, gamma, max_disp))
+
+    def run(self, nsteps: int, use_graph: bool = True):
+        check(lib().sofia_md_run(self.ctx, nsteps, 1 if use_graph else 0))
+
+    def synchronize(self):
+        check(lib().sofia_synchronize(self.ctx))
+
+    def energies(self) -> np.ndarray:
+        """(nrep, 8): potential, e_coul, pot_14, e_coul_14, e_bond, e_angle, e_dihedral, ke."""
+        out = np.empty(len(self.reps) * 8, f64)
+        check(lib().sofia_md_energies(self.ctx, ptr(out)))
+        return out.reshape(-1, 8)
+
+    def rebuilds(self) -> np.ndarray:
+        out = np.empty(len(self.reps), i32)
+        check(lib().sofia_md_rebuilds(self.ctx, ptr(out)))
+        return out
+
+    def state(self):
+        """FP64 positions/velocities and global ids per replica, in each replica's own
+        original atom order (global_id order)."""
+        x = np.empty((self.N, 3), f64)
+        v = np.empty((self.N, 3), f64)
+        check(lib().sofia_get_state_f64(self.ctx, ptr(x), ptr(v)))
+        a = np.empty((self.N, 4), f32)
+        gid = np.empty(self.N, i32)
+        check(lib().sofia_get_state(self.ctx, ptr(a), None, ptr(gid)))
+        out = []
+        off = 0
+        for n in self.natoms:
+            sl = slice(off, off + n)
+            order = np.argsort(gid[sl])
+            out.append((x[sl][order], v[sl][order]))
+            off += n
+        return out
+
+    def largest_cluster(self, rcontact: float = 3.5) -> np.ndarray:
+        o = _c([4, 5], i32)
+        nt = _c([3], i32)
+        out = np.empty(len(self.reps), i32)
+        check(lib().sofia_cluster_largest(self.ctx, float(rcontact), len(o), ptr(o), len(nt), ptr(nt),
+                                          ptr(out)))
+        return out

@@ -353,83 +353,114 @@ def box_for(n_gly: int, n_wat: int, density_g_cm3: float = 1.0) -> float:
 
 def make_system(n_gly: int, n_wat: int, seed: int = 20261202, params: Params | None = None,
                 box: float | None = None, temp: float | None = None,
-                min_sep: float = 1.9, jitter: float = 0.2) -> SofiaSystem:
+                jitter: float = 0.2) -> SofiaSystem:
     """Seeded synthetic glycine/water box at 1.0 g/cm^3.
 
-    Molecule centres sit on a jittered cubic lattice (glycines first, on a sparse
-    sub-lattice), orientations are uniform on SO(3) and re-drawn where atoms of two
-    molecules come closer than ``min_sep``; velocities are Maxwell-Boltzmann at ``temp``
+    Glycine centres take a sparse sub-lattice of a cubic lattice, waters the lattice sites
+    that are at least 2.9 A from every glycine atom; centres are jittered, orientations
+    are uniform on SO(3) and re-drawn for molecules in a clash (heavy atoms closer than
+    2.4 A or any atoms closer than 1.5 A).  Velocities are Maxwell-Boltzmann at ``temp``
     with the centre-of-mass drift removed and the constraint components projected out.
     """
+    from scipy.spatial import cKDTree
     params = params or Params()
     if temp is not None:
         params.temp = temp
     rng = np.random.default_rng(seed)
     nmol = n_gly + n_wat
     L = float(box) if box is not None else box_for(n_gly, n_wat)
-    # lattice with room for the glycines (each blocks ~8 water sites)
-    need = n_wat + 8 * n_gly
-    nl = int(math.ceil(need ** (1.0 / 3.0)))
-    a = L / nl
-    idx = np.arange(nl ** 3)
-    grid = np.stack([idx % nl, (idx // nl) % nl, idx // (nl * nl)], axis=1).astype(f64)
-    centres = (grid + 0.5) * a
-    # glycines on the even sub-lattice, chosen at random
-    even = np.where((grid[:, 0] % 2 == 0) & (grid[:, 1] % 2 == 0) & (grid[:, 2] % 2 == 0))[0]
-    if n_gly > len(even):
-        raise ValueError("too many glycines for the lattice")
-    gsites = rng.choice(even, size=n_gly, replace=False)
-    blocked = np.zeros(len(centres), bool)
-    blocked[gsites] = True
-    if n_gly:
-        # block lattice sites within 2.9 A of a glycine centre
-        from scipy.spatial import cKDTree
-        tree = cKDTree(centres, boxsize=L)
-        for lst in tree.query_ball_point(centres[gsites], r=max(2.9, 1.01 * a)):
-            blocked[lst] = True
-    free = np.where(~blocked)[0]
-    if n_wat > len(free):
-        raise ValueError("lattice too small for the waters")
-    wsites = rng.choice(free, size=n_wat, replace=False)
-    mol_centres = np.concatenate([centres[gsites], centres[wsites]])
-    mol_centres += rng.uniform(-jitter, jitter, size=mol_centres.shape)
+    heavy_types = np.array([1, 2, 3, 4, 5, 11])
 
-    templates = [GLYCINE] * n_gly + [WATER] * n_wat
+    def lattice(nl):
+        a = L / nl
+        idx = np.arange(nl ** 3)
+        grid = np.stack([idx % nl, (idx // nl) % nl, idx // (nl * nl)], axis=1)
+        return grid, (grid + 0.5) * a, a
+
+    R_gly = _random_rotations(rng, n_gly)
+    gly_xyz = np.zeros((0, 3))
+    nl = int(math.ceil((n_wat + 10 * n_gly) ** (1.0 / 3.0)))
+    for _attempt in range(6):
+        grid, centres, a = lattice(nl)
+        stride = 3 if (nl // 3) ** 3 >= n_gly else 2
+        sub = np.where(np.all(grid % stride == 0, axis=1))[0]
+        if n_gly > len(sub):
+            nl += 1
+            continue
+        gsites = rng.choice(sub, size=n_gly, replace=False) if n_gly else np.zeros(0, int)
+        gc = centres[gsites] + rng.uniform(-jitter, jitter, size=(n_gly, 3))
+        # glycines: re-draw orientations of clashing pairs
+        for _ in range(60):
+            gly_xyz = (gc[:, None, :] + np.einsum("nij,aj->nai", R_gly, GLYCINE.geom)).reshape(-1, 3)
+            if n_gly < 2:
+                break
+            tree = cKDTree(np.mod(gly_xyz, L), boxsize=L)
+            pr = tree.query_pairs(2.4, output_type="ndarray")
+            molid = np.repeat(np.arange(n_gly), 10)
+            pr = pr[molid[pr[:, 0]] != molid[pr[:, 1]]] if len(pr) else pr
+            if len(pr) == 0:
+                break
+            bad = np.unique(molid[pr[:, 0]])
+            R_gly[bad] = _random_rotations(rng, len(bad))
+        free = np.ones(len(centres), bool)
+        if n_gly:
+            free[gsites] = False
+            tree = cKDTree(np.mod(gly_xyz, L), boxsize=L)
+            near = tree.query_ball_point(np.mod(centres, L), r=2.9, return_length=True)
+            free &= near == 0
+        if free.sum() >= n_wat:
+            break
+        nl += 1
+    else:
+        raise ValueError("could not fit the waters on the lattice")
+    wsites = rng.choice(np.where(free)[0], size=n_wat, replace=False)
+    wc = centres[wsites] + rng.uniform(-jitter, jitter, size=(n_wat, 3))
+    R_w = _random_rotations(rng, n_wat)
+
     natom_mol = np.array([10] * n_gly + [3] * n_wat)
     offsets = np.concatenate([[0], np.cumsum(natom_mol)])
     N = int(offsets[-1])
-
-    # orientations, redrawn for molecules involved in close contacts
-    R = _random_rotations(rng, nmol)
     mol_of_atom = np.repeat(np.arange(nmol), natom_mol)
-    local = np.concatenate([GLYCINE.geom] * n_gly + [WATER.geom] * n_wat) if nmol else np.zeros((0, 3))
+    types = np.concatenate([np.array(GLYCINE.types * n_gly, np.int32),
+                            np.array(WATER.types * n_wat, np.int32)]) if nmol else np.zeros(0, np.int32)
+    is_heavy = np.isin(types, heavy_types)
 
-    def place(Rm):
-        return mol_centres[mol_of_atom] + np.einsum("nij,nj->ni", Rm[mol_of_atom], local)
+    def place():
+        w = (wc[:, None, :] + np.einsum("nij,aj->nai", R_w, WATER.geom)).reshape(-1, 3)
+        return np.concatenate([gly_xyz, w]) if n_gly else w
 
-    from scipy.spatial import cKDTree
-    xyz = place(R)
-    for _ in range(40):
+    xyz = place()
+    for _ in range(80):
         tree = cKDTree(np.mod(xyz, L), boxsize=L)
-        pairs = tree.query_pairs(min_sep, output_type="ndarray")
-        if len(pairs):
-            inter = mol_of_atom[pairs[:, 0]] != mol_of_atom[pairs[:, 1]]
-            pairs = pairs[inter]
-        if len(pairs) == 0:
+        pr = tree.query_pairs(2.4, output_type="ndarray")
+        if len(pr):
+            inter = mol_of_atom[pr[:, 0]] != mol_of_atom[pr[:, 1]]
+            pr = pr[inter]
+        if len(pr):
+            dv = xyz[pr[:, 0]] - xyz[pr[:, 1]]
+            dv -= L * np.round(dv / L)
+            dd = np.sqrt((dv * dv).sum(1))
+            clash = (is_heavy[pr[:, 0]] & is_heavy[pr[:, 1]]) | (dd < 1.5)
+            pr = pr[clash]
+        if len(pr) == 0:
             break
-        bad = np.unique(mol_of_atom[pairs.ravel()])
-        # redraw orientation for one molecule of each clashing pair (favour waters)
-        R[bad] = _random_rotations(rng, len(bad))
-        xyz = place(R)
+        bad = np.unique(mol_of_atom[pr.ravel()])
+        bad_w = bad[bad >= n_gly] - n_gly
+        if len(bad_w) == 0:
+            break
+        R_w[bad_w] = _random_rotations(rng, len(bad_w))
+        # nudge stubborn waters a little as well
+        wc[bad_w] += rng.uniform(-0.15, 0.15, size=(len(bad_w), 3))
+        xyz = place()
 
     # wrap into (0, box] as pbc() does (Source/mod_pbc.f90:6-39)
     box32 = f32(L)
     pos = np.mod(xyz, L).astype(f32)
     pos = np.where(pos <= 0, pos + box32, pos).astype(f32)
     pos = np.where(pos > box32, pos - box32, pos).astype(f32)
+    templates = [GLYCINE] * n_gly + [WATER] * n_wat
 
     # ---- topology -------------------------------------------------------------------
-    types = np.concatenate([np.array(t.types, np.int32) for t in templates]) if nmol else np.zeros(0, np.int32)
     qt = type_charge()
     q = qt[types - 1].astype(f32)
     moltype = np.repeat(np.array([1] * n_gly + [2] * n_wat, np.int32), natom_mol)

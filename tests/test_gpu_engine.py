@@ -1,0 +1,159 @@
+"""Device-resident engine on an MI355X: minimisation, MD segments, determinism, batch
+invariance (a replica's trajectory is the same alone and inside a batch), precision modes
+and the union-find cluster analysis."""
+import numpy as np
+import pytest
+
+from molecular_simulation_code_amd import system as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _engine():
+    from molecular_simulation_code_amd import _lib
+    if _lib.device_count() < 1:
+        pytest.skip("no HIP device")
+    from molecular_simulation_code_amd import host
+    return host
+
+
+def _sys(ng, nw, seed, **kw):
+    return S.make_system(ng, nw, seed=seed, params=S.Params(**kw))
+
+
+def test_minimize_then_nve_energy_is_stable():
+    host = _engine()
+    s = _sys(100, 600, 31)
+    say("[engine] minimize + NVE, np", s.np)
+    e = host.Engine([s], precision=host.PREC_FP64)
+    try:
+        e.init()
+        e0 = e.energies()[0]
+        e.minimize(300)
+        e1 = e.energies()[0]
+        assert e1[:7].sum() < e0[:7].sum()          # potential went down
+        e.run(50)
+        E = e.energies()[0]
+        assert np.all(np.isfinite(E))
+        tot1 = E.sum()
+        e.run(50)
+        tot2 = e.energies()[0].sum()
+        say("  total energy", tot1, tot2)
+        assert abs(tot2 - tot1) < 0.02 * abs(E[7]) + 5.0   # bounded drift over 50 steps
+    finally:
+        e.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_run_is_deterministic_and_graph_equals_eager(prec):
+    host = _engine()
+    P = host.PREC_FP64 if prec == "fp64" else host.PREC_MIXED
+    s = _sys(100, 600, 32)
+    outs = []
+    for use_graph in (True, True, False):
+        say(f"[engine {prec}] run graph={use_graph}")
+        e = host.Engine([host.ReplicaSpec(s, temp=300.0, gamma_ps=1.0, seed=5)], precision=P)
+        try:
+            e.init()
+            e.minimize(100)
+            e.run(40, use_graph=use_graph)
+            x, v = e.state()[0]
+            outs.append((x, v, e.energies()[0], e.rebuilds()[0]))
+        finally:
+            e.close()
+    for k in (1, 2):
+        assert np.array_equal(outs[0][0], outs[k][0])     # bit-identical positions
+        assert np.array_equal(outs[0][1], outs[k][1])
+        assert outs[0][3] == outs[k][3]
+
+
+def test_replica_alone_equals_in_batch_mixed():
+    host = _engine()
+    a = host.ReplicaSpec(_sys(100, 600, 41), temp=300.0, gamma_ps=1.0, seed=11)
+    b = host.ReplicaSpec(_sys(120, 900, 42), temp=290.0, gamma_ps=1.0, seed=12)
+    c = host.ReplicaSpec(_sys(90, 700, 43), temp=310.0, gamma_ps=0.0, seed=13)
+
+    def run(reps):
+        e = host.Engine(reps, precision=host.PREC_MIXED)
+        try:
+            e.init()
+            e.run(60)
+            return e.state(), e.energies(), e.rebuilds()
+        finally:
+            e.close()
+
+    say("[engine] alone vs batch")
+    alone = [run([r]) for r in (a, b, c)]
+    mix = run([a, b, c])
+    for k in range(3):
+        xa, va = alone[k][0][0]
+        xm, vm = mix[0][k]
+        assert np.array_equal(xa, xm), f"replica {k} positions differ"
+        assert np.array_equal(va, vm), f"replica {k} velocities differ"
+        assert alone[k][2][0] == mix[2][k]
+        np.testing.assert_allclose(alone[k][1][0], mix[1][k], rtol=1e-12, atol=1e-9)
+
+
+def test_mixed_tracks_fp64_over_short_run():
+    host = _engine()
+    s = _sys(100, 600, 51)
+    res = {}
+    for name, P in (("fp64", host.PREC_FP64), ("mixed", host.PREC_MIXED)):
+        e = host.Engine([s], precision=P)
+        try:
+            e.init()
+            e.minimize(200)
+            x0, _ = e.state()[0]
+            e.run(10, use_graph=False)
+            res[name] = (x0, e.state()[0][0], e.energies()[0])
+        finally:
+            e.close()
+    assert np.array_equal(res["fp64"][0] * 0, res["mixed"][0] * 0)
+    dx = res["fp64"][1] - res["mixed"][1]
+    L = float(s.box)
+    dx -= L * np.round(dx / L)
+    assert np.abs(dx).max() < 0.05          # trajectories stay together over 10 steps
+    assert res["mixed"][2][:7].sum() == pytest.approx(res["fp64"][2][:7].sum(), rel=1e-3)
+
+
+def test_cluster_union_find_matches_cpu_bfs():
+    host = _engine()
+    s = _sys(100, 600, 61)
+    e = host.Engine([s], precision=host.PREC_MIXED)
+    try:
+        e.init()
+        e.run(5, use_graph=False)
+        big = e.largest_cluster(3.5)[0]
+        x, _ = e.state()[0]
+    finally:
+        e.close()
+    # CPU: molecules joined by an N(type 3)-O(types 4,5) contact < 3.5 A
+    L = float(s.box)
+    mol = s.mol
+    ns = np.where((s.types == 3) & (s.moltype == 1))[0]
+    os_ = np.where(np.isin(s.types, [4, 5]) & (s.moltype == 1))[0]
+    adj = {m: set() for m in range(s.n_gly)}
+    for i in ns:
+        d = x[os_] - x[i]
+        d -= L * np.round(d / L)
+        close = os_[np.sqrt((d * d).sum(1)) < 3.5]
+        for j in close:
+            if mol[j] != mol[i]:
+                adj[mol[i]].add(mol[j]); adj[mol[j]].add(mol[i])
+    seen, best = set(), 0
+    for m in range(s.n_gly):
+        if m in seen:
+            continue
+        stack, size = [m], 0
+        seen.add(m)
+        while stack:
+            u = stack.pop(); size += 1
+            for w in adj[u]:
+                if w not in seen:
+                    seen.add(w); stack.append(w)
+        best = max(best, size)
+    assert big == best
