@@ -200,6 +200,15 @@ int sofia_minimize(sofia_ctx *ctx, int32_t nsteps, double gamma, double max_disp
 int sofia_md_energies(sofia_ctx *ctx, double *out);
 /* Number of list builds per replica since md_init (nrep ints). */
 int sofia_md_rebuilds(sofia_ctx *ctx, int32_t *out);
+/* Per-replica step counters (nrep uint64), the Langevin RNG key; a replica moved into a
+ * new batch keeps its random stream by carrying its counter over. */
+int sofia_set_replica_steps(sofia_ctx *ctx, const uint64_t *steps);
+int sofia_get_replica_steps(sofia_ctx *ctx, uint64_t *steps);
+/* Times `launches` back-to-back launches of the engine's pair kernel with HIP events on
+ * the context stream (average ms per launch) and counts the list entries inside the
+ * force cutoff (the algorithmic pairs of one launch). */
+int sofia_profile_nonbond(sofia_ctx *ctx, int32_t launches, double *ms_per_launch,
+                          int64_t *pairs_in_cutoff);
 /* Tuning knobs (autotuner): lanes per atom in the pair kernel (4..64, power of two),
  * threads per block of the pair kernel, waves per block in the list build. */
 int sofia_set_tuning(sofia_ctx *ctx, int32_t lanes_per_atom, int32_t block, int32_t list_waves);

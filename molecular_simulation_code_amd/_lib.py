@@ -90,6 +90,9 @@ SIGNATURES = {
     "sofia_minimize": [_vp, _i32, _d, _d],
     "sofia_md_energies": [_vp, _vp],
     "sofia_md_rebuilds": [_vp, _vp],
+    "sofia_set_replica_steps": [_vp, _vp],
+    "sofia_get_replica_steps": [_vp, _vp],
+    "sofia_profile_nonbond": [_vp, _i32, _vp, _vp],
     "sofia_set_tuning": [_vp, _i32, _i32, _i32],
     "sofia_set_molecules": [_vp, _vp],
     "sofia_cluster_largest": [_vp, _f, _i32, _vp, _i32, _vp, _vp],

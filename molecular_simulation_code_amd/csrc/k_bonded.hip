@@ -58,10 +58,10 @@ __global__ void k_correct(int32_t items, int32_t width, int32_t colb, const int3
         vy = dy * dy * force;
         vz = dz * dz * force;
     }
-    wave_add(to_fix(e), live, r, eacc, E_CORR);
-    wave_add(to_fix(vx), live, r, eacc, V_X);
-    wave_add(to_fix(vy), live, r, eacc, V_Y);
-    wave_add(to_fix(vz), live, r, eacc, V_Z);
+    wave_add(to_efix(e), live, r, eacc, E_CORR);
+    wave_add(to_efix(vx), live, r, eacc, V_X);
+    wave_add(to_efix(vy), live, r, eacc, V_Y);
+    wave_add(to_efix(vz), live, r, eacc, V_Z);
 }
 
 // The 1-4 pair of dihedral_opls_DSF, Source/mod_force_dihed.f90:887-895 and 1003-1048:
@@ -127,8 +127,8 @@ __global__ void k_pair14(int32_t items, const int32_t *__restrict__ dl, const fl
             atomic_fix(&fout[(size_t)c * n + i4], -d[c] * force);
         }
     }
-    wave_add(to_fix(ec), live, r, eacc, E_COUL14);
-    wave_add(to_fix(el), live, r, eacc, E_POT14);
+    wave_add(to_efix(ec), live, r, eacc, E_COUL14);
+    wave_add(to_efix(el), live, r, eacc, E_POT14);
 }
 
 __device__ inline void dvec64(const double *xd, int n, int a, int b, double box, double ibox, double *o) {
@@ -164,7 +164,7 @@ __global__ void k_bond(int32_t items, const int32_t *__restrict__ bl, const doub
             atomic_fix(&fout[(size_t)c * n + i2], -d[c] * force);
         }
     }
-    wave_add(to_fix(e), live, r, eacc, E_BOND);
+    wave_add(to_efix(e), live, r, eacc, E_BOND);
 }
 
 // angle_harmonic, Source/mod_force_angle.f90:29-90: E = k (theta - theta0)^2
@@ -205,7 +205,7 @@ __global__ void k_angle(int32_t items, const int32_t *__restrict__ al, const dou
             atomic_fix(&fout[(size_t)k * n + i3], -v2);
         }
     }
-    wave_add(to_fix(e), live, r, eacc, E_ANGLE);
+    wave_add(to_efix(e), live, r, eacc, E_ANGLE);
 }
 
 // Torsion E = K (1 + cos(n phi - shift)) (the AMBER form of Source/mod_force_dihed_amber.f90)
@@ -256,7 +256,7 @@ __global__ void k_torsion(int32_t items, const int32_t *__restrict__ tl, const d
             }
         }
     }
-    wave_add(to_fix(e), live, r, eacc, E_DIHED);
+    wave_add(to_efix(e), live, r, eacc, E_DIHED);
 }
 
 }  // namespace

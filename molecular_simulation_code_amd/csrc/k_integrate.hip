@@ -53,7 +53,7 @@ __global__ void k_nve_v32(int32_t n, float dt_partial, const float4 *__restrict_
         }
         k = 0.5 * amu2e * m * (double)((vv[0] * vv[0] + vv[1] * vv[1]) + vv[2] * vv[2]);
     }
-    wave_add(to_fix(k), live, r, eacc, E_KE);
+    wave_add(to_efix(k), live, r, eacc, E_KE);
 }
 
 __global__ void k_nve_x32(int32_t n, float dt, const float *__restrict__ v, float4 *__restrict__ pos4,
@@ -273,9 +273,9 @@ __global__ void k_shake32(int32_t ns, const int32_t *__restrict__ num, const int
             }
         }
     }
-    wave_add(to_fix(vir[0]), live, rr, eacc, V_X);
-    wave_add(to_fix(vir[1]), live, rr, eacc, V_Y);
-    wave_add(to_fix(vir[2]), live, rr, eacc, V_Z);
+    wave_add(to_efix(vir[0]), live, rr, eacc, V_X);
+    wave_add(to_efix(vir[1]), live, rr, eacc, V_Y);
+    wave_add(to_efix(vir[2]), live, rr, eacc, V_Z);
 }
 
 // ------------------------------------------------------------------ engine (FP64 state)
@@ -328,7 +328,7 @@ __global__ void k_nve_v64(int32_t n, double dthalf, const float4 *__restrict__ p
         }
         k = 0.5 * amu2e * m * v2;
     }
-    if (ke) wave_add(to_fix(k), live, r, eacc, E_KE);
+    if (ke) wave_add(to_efix(k), live, r, eacc, E_KE);
 }
 
 // x += dt v, displacement += dt v, wrap into (0, box], refresh the float4 copy
@@ -366,7 +366,7 @@ __global__ void k_langevin64(int32_t n, const float4 *__restrict__ pos4, const d
     if (R.fric == 0.0 && R.gconst == 0.0) return;
     const double m = mass[__float_as_int(pos4[i].w) - 1];
     float u[3];
-    uniform3(R.seed, *step, (uint32_t)gid[i], u);
+    uniform3(R.seed, step[rep[i]], (uint32_t)gid[i], u);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const size_t o = (size_t)c * n + i;
@@ -427,9 +427,9 @@ __global__ void k_shake64(int32_t ns, const int32_t *__restrict__ num, const int
             }
         }
     }
-    wave_add(to_fix(vir[0]), live, rr, eacc, V_X);
-    wave_add(to_fix(vir[1]), live, rr, eacc, V_Y);
-    wave_add(to_fix(vir[2]), live, rr, eacc, V_Z);
+    wave_add(to_efix(vir[0]), live, rr, eacc, V_X);
+    wave_add(to_efix(vir[1]), live, rr, eacc, V_Y);
+    wave_add(to_efix(vir[2]), live, rr, eacc, V_Z);
 }
 
 // Capped steepest descent: every atom moves along its force by min(gamma |F|, max_disp).
@@ -466,7 +466,10 @@ __global__ void k_minimize_step(int32_t n, double gamma, double max_disp, double
     pos4[i] = p;
 }
 
-__global__ void k_step_advance(uint64_t *step) { *step += 1; }
+__global__ void k_step_advance(int32_t nrep, uint64_t *step) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrep) step[r] += 1;
+}
 
 }  // namespace
 
@@ -557,7 +560,8 @@ void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp) {
 }
 
 void launch_step_advance(sofia_ctx *c) {
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, c->stream, c->step_d);
+    hipLaunchKernelGGL(k_step_advance, dim3(nblocks(c->nrep, 256)), dim3(256), 0, c->stream, c->nrep,
+                       c->step_d);
 }
 
 }  // namespace sofia

@@ -197,11 +197,11 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
         vx = group_sum<L>(vx);
         vy = group_sum<L>(vy);
         vz = group_sum<L>(vz);
-        wave_add(to_fix(0.5 * pot), owner, r, eacc, E_LJ);
-        wave_add(to_fix(0.5 * ecl), owner, r, eacc, E_COUL);
-        wave_add(to_fix(0.5 * vx), owner, r, eacc, V_X);
-        wave_add(to_fix(0.5 * vy), owner, r, eacc, V_Y);
-        wave_add(to_fix(0.5 * vz), owner, r, eacc, V_Z);
+        wave_add(to_efix(0.5 * pot), owner, r, eacc, E_LJ);
+        wave_add(to_efix(0.5 * ecl), owner, r, eacc, E_COUL);
+        wave_add(to_efix(0.5 * vx), owner, r, eacc, V_X);
+        wave_add(to_efix(0.5 * vy), owner, r, eacc, V_Y);
+        wave_add(to_efix(0.5 * vz), owner, r, eacc, V_Z);
     }
 }
 
@@ -241,6 +241,37 @@ void launch_nonbond_out(sofia_ctx *c, int mode, bool energies, float *ff32) {
 
 void launch_nonbond(sofia_ctx *c, int mode, bool energies) {
     launch_nonbond_out(c, mode, energies, nullptr);
+}
+
+// Number of list entries inside the force cutoff (the algorithmic pair count of a launch).
+__global__ void k_count_pairs(int32_t n, const float4 *__restrict__ pos4, const int32_t *__restrict__ rep,
+                              const RepInfo *__restrict__ reps, const int32_t *__restrict__ nlist,
+                              const int32_t *__restrict__ numneigh, int32_t na, float rc2,
+                              unsigned long long *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cnt = 0;
+    if (i < n) {
+        const float4 xi = pos4[i];
+        const RepInfo &R = reps[rep[i]];
+        const int32_t *lst = nlist + (size_t)i * na;
+        for (int k = 0; k < numneigh[i]; ++k) {
+            const float4 pj = pos4[lst[k]];
+            float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
+            dx -= R.box * __builtin_rintf(dx * R.ibox);
+            dy -= R.box * __builtin_rintf(dy * R.ibox);
+            dz -= R.box * __builtin_rintf(dz * R.ibox);
+            cnt += (dx * dx + dy * dy + dz * dz <= rc2) ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, cnt);
+}
+
+void launch_count_pairs(sofia_ctx *c, unsigned long long *out) {
+    const float rc2 = (float)(c->dp.rcut2 > c->dp.cut_coulsq ? c->dp.rcut2 : c->dp.cut_coulsq);
+    hipLaunchKernelGGL(k_count_pairs, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N,
+                       c->a.pos4, c->rep, c->reps_d, c->nlist, c->numneigh, c->neigh_alloc, rc2, out);
 }
 
 }  // namespace sofia

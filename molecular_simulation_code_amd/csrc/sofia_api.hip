@@ -250,7 +250,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     TRY(dalloc(&c->shake_err, 1));
     TRY(dalloc(&c->shake_iters, 1));
     TRY(dalloc(&c->eacc, (size_t)nrep * NE));
-    TRY(dalloc(&c->step_d, 1));
+    TRY(dalloc(&c->step_d, nrep));
     TRY(dalloc(&c->reps_d, nrep));
     SOFIA_HIP(hipMemset(c->dr32, 0, sizeof(float) * 3 * (size_t)n));
     SOFIA_HIP(hipMemset(c->drd, 0, sizeof(double) * 3 * (size_t)n));
@@ -259,7 +259,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     SOFIA_HIP(hipMemset(c->rebuild, 0, sizeof(int32_t) * nrep));
     SOFIA_HIP(hipMemset(c->nbuilds, 0, sizeof(int32_t) * nrep));
     SOFIA_HIP(hipMemset(c->numneigh, 0, sizeof(int32_t) * n));
-    SOFIA_HIP(hipMemset(c->step_d, 0, sizeof(uint64_t)));
+    SOFIA_HIP(hipMemset(c->step_d, 0, sizeof(uint64_t) * nrep));
     std::vector<int32_t> rep(n), cnum((size_t)27 * c->ncell_total), crep(c->ncell_total);
     for (int r = 0; r < nrep; ++r) {
         const sofia::RepInfo &R = c->reps_h[r];
@@ -297,7 +297,7 @@ int read_energies(sofia_ctx *c, std::vector<long long> &e) {
     return sync_check(c);
 }
 
-double E(const std::vector<long long> &e, int r, int slot) { return (double)e[(size_t)r * NE + slot] * FIX_INV; }
+double E(const std::vector<long long> &e, int r, int slot) { return (double)e[(size_t)r * NE + slot] * EFIX_INV; }
 
 // Full sort + list build of every flagged replica; grow-and-redo on overflow.
 int rebuild_now(sofia_ctx *c, bool engine, bool reference_growth) {
@@ -965,6 +965,49 @@ int sofia_minimize(sofia_ctx *c, int32_t nsteps, double gamma, double max_disp) 
     TRY(collect_energies(c));
     if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     return check_engine_flags(c);
+}
+
+int sofia_set_replica_steps(sofia_ctx *c, const uint64_t *steps) {
+    if (!c || !steps) return fail(SOFIA_E_ARG, "null argument");
+    SOFIA_HIP(hipMemcpyAsync(c->step_d, steps, sizeof(uint64_t) * c->nrep, hipMemcpyHostToDevice, c->stream));
+    return sync_check(c);
+}
+
+int sofia_get_replica_steps(sofia_ctx *c, uint64_t *steps) {
+    if (!c || !steps) return fail(SOFIA_E_ARG, "null argument");
+    SOFIA_HIP(hipMemcpyAsync(steps, c->step_d, sizeof(uint64_t) * c->nrep, hipMemcpyDeviceToHost, c->stream));
+    return sync_check(c);
+}
+
+int sofia_profile_nonbond(sofia_ctx *c, int32_t launches, double *ms_per_launch,
+                          int64_t *pairs_in_cutoff) {
+    if (!c || launches < 1) return fail(SOFIA_E_ARG, "bad argument");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no Verlet list");
+    hipEvent_t t0, t1;
+    SOFIA_HIP(hipEventCreate(&t0));
+    SOFIA_HIP(hipEventCreate(&t1));
+    SOFIA_HIP(hipEventRecord(t0, c->stream));
+    for (int k = 0; k < launches; ++k) launch_nonbond(c, nb_mode(c), false);
+    SOFIA_HIP(hipEventRecord(t1, c->stream));
+    SOFIA_HIP(hipEventSynchronize(t1));
+    float ms = 0.0f;
+    SOFIA_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (ms_per_launch) *ms_per_launch = (double)ms / launches;
+    if (pairs_in_cutoff) {
+        unsigned long long *d = nullptr;
+        TRY(dalloc(&d, 1));
+        SOFIA_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), c->stream));
+        launch_count_pairs(c, d);
+        unsigned long long h = 0;
+        SOFIA_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        TRY(sync_check(c));
+        dfree(d);
+        *pairs_in_cutoff = (int64_t)h;
+    }
+    // the profiled launches overwrote the nonbond forces with the same values; nothing to restore
+    return sync_check(c);
 }
 
 int sofia_md_energies(sofia_ctx *c, double *out) {

@@ -32,6 +32,7 @@ __device__ inline T group_sum(T v) {
 }
 
 __device__ inline long long to_fix(double v) { return __double2ll_rn(v * FIX_SCALE); }
+__device__ inline long long to_efix(double v) { return __double2ll_rn(v * EFIX_SCALE); }
 
 // Add an int64 contribution to a per-replica accumulator slot.  One atomic per wave when
 // every contributing lane belongs to the same replica, per-lane atomics otherwise;

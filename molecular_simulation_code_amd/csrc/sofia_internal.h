@@ -12,8 +12,10 @@
 namespace sofia {
 
 constexpr int SPEC = 12;                     // specbond slots per atom
-constexpr double FIX_SCALE = 4294967296.0;   // 2^32 units per kcal/mol/A and per kcal/mol
+constexpr double FIX_SCALE = 4294967296.0;   // forces: 2^32 units per kcal/mol/A
 constexpr double FIX_INV = 1.0 / FIX_SCALE;
+constexpr double EFIX_SCALE = 16777216.0;    // energies and virial: 2^24 units per kcal/mol
+constexpr double EFIX_INV = 1.0 / EFIX_SCALE;
 constexpr float PAIR_QUANT = 16777216.0f;    // 2^24: per-pair quantum of the mixed kernel
 constexpr int PAIR_SHIFT = 8;                // 2^32 / 2^24
 
@@ -130,7 +132,7 @@ struct sofia_ctx {
 
     // engine
     bool engine_ready = false;
-    uint64_t *step_d = nullptr;  // device step counter (Langevin key)
+    uint64_t *step_d = nullptr;  // per-replica step counters (Langevin key)
     uint64_t step_h = 0;
     int lanes = 16, block = 256, list_waves = 4;
     hipGraphExec_t graph = nullptr;
@@ -159,6 +161,7 @@ void launch_finish_rebuild(sofia_ctx *c, bool engine);
 
 // nonbond (k_nonbond.hip): mode 0 = DSF FP64, 1 = DSF mixed, 2 = Ewald real space (FP32)
 void launch_nonbond(sofia_ctx *c, int mode, bool energies);
+void launch_count_pairs(sofia_ctx *c, unsigned long long *out);
 
 // bonded (k_bonded.hip)
 void launch_correct(sofia_ctx *c, int which, long long *fout);

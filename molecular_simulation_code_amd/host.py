@@ -357,20 +357,26 @@ class Engine:
         check(lib().sofia_ctx_create_batch(C.byref(self.params), len(reps), ptr(n), ptr(boxes), C.byref(h)))
         self.ctx = h
         self._keep = _forcefield(h)
-        cat = lambda f: np.concatenate([f(r.system) for r in reps])
+        # every array handed to the C ABI is bound to a name for the duration of the call
+        def cat(f, dtype):
+            return np.ascontiguousarray(np.concatenate([f(r.system) for r in reps]), dtype=dtype)
+
         a = np.empty((self.N, 4), f32)
-        a[:, :3] = cat(lambda s: s.pos)
-        a[:, 3] = cat(lambda s: s.types).astype(i32).view(f32)
-        check(lib().sofia_put_state(h, ptr(a), ptr(_c(cat(lambda s: s.q), f32)),
-                                    ptr(_c(cat(lambda s: s.v), f32)),
-                                    ptr(_c(cat(lambda s: s.global_id), i32))))
-        check(lib().sofia_set_exclusions(h, ptr(_c(cat(lambda s: s.num1bond), i32)),
-                                         ptr(_c(cat(lambda s: s.num2bond), i32)),
-                                         ptr(_c(cat(lambda s: s.num3bond), i32)),
-                                         ptr(_c(cat(lambda s: s.specbond), i32))))
+        a[:, :3] = cat(lambda s: s.pos, f32)
+        a[:, 3] = cat(lambda s: s.types, i32).view(f32)
+        q = cat(lambda s: s.q, f32)
+        v = cat(lambda s: s.v, f32)
+        gid = cat(lambda s: s.global_id, i32)
+        check(lib().sofia_put_state(h, ptr(a), ptr(q), ptr(v), ptr(gid)))
+        n1 = cat(lambda s: s.num1bond, i32)
+        n2 = cat(lambda s: s.num2bond, i32)
+        n3 = cat(lambda s: s.num3bond, i32)
+        sp = cat(lambda s: s.specbond, i32)
+        check(lib().sofia_set_exclusions(h, ptr(n1), ptr(n2), ptr(n3), ptr(sp)))
         per = lambda f: _c([len(f(r.system)) for r in reps], i32)
-        bl = _c(cat(lambda s: s.bondlist), i32); al = _c(cat(lambda s: s.anglelist), i32)
-        dl = _c(cat(lambda s: s.dihedlist), i32)
+        bl = cat(lambda s: s.bondlist, i32)
+        al = cat(lambda s: s.anglelist, i32)
+        dl = cat(lambda s: s.dihedlist, i32)
         nb, na, nd = per(lambda s: s.bondlist), per(lambda s: s.anglelist), per(lambda s: s.dihedlist)
         check(lib().sofia_set_bonded(h, ptr(bl), len(bl), ptr(nb), ptr(al), len(al), ptr(na),
                                      ptr(dl), len(dl), ptr(nd)))
@@ -378,11 +384,13 @@ class Engine:
             tl = [torsion_list(r.system) for r in reps]
             self._keep += _bonded_coeffs(h, np.concatenate(tl) if tl else np.zeros((0, 5), i32),
                                          [len(t) for t in tl])
-        sn = _c(cat(lambda s: s.shake_num), i32); sa = _c(cat(lambda s: s.shake_atoms), i32)
-        sb = _c(cat(lambda s: s.shake_bond), f64); ns = per(lambda s: s.shake_num)
+        sn = cat(lambda s: s.shake_num, i32)
+        sa = cat(lambda s: s.shake_atoms, i32)
+        sb = cat(lambda s: s.shake_bond, f64)
+        ns = per(lambda s: s.shake_num)
         check(lib().sofia_set_shake(h, len(sn), ptr(ns), ptr(sn), ptr(sa), ptr(sb)))
-        mol = cat(lambda s: np.where(s.moltype == 1, s.mol, -1).astype(i32))
-        check(lib().sofia_set_molecules(h, ptr(_c(mol, i32))))
+        mol = cat(lambda s: np.where(s.moltype == 1, s.mol, -1), i32)
+        check(lib().sofia_set_molecules(h, ptr(mol)))
         th = (SofiaThermostat * len(reps))(*[SofiaThermostat(r.temp, r.gamma_ps, r.seed) for r in reps])
         check(lib().sofia_set_thermostat(h, th))
         if tuning:

@@ -377,45 +377,27 @@ def make_system(n_gly: int, n_wat: int, seed: int = 20261202, params: Params | N
         grid = np.stack([idx % nl, (idx // nl) % nl, idx // (nl * nl)], axis=1)
         return grid, (grid + 0.5) * a, a
 
-    R_gly = _random_rotations(rng, n_gly)
-    gly_xyz = np.zeros((0, 3))
-    nl = int(math.ceil((n_wat + 10 * n_gly) ** (1.0 / 3.0)))
-    for _attempt in range(6):
-        grid, centres, a = lattice(nl)
-        stride = 3 if (nl // 3) ** 3 >= n_gly else 2
-        sub = np.where(np.all(grid % stride == 0, axis=1))[0]
-        if n_gly > len(sub):
-            nl += 1
-            continue
-        gsites = rng.choice(sub, size=n_gly, replace=False) if n_gly else np.zeros(0, int)
-        gc = centres[gsites] + rng.uniform(-jitter, jitter, size=(n_gly, 3))
-        # glycines: re-draw orientations of clashing pairs
-        for _ in range(60):
-            gly_xyz = (gc[:, None, :] + np.einsum("nij,aj->nai", R_gly, GLYCINE.geom)).reshape(-1, 3)
-            if n_gly < 2:
-                break
-            tree = cKDTree(np.mod(gly_xyz, L), boxsize=L)
-            pr = tree.query_pairs(2.4, output_type="ndarray")
-            molid = np.repeat(np.arange(n_gly), 10)
-            pr = pr[molid[pr[:, 0]] != molid[pr[:, 1]]] if len(pr) else pr
-            if len(pr) == 0:
-                break
-            bad = np.unique(molid[pr[:, 0]])
-            R_gly[bad] = _random_rotations(rng, len(bad))
-        free = np.ones(len(centres), bool)
-        if n_gly:
-            free[gsites] = False
-            tree = cKDTree(np.mod(gly_xyz, L), boxsize=L)
-            near = tree.query_ball_point(np.mod(centres, L), r=2.9, return_length=True)
-            free &= near == 0
-        if free.sum() >= n_wat:
-            break
-        nl += 1
-    else:
-        raise ValueError("could not fit the waters on the lattice")
-    wsites = rng.choice(np.where(free)[0], size=n_wat, replace=False)
-    wc = centres[wsites] + rng.uniform(-jitter, jitter, size=(n_wat, 3))
-    R_w = _random_rotations(rng, n_wat)
+    # lattice with one site per water and ~5 per glycine (its volume is ~4 waters)
+    nl = max(3, int(math.ceil((n_wat + 5 * n_gly) ** (1.0 / 3.0))))
+    grid, centres, a = lattice(nl)
+    stride = 3 if (nl // 3) ** 3 >= n_gly else 2
+    sub = np.where(np.all(grid % stride == 0, axis=1))[0]
+    if n_gly > len(sub):
+        raise ValueError("too many glycines for the lattice")
+    gsites = rng.choice(sub, size=n_gly, replace=False) if n_gly else np.zeros(0, int)
+    free = np.ones(len(centres), bool)
+    free[gsites] = False
+    if n_gly:
+        # the 4 nearest lattice neighbours of each glycine site are its own volume
+        tree = cKDTree(centres, boxsize=L)
+        _, nb = tree.query(centres[gsites], k=5)
+        free[nb.ravel()] = False
+    fidx = np.where(free)[0]
+    if n_wat > len(fidx):
+        raise ValueError("lattice too small for the waters")
+    wsites = rng.choice(fidx, size=n_wat, replace=False)
+    C = np.concatenate([centres[gsites], centres[wsites]]) + rng.uniform(-jitter, jitter, (nmol, 3))
+    R = _random_rotations(rng, nmol)
 
     natom_mol = np.array([10] * n_gly + [3] * n_wat)
     offsets = np.concatenate([[0], np.cumsum(natom_mol)])
@@ -424,33 +406,41 @@ def make_system(n_gly: int, n_wat: int, seed: int = 20261202, params: Params | N
     types = np.concatenate([np.array(GLYCINE.types * n_gly, np.int32),
                             np.array(WATER.types * n_wat, np.int32)]) if nmol else np.zeros(0, np.int32)
     is_heavy = np.isin(types, heavy_types)
+    local = np.concatenate([np.tile(GLYCINE.geom, (n_gly, 1)), np.tile(WATER.geom, (n_wat, 1))])
 
     def place():
-        w = (wc[:, None, :] + np.einsum("nij,aj->nai", R_w, WATER.geom)).reshape(-1, 3)
-        return np.concatenate([gly_xyz, w]) if n_gly else w
+        return C[mol_of_atom] + np.einsum("nij,nj->ni", R[mol_of_atom], local)
 
+    # rigid-body overlap removal: molecules in contact are pushed apart along the contact
+    # (heavy-heavy below 2.5 A, any pair below 1.7 A); clashing waters are also re-rotated
     xyz = place()
-    for _ in range(80):
+    for it in range(150):
         tree = cKDTree(np.mod(xyz, L), boxsize=L)
-        pr = tree.query_pairs(2.4, output_type="ndarray")
+        pr = tree.query_pairs(2.5, output_type="ndarray")
         if len(pr):
-            inter = mol_of_atom[pr[:, 0]] != mol_of_atom[pr[:, 1]]
-            pr = pr[inter]
-        if len(pr):
-            dv = xyz[pr[:, 0]] - xyz[pr[:, 1]]
-            dv -= L * np.round(dv / L)
-            dd = np.sqrt((dv * dv).sum(1))
-            clash = (is_heavy[pr[:, 0]] & is_heavy[pr[:, 1]]) | (dd < 1.5)
-            pr = pr[clash]
+            pr = pr[mol_of_atom[pr[:, 0]] != mol_of_atom[pr[:, 1]]]
         if len(pr) == 0:
             break
-        bad = np.unique(mol_of_atom[pr.ravel()])
-        bad_w = bad[bad >= n_gly] - n_gly
-        if len(bad_w) == 0:
+        dv = xyz[pr[:, 0]] - xyz[pr[:, 1]]
+        dv -= L * np.round(dv / L)
+        dd = np.sqrt((dv * dv).sum(1))
+        thr = np.where(is_heavy[pr[:, 0]] & is_heavy[pr[:, 1]], 2.5, 1.7)
+        hit = dd < thr
+        if not hit.any():
             break
-        R_w[bad_w] = _random_rotations(rng, len(bad_w))
-        # nudge stubborn waters a little as well
-        wc[bad_w] += rng.uniform(-0.15, 0.15, size=(len(bad_w), 3))
+        pr, dv, dd, thr = pr[hit], dv[hit], dd[hit], thr[hit]
+        u = dv / np.maximum(dd, 1e-3)[:, None]
+        push = (0.5 * (thr - dd) + 0.02)[:, None] * u
+        D = np.zeros((nmol, 3))
+        np.add.at(D, mol_of_atom[pr[:, 0]], push)
+        np.add.at(D, mol_of_atom[pr[:, 1]], -push)
+        norm = np.linalg.norm(D, axis=1, keepdims=True)
+        D *= np.minimum(1.0, 0.3 / np.maximum(norm, 1e-12))
+        C += D
+        if it % 4 == 3:
+            bad = np.unique(mol_of_atom[pr.ravel()])
+            bad = bad[bad >= n_gly]
+            R[bad] = _random_rotations(rng, len(bad))
         xyz = place()
 
     # wrap into (0, box] as pbc() does (Source/mod_pbc.f90:6-39)

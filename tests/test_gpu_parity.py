@@ -115,9 +115,10 @@ def test_dsf_fp64(case, gpu_sys):
     g.lj_cut_coul_dsf_nonewton(0)
     assert rel_rms(g.ff64, ff64) < 1e-12          # FP64 arithmetic: far inside 1e-5
     assert rel_rms(g.ff, ff) < FORCE_RTOL
-    assert g.potential == pytest.approx(pot, rel=1e-10)
-    assert g.e_coul == pytest.approx(ec, rel=1e-10)
-    np.testing.assert_allclose(g.virial, vir, rtol=1e-9)
+    # energies are summed in int64 units of 2^-24 kcal/mol: one quantum per atom at most
+    assert g.potential == pytest.approx(pot, rel=1e-7)
+    assert g.e_coul == pytest.approx(ec, rel=1e-7)
+    np.testing.assert_allclose(g.virial, vir, rtol=1e-7)
 
 
 def test_dsf_mixed(case, gpu_sys):
@@ -191,7 +192,7 @@ def test_integrator_pbc_shake_langevin(case):
         v_ref, ke_ref = O.nve_v(d["dthalf"], s.pos, s.types, mass, s.v, g.ff, float(S.AMU2E))
         g.nve_v(d["dthalf"])
         assert np.array_equal(g.v, v_ref)
-        assert g.ke == pytest.approx(ke_ref, rel=1e-12)
+        assert g.ke == pytest.approx(ke_ref, rel=1e-7)
         say(f"[gpu {name}] nve_x + pbc")
         x_ref, drx, dry, drz = O.nve_x(d["dt"], g.v, g.position, g.type, g.drx, g.dry, g.drz)
         g.nve_x()
