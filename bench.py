@@ -1,0 +1,316 @@
+#!/usr/bin/env python
+"""Throughput benchmark of the MI355X hot path (one JSON line on stdout).
+
+Metric: aggregate atom-steps/s of batched replicas vs the plain one-system-per-launch
+path, at a mixed-precision force error <= 1e-5 (relative RMS vs FP64), plus p50/p99
+latency of served MD-segment requests.
+
+* plain   -- 1,200 glycine + 64,000 water (204,000 atoms), FP64 pair arithmetic, FP64
+             state, one system per launch.
+* batched -- 64 replicas, 5,000-60,000 atoms (log-uniform, seeded), glycine mole
+             fraction 0.01-0.04, FP32 pair arithmetic with int64 fixed-point sums, FP64
+             state; all replicas advance in one launch sequence (hipGraph-captured step).
+  `value` is the batched aggregate; a step = one velocity-Verlet step of every replica
+  (list maintenance, LJ+DSF pair forces, 1-4 pairs, bonded terms, Langevin, SHAKE).
+* served  -- Poisson arrivals of 500-step requests (3,000-150,000 atoms, 280-320 K),
+             continuous batching, union-find cluster analysis every 100 steps.
+* cpu_baseline -- the C oracle (OpenMP) on the 13,500-atom system, same step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Matom-steps/s batched vs plain @ force rel.err≤1e-5; p50/p99 segment ms"
+FLOP_PER_PAIR = 42.0          # per ordered in-cutoff list entry (84 per unordered pair)
+PEAK_FP32 = 157.3e12          # MI355X FP32 vector/matrix peak, FLOP/s
+PEAK_FP64 = 78.6e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ensure_built():
+    from molecular_simulation_code_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from molecular_simulation_code_amd.build import build
+        build()
+    oracle_lib = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(oracle_lib):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def _make(args):
+    from molecular_simulation_code_amd import system as S
+    g, w, seed = args
+    return S.make_system(g, w, seed=seed)
+
+
+def make_systems(specs, workers):
+    if workers > 1 and len(specs) > 1:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(workers) as pool:
+            return pool.map(_make, specs)
+    return [_make(s) for s in specs]
+
+
+def time_engine(eng, steps, warmup):
+    eng.run(max(1, warmup))
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.run(steps)
+    eng.synchronize()
+    return time.perf_counter() - t0
+
+
+def force_error_mixed_vs_fp64(system):
+    """Relative RMS of the mixed-precision pair forces against the FP64 kernel."""
+    from molecular_simulation_code_amd.host import SofiaGPU
+    g = SofiaGPU(system)
+    try:
+        g.neighbor_wrapper(0)
+        g.lj_cut_coul_dsf_nonewton(0)
+        f64 = g.ff64.copy()
+        g.lj_cut_coul_dsf_nonewton_mixed(0)
+        fm = g.ff64.copy()
+    finally:
+        g.close()
+    return float(np.sqrt(((fm - f64) ** 2).sum() / (f64 ** 2).sum()))
+
+
+def cpu_baseline(system, budget_s=12.0):
+    """Oracle MD steps (list rebuild when triggered, DSF pair forces, 1-4 pairs, bonded,
+    SHAKE, velocity Verlet) on the host cores; returns (atom-steps/s, steps, threads)."""
+    from oracle import oracle as O
+    from molecular_simulation_code_amd import system as S
+    s = system.copy()
+    d = s.derived()
+    prm = O.params_from(d, S.NTYPES)
+    lj = S.lj_tables()
+    mass = S.type_mass()
+    kb = np.array([c[0] for c in S.BOND_COEFF]); r0 = np.array([c[1] for c in S.BOND_COEFF])
+    ka = np.array([c[0] for c in S.ANGLE_COEFF]); th = np.radians([c[1] for c in S.ANGLE_COEFF])
+    rn, nT, nD = O.cell_grid(d["box"], d["rcut"], d["cut_coul"])
+    cnum = O.neighcell(nT, nD)
+    st = {}
+
+    def rebuild():
+        tt, perm, start, endp, atombin = O.sort_cells(s.pos, s.types, d["box"], rn, nT, nD)
+        inv = perm - 1
+        s.pos, s.types, s.q, s.v = s.pos[inv], s.types[inv], s.q[inv], s.v[inv]
+        sp = s.specbond[inv]
+        s.specbond = np.where(sp > 0, tt[np.maximum(sp, 1) - 1], 0).astype(np.int32)
+        s.num3bond = s.num3bond[inv]
+        s.bondlist[:, :2] = tt[s.bondlist[:, :2] - 1]
+        s.anglelist[:, :3] = tt[s.anglelist[:, :3] - 1]
+        s.dihedlist[:, :4] = tt[s.dihedlist[:, :4] - 1]
+        sa = s.shake_atoms
+        s.shake_atoms = np.where(sa > 0, tt[np.maximum(sa, 1) - 1], 0).astype(np.int32)
+        st["nl"], st["nn"], st["na"] = O.build_neighbor(s.pos, s.types, atombin, start, endp, cnum,
+                                                        s.num3bond, s.specbond, d["box"], d["ibox"],
+                                                        d["rv2"], st.get("na", S.NEIGH_ALLOC0))
+        st["dr"] = [np.zeros(s.np, np.float32) for _ in range(3)]
+
+    def forces():
+        ff, _, _, _, _ = O.dsf(s.pos, s.types, s.q, st["nl"], st["nn"], st["na"], lj, prm)
+        fh, _, _ = O.pair14_dsf(s.dihedlist, s.pos, s.types, s.q, lj, prm)
+        fb, _ = O.bond_harmonic(s.bondlist, s.pos, s.types, kb, r0, d["box"], d["ibox"])
+        fa, _ = O.angle_harmonic(s.anglelist, s.pos, s.types, ka, th, d["box"], d["ibox"])
+        f = (ff.astype(np.float64) + fh + fb + fa).astype(np.float32)
+        f, _, _, _ = O.shake(1, s.pos, s.types, s.v, f, mass, d["dt"], S.KCAL2AMU, d["box"],
+                             d["ibox"], s.shake_num, s.shake_atoms, s.shake_bond,
+                             float(S.SHAKE_TOL), S.SHAKE_MAXITER)
+        return f
+
+    def step(f):
+        s.v, _ = O.nve_v(d["dthalf"], s.pos, s.types, mass, s.v, f, float(S.AMU2E))
+        s.pos, *st["dr"] = O.nve_x(d["dt"], s.v, s.pos, s.types, *st["dr"])
+        s.pos = O.pbc(s.pos, s.types, d["box"])
+        if O.check_neighbor(*st["dr"], d["rvrc2"]):
+            rebuild()
+        f = forces()
+        s.v, _ = O.nve_v(d["dthalf"], s.pos, s.types, mass, s.v, f, float(S.AMU2E))
+        return f
+
+    rebuild()
+    f = forces()
+    f = step(f)                                   # warm-up step
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        f = step(f)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 200:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return s.np * n / el, n, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--minimize", type=int, default=150)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); replicas shard "
+              "across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    ensure_built()
+    from molecular_simulation_code_amd import system as S
+    workers = min(16, os.cpu_count() or 1)
+    K, W = args.steps, args.warmup
+
+    # ---------------------------------------------------------------- inputs (CPU, before any GPU use)
+    t = time.perf_counter()
+    comps = S.replica_sizes(args.replicas)
+    specs_cpu = [(1200, 64000, 20261203)] + [(g, w, 20261210 + k) for k, (g, w) in enumerate(comps)]
+    pool_sizes = []
+    if not args.no_serve:
+        lo, hi = 3000, 150000
+        pool_sizes = [int(math.exp(x)) for x in np.linspace(math.log(lo), math.log(hi), 10)]
+        prng = np.random.default_rng(20261206)
+        for n in pool_sizes:
+            x = prng.uniform(0.01, 0.04)
+            g = int(round(x * n / (10 * x + 3 * (1 - x))))
+            specs_cpu.append((g, int(round((n - 10 * g) / 3)), int(prng.integers(1 << 31))))
+    specs_cpu.append((150, 4000, 20261202))
+    log(f"[inputs] generating {len(specs_cpu)} systems with {workers} workers")
+    systems = make_systems(specs_cpu, workers)
+    plain_sys, reps = systems[0], systems[1:1 + len(comps)]
+    pool = systems[1 + len(comps):1 + len(comps) + len(pool_sizes)]
+    small = systems[-1]
+    log(f"  done in {time.perf_counter() - t:.1f}s")
+
+    from molecular_simulation_code_amd import _lib, host
+    from molecular_simulation_code_amd import server as SV
+    if _lib.device_count() < 1:
+        log("no HIP device")
+        sys.exit(3)
+
+    # ---------------------------------------------------------------- plain: 204k FP64
+    log(f"[plain] {plain_sys.np} atoms, FP64")
+    eng = host.Engine([host.ReplicaSpec(plain_sys, temp=300.0, gamma_ps=1.0, seed=1)],
+                      precision=host.PREC_FP64)
+    eng.init()
+    eng.minimize(args.minimize)
+    dt_plain = time_engine(eng, K, W)
+    plain_rate = plain_sys.np * K / dt_plain
+    ms_p, pairs_p = eng.profile_nonbond(10)
+    log(f"[plain] {plain_rate / 1e6:.1f} Matom-steps/s  ({1e3 * dt_plain / K:.3f} ms/step, "
+        f"pair kernel {ms_p:.3f} ms, {pairs_p / plain_sys.np:.1f} pairs/atom in cutoff)")
+    eng.close()
+
+    # ---------------------------------------------------------------- batched: 64 replicas mixed
+    natoms = sum(r.np for r in reps)
+    log(f"[batched] {len(reps)} replicas, {natoms} atoms, mixed precision")
+    rng = np.random.default_rng(5)
+    specs = [host.ReplicaSpec(r, temp=float(rng.uniform(280, 320)), gamma_ps=1.0, seed=100 + k)
+             for k, r in enumerate(reps)]
+    eng = host.Engine(specs, precision=host.PREC_MIXED)
+    eng.init()
+    eng.minimize(args.minimize)
+    dt_b = time_engine(eng, K, W)
+    batched_rate = natoms * K / dt_b
+    ms_b, pairs_b = eng.profile_nonbond(10)
+    rebuilds = eng.rebuilds()
+    log(f"[batched] {batched_rate / 1e6:.1f} Matom-steps/s ({1e3 * dt_b / K:.3f} ms/step, "
+        f"pair kernel {ms_b:.3f} ms)")
+    eng.close()
+
+    # force error of the mixed kernel against FP64 on a replica-sized system
+    ferr = force_error_mixed_vs_fp64(reps[0])
+    log(f"[precision] mixed vs FP64 force rel RMS = {ferr:.2e}")
+
+    # ---------------------------------------------------------------- served stream
+    p50 = p99 = served_rate = None
+    nreq = 0
+    if not args.no_serve:
+        log(f"[serve] relaxing a pool of {len(pool)} systems")
+        SV.relax(pool, args.minimize)
+        reqs = SV.poisson_stream(pool, args.serve_rate, args.serve_seconds)
+        nreq = len(reqs)
+        srv = SV.SegmentServer(segment=100)
+        res, wall, atom_steps = srv.serve(reqs)
+        lat = np.array([r.latency_ms for r in res])
+        p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+        served_rate = atom_steps / wall
+        log(f"[serve] {nreq} requests, p50 {p50:.0f} ms, p99 {p99:.0f} ms, "
+            f"{served_rate / 1e6:.1f} Matom-steps/s, {srv.rebatches} re-batches")
+
+    # ---------------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        rate, nsteps, threads = cpu_baseline(small, args.cpu_seconds)
+        cpu = {"value": rate / 1e6, "unit": "Matom-steps/s", "cores": threads, "kind": "port",
+               "sample": f"C oracle (OpenMP pair and list loops), 13,500-atom glycine/water, "
+                         f"{nsteps} full MD steps"}
+        log(f"[cpu] {rate / 1e6:.3f} Matom-steps/s on {threads} threads")
+
+    flops = FLOP_PER_PAIR * pairs_b
+    achieved = flops / (ms_b * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": round(batched_rate / 1e6, 3),
+        "unit": "Matom-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1e3 * dt_b / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched: {len(reps)} glycine/water replicas, 5,000-60,000 atoms "
+                        f"log-uniform ({natoms} atoms), FP32 pair math + int64 fixed-point "
+                        "sums, FP64 state, Langevin 1/ps, SHAKE, dt 2 fs, hipGraph steps",
+            "replicas": len(reps), "atoms": natoms,
+            "plain": {"workload": "204,000-atom glycine/water, FP64 pair math and state",
+                      "value": round(plain_rate / 1e6, 3), "unit": "Matom-steps/s",
+                      "ms_per_step": round(1e3 * dt_plain / K, 4), "dtype": "f64"},
+            "served": {"workload": f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g}s, "
+                                   "500-step requests, 3,000-150,000 atoms",
+                       "requests": nreq, "p50_ms": p50, "p99_ms": p99,
+                       "value": None if served_rate is None else round(served_rate / 1e6, 3)},
+        },
+        "batched_over_plain": round(batched_rate / plain_rate, 3),
+        "plain_over_cpu": None if cpu is None else round(plain_rate / 1e6 / cpu["value"], 1),
+        "p50_ms": p50,
+        "p99_ms": p99,
+        "force_rel_rms_mixed_vs_fp64": ferr,
+        "list_rebuilds_per_replica_mean": float(np.mean(rebuilds)),
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "k_nonbond<mixed> pair kernel", "ms_per_launch": ms_b,
+                     "pairs_in_cutoff_per_launch": pairs_b, "flop_per_pair": FLOP_PER_PAIR},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -191,6 +191,11 @@ int sofia_md_init(sofia_ctx *ctx);
  * nve_v, nve_x, pbc, check_neighbor, list rebuild when flagged, forces, Langevin,
  * SHAKE, nve_v.  use_graph != 0 captures one step in a hipGraph and replays it. */
 int sofia_md_run(sofia_ctx *ctx, int32_t nsteps, int32_t use_graph);
+/* Continue from FP64 state uploaded with sofia_put_state_f64 (and the replica step
+ * counters with sofia_set_replica_steps), e.g. after moving replicas into a new batch:
+ * rebuilds the lists and re-evaluates the forces of the step that produced the state, so
+ * the continuation is the same trajectory as if the replica had never moved. */
+int sofia_md_resume(sofia_ctx *ctx);
 /* Capped steepest descent on the engine state (no SHAKE, no thermostat): each atom
  * moves along its force by min(gamma*|F|, max_disp) A per step; ends with a full force
  * evaluation so sofia_md_run can continue.  For relaxing generated starting structures. */

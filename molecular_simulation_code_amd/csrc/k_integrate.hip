@@ -466,9 +466,9 @@ __global__ void k_minimize_step(int32_t n, double gamma, double max_disp, double
     pos4[i] = p;
 }
 
-__global__ void k_step_advance(int32_t nrep, uint64_t *step) {
+__global__ void k_step_add(int32_t nrep, uint64_t *step, long long delta) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nrep) step[r] += 1;
+    if (r < nrep) step[r] += (uint64_t)delta;
 }
 
 }  // namespace
@@ -560,8 +560,12 @@ void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp) {
 }
 
 void launch_step_advance(sofia_ctx *c) {
-    hipLaunchKernelGGL(k_step_advance, dim3(nblocks(c->nrep, 256)), dim3(256), 0, c->stream, c->nrep,
-                       c->step_d);
+    launch_step_add(c, 1);
+}
+
+void launch_step_add(sofia_ctx *c, long long delta) {
+    hipLaunchKernelGGL(k_step_add, dim3(nblocks(c->nrep, 256)), dim3(256), 0, c->stream, c->nrep,
+                       c->step_d, delta);
 }
 
 }  // namespace sofia

@@ -885,32 +885,57 @@ int check_engine_flags(sofia_ctx *c) {
 
 }  // namespace
 
+namespace {
+
+// Build every replica's list with 50 % headroom (so an MD segment never has to regrow it)
+// and reset the engine's counters and flags.
+int engine_lists(sofia_ctx *c) {
+    launch_set_rebuild(c, 1);
+    TRY(rebuild_now(c, true, false));
+    std::vector<int32_t> nn(c->N);
+    TRY(download(nn.data(), c->numneigh, sizeof(int32_t) * c->N));
+    int32_t mx = 0;
+    for (int v : nn) mx = std::max(mx, v);
+    const int want = std::max(64, ((mx * 3 / 2 + 15) / 16) * 16);
+    if (want > c->neigh_alloc) {
+        TRY(ensure_list(c, want));
+        launch_set_rebuild(c, 1);
+        TRY(rebuild_now(c, true, false));
+    }
+    SOFIA_HIP(hipMemsetAsync(c->nbuilds, 0, sizeof(int32_t) * c->nrep, c->stream));
+    SOFIA_HIP(hipMemsetAsync(c->shake_err, 0, sizeof(int32_t), c->stream));
+    SOFIA_HIP(hipMemsetAsync(c->maxcnt, 0, sizeof(int32_t), c->stream));
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return SOFIA_OK;
+}
+
+}  // namespace
+
 int sofia_md_init(sofia_ctx *c) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
     launch_engine_from32(c);
     c->engine_ready = true;
-    launch_set_rebuild(c, 1);
-    // engine lists get 50 % headroom so a segment never has to regrow them
-    TRY(rebuild_now(c, true, false));
-    int32_t mx = 0;
-    {
-        std::vector<int32_t> nn(c->N);
-        TRY(download(nn.data(), c->numneigh, sizeof(int32_t) * c->N));
-        for (int v : nn) mx = std::max(mx, v);
-        const int want = std::max(64, ((mx * 3 / 2 + 15) / 16) * 16);
-        if (want > c->neigh_alloc) {
-            TRY(ensure_list(c, want));
-            launch_set_rebuild(c, 1);
-            TRY(rebuild_now(c, true, false));
-        }
-    }
-    SOFIA_HIP(hipMemsetAsync(c->nbuilds, 0, sizeof(int32_t) * c->nrep, c->stream));
-    SOFIA_HIP(hipMemsetAsync(c->shake_err, 0, sizeof(int32_t), c->stream));
-    SOFIA_HIP(hipMemsetAsync(c->maxcnt, 0, sizeof(int32_t), c->stream));
+    TRY(engine_lists(c));
     force_sequence(c, true);
     TRY(collect_energies(c));
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return check_engine_flags(c);
+}
+
+int sofia_md_resume(sofia_ctx *c) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    // FP64 state came in through sofia_put_state_f64: refresh the float copies
+    launch_engine_to32(c);
+    SOFIA_HIP(hipMemsetAsync(c->drd, 0, sizeof(double) * 3 * (size_t)c->N, c->stream));
+    c->engine_ready = true;
+    TRY(engine_lists(c));
+    // the forces of the step that produced this state carried the Langevin draw of the
+    // previous counter value
+    launch_step_add(c, -1);
+    force_sequence(c, true);
+    launch_step_add(c, 1);
+    TRY(collect_energies(c));
     return check_engine_flags(c);
 }
 

@@ -185,6 +185,7 @@ void launch_nve_x64(sofia_ctx *c);
 void launch_langevin64(sofia_ctx *c);
 void launch_shake64(sofia_ctx *c);
 void launch_step_advance(sofia_ctx *c);
+void launch_step_add(sofia_ctx *c, long long delta);
 void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp);
 
 // cluster analysis (k_cluster.hip)

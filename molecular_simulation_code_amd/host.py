@@ -451,6 +451,32 @@ class Engine:
             off += n
         return out
 
+    def put_state(self, states):
+        """Upload FP64 (x, v) per replica, each in its own global_id order; valid before
+        the first sort (a fresh engine holds every replica in that order)."""
+        x = np.ascontiguousarray(np.concatenate([s[0] for s in states]), dtype=f64)
+        v = np.ascontiguousarray(np.concatenate([s[1] for s in states]), dtype=f64)
+        check(lib().sofia_put_state_f64(self.ctx, ptr(x), ptr(v)))
+
+    def steps(self) -> np.ndarray:
+        out = np.empty(len(self.reps), np.uint64)
+        check(lib().sofia_get_replica_steps(self.ctx, ptr(out)))
+        return out
+
+    def set_steps(self, steps):
+        s = np.ascontiguousarray(steps, dtype=np.uint64)
+        check(lib().sofia_set_replica_steps(self.ctx, ptr(s)))
+
+    def resume(self):
+        check(lib().sofia_md_resume(self.ctx))
+
+    def profile_nonbond(self, launches: int = 20):
+        """(ms per pair-kernel launch, list entries inside the cutoff)."""
+        ms = C.c_double()
+        pairs = C.c_int64()
+        check(lib().sofia_profile_nonbond(self.ctx, launches, C.byref(ms), C.byref(pairs)))
+        return ms.value, pairs.value
+
     def largest_cluster(self, rcontact: float = 3.5) -> np.ndarray:
         o = _c([4, 5], i32)
         nt = _c([3], i32)

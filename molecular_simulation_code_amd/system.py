@@ -584,19 +584,20 @@ def maxwell_boltzmann(system: SofiaSystem, temp: float, rng: np.random.Generator
 
 def constraint_pairs(system: SofiaSystem):
     """Flattened (a, b, d) constraint pairs (0-based) of every SHAKE cluster."""
+    num = system.shake_num
+    at = system.shake_atoms.astype(np.int64) - 1
+    bd = system.shake_bond
+    # (first atom slot, second atom slot, bond slot) per constraint of each cluster type
+    table = {1: [(0, 1, 0)], 3: [(0, 1, 0), (0, 2, 1)], 4: [(0, 1, 0), (0, 2, 1), (0, 3, 2)],
+             -3: [(0, 1, 0), (0, 2, 1), (1, 2, 2)]}
     a, b, d = [], [], []
-    for k, num in enumerate(system.shake_num):
-        at = system.shake_atoms[k] - 1
-        bd = system.shake_bond[k]
-        if num == 1:
-            a.append(at[0]); b.append(at[1]); d.append(bd[0])
-        elif num == 3:
-            a += [at[0], at[0]]; b += [at[1], at[2]]; d += [bd[0], bd[1]]
-        elif num == 4:
-            a += [at[0]] * 3; b += [at[1], at[2], at[3]]; d += list(bd)
-        elif num == -3:
-            a += [at[0], at[0], at[1]]; b += [at[1], at[2], at[2]]; d += list(bd)
-    return np.array(a, np.int64), np.array(b, np.int64), np.array(d, f64)
+    for kind, cons in table.items():
+        sel = np.where(num == kind)[0]
+        for p, q, k in cons:
+            a.append(at[sel, p]); b.append(at[sel, q]); d.append(bd[sel, k])
+    if not a:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, f64)
+    return np.concatenate(a), np.concatenate(b), np.concatenate(d).astype(f64)
 
 
 def project_constraints(system: SofiaSystem, v: np.ndarray, sweeps: int = 30) -> np.ndarray:
@@ -611,13 +612,15 @@ def project_constraints(system: SofiaSystem, v: np.ndarray, sweeps: int = 30) ->
     r -= L * np.round(r / L)
     r2 = (r * r).sum(1)
     ima, imb = 1.0 / m[a], 1.0 / m[b]
-    v = v.copy()
+    v = v.astype(f64).copy()
+    n = len(v)
     for _ in range(sweeps):
         dv = v[a] - v[b]
         g = (dv * r).sum(1) / (r2 * (ima + imb))
         corr = g[:, None] * r
-        np.add.at(v, a, -0.5 * corr * ima[:, None])
-        np.add.at(v, b, 0.5 * corr * imb[:, None])
+        for c in range(3):
+            v[:, c] += (np.bincount(a, -0.5 * corr[:, c] * ima, minlength=n) +
+                        np.bincount(b, 0.5 * corr[:, c] * imb, minlength=n))
     return v
 
 

@@ -157,3 +157,29 @@ def test_cluster_union_find_matches_cpu_bfs():
                     seen.add(w); stack.append(w)
         best = max(best, size)
     assert big == best
+
+
+def test_served_request_same_alone_and_in_mix():
+    """Continuous batching: a request's result does not depend on what else is served."""
+    host = _engine()
+    from molecular_simulation_code_amd import server as SV
+    pool = SV.request_pool([3000, 4200, 5200], minimize_steps=60)
+    say("[serve] alone vs mix")
+
+    def req(rid, k, arrival, seed, temp):
+        s = pool[k].copy()
+        s.v = S.maxwell_boltzmann(s, temp, np.random.default_rng(seed))
+        return SV.Request(rid, arrival, s, seed, temp, nsteps=300)
+
+    srv = SV.SegmentServer(segment=100, keep_state=True)
+    alone, _, _ = srv.serve([req(0, 0, 0.0, 101, 300.0)], realtime=False)
+    mix_reqs = [req(0, 0, 0.0, 101, 300.0), req(1, 1, 0.0, 202, 285.0), req(2, 2, 1.5, 303, 315.0),
+                req(3, 1, 2.5, 404, 295.0)]
+    srv2 = SV.SegmentServer(segment=100, keep_state=True)
+    mix, _, _ = srv2.serve(mix_reqs, realtime=False)
+    assert srv2.rebatches >= 3                       # the batch really changed under request 0
+    a, m = alone[0], mix[0]
+    assert np.array_equal(a.state[0], m.state[0]) and np.array_equal(a.state[1], m.state[1])
+    assert a.clusters == m.clusters
+    np.testing.assert_allclose(a.energies, m.energies, rtol=1e-12, atol=1e-9)
+    assert all(r.finished >= r.arrival for r in mix)
