@@ -172,6 +172,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--minimize", type=int, default=150)
+    ap.add_argument("--workers", type=int, default=0,
+                    help="input-generation processes (0 = one per core up to 16; use 1 under a "
+                         "profiler, which initialises the GPU before the program starts)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); replicas shard "
@@ -180,7 +183,7 @@ def main():
 
     ensure_built()
     from molecular_simulation_code_amd import system as S
-    workers = min(16, os.cpu_count() or 1)
+    workers = args.workers if args.workers > 0 else min(16, os.cpu_count() or 1)
     K, W = args.steps, args.warmup
 
     # ---------------------------------------------------------------- inputs (CPU, before any GPU use)

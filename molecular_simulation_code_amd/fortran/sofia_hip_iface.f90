@@ -1,0 +1,324 @@
+! ISO_C_BINDING shim between the reference's Fortran host and libsofia_hip.so.
+!
+! Module sofia_hip_iface declares the C ABI (include/sofia_hip.h).  Module sofia_hip_dropin
+! re-exposes the hot-path subroutines under the reference's names with a `_hip` suffix and
+! the reference's argument lists; each one reads and writes `module global` exactly like
+! the routine it replaces:
+!
+!   neighbor_wrapper(step)                  Source/mod_neighbor.f90:8-86
+!   check_neighbor(nflag, step)             Source/mod_neighbor.f90:895-913
+!   lj_cut_coul_dsf_nonewton_NOMIC(step)    Source/mod_force_nonbond.f90:105-196
+!   lj_cut_coul_long_nonewton_NOMIC         Source/mod_force_nonbond.f90:300-400
+!   correct_12/13/14_bonds                  Source/mod_force_correction.f90:6-278
+!   nve_v(dt_partial), nve_x                Source/mod_ensemble.f90:11-87
+!   pbc(step)                               Source/mod_pbc.f90:6-39
+!   shake(iflag)                            Source/mod_shake.f90:8-85
+!
+! A maintainer swaps the calls (or renames the wrappers) in force_driver / the time loops;
+! sofia_hip_init is called once after the reference's initialisation sequence
+! (Source/mod_restart_datadump.f90:139-233), when global holds the topology.
+module sofia_hip_iface
+  use iso_c_binding
+  implicit none
+
+  type, bind(c) :: sofia_params_t
+     integer(c_int32_t) :: ntypes
+     real(c_float) :: rcut, cut_coul, pad, coulpre, dt, kcal2amu
+     real(c_double) :: rcut2, cut_coulsq, rv2, rvrc2, alpha, gewald, e_shift, f_shift
+     real(c_double) :: ewald_p, ewald_f, a1, a2, a3, a4, a5, my_pis, my_pis_inv, amu2e
+     real(c_double) :: fudge_12, fudge_13, fudge_14_coul, shake_tol
+     integer(c_int32_t) :: shake_maxiter, precision, coul_flag, neigh_alloc0
+  end type sofia_params_t
+
+  interface
+     integer(c_int) function sofia_ctx_create(p, np, box, ctx) bind(c, name="sofia_ctx_create")
+       import :: c_int, c_int32_t, c_float, c_ptr, sofia_params_t
+       type(sofia_params_t), intent(in) :: p
+       integer(c_int32_t), value :: np
+       real(c_float), value :: box
+       type(c_ptr), intent(out) :: ctx
+     end function
+     integer(c_int) function sofia_ctx_destroy(ctx) bind(c, name="sofia_ctx_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function
+     integer(c_int) function sofia_set_forcefield(ctx, lj1, lj2, lj3, lj4, l141, l142, l143, l144, &
+          mass) bind(c, name="sofia_set_forcefield")
+       import :: c_int, c_ptr, c_float, c_double
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: lj1(*), lj2(*), lj3(*), lj4(*)
+       real(c_double), intent(in) :: l141(*), l142(*), l143(*), l144(*), mass(*)
+     end function
+     integer(c_int) function sofia_set_exclusions(ctx, n1, n2, n3, spec) &
+          bind(c, name="sofia_set_exclusions")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: n1(*), n2(*), n3(*), spec(*)
+     end function
+     integer(c_int) function sofia_set_bonded(ctx, bl, nb, nbr, al, na, nar, dl, nd, ndr) &
+          bind(c, name="sofia_set_bonded")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, nbr, nar, ndr
+       integer(c_int32_t), intent(in) :: bl(*), al(*), dl(*)
+       integer(c_int32_t), value :: nb, na, nd
+     end function
+     integer(c_int) function sofia_set_shake(ctx, ns, nsr, num, atoms, bond) &
+          bind(c, name="sofia_set_shake")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, nsr
+       integer(c_int32_t), value :: ns
+       integer(c_int32_t), intent(in) :: num(*), atoms(*)
+       real(c_double), intent(in) :: bond(*)
+     end function
+     ! position is type(ainfo) and v/ff are type(coords): assumed-type dummies pass them
+     ! without copies (same 16 B / 12 B layouts as sofia_ainfo / sofia_coords)
+     integer(c_int) function sofia_put_state(ctx, pos, q, v, gid) bind(c, name="sofia_put_state")
+       import :: c_int, c_ptr, c_float, c_int32_t
+       type(c_ptr), value :: ctx
+       type(*), intent(in) :: pos(*), v(*)
+       real(c_float), intent(in) :: q(*)
+       integer(c_int32_t), intent(in) :: gid(*)
+     end function
+     integer(c_int) function sofia_get_state(ctx, pos, v, gid) bind(c, name="sofia_get_state")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       type(*), intent(inout) :: pos(*), v(*)
+       integer(c_int32_t), intent(inout) :: gid(*)
+     end function
+     integer(c_int) function sofia_put_displacement(ctx, dx, dy, dz) &
+          bind(c, name="sofia_put_displacement")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: dx(*), dy(*), dz(*)
+     end function
+     integer(c_int) function sofia_neighbor_wrapper(ctx, step, tt) bind(c, name="sofia_neighbor_wrapper")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: step
+       integer(c_int32_t), intent(inout) :: tt(*)
+     end function
+     integer(c_int) function sofia_get_nlist(ctx, nlist, numneigh, na) bind(c, name="sofia_get_nlist")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(inout) :: nlist(*), numneigh(*)
+       integer(c_int32_t), intent(out) :: na
+     end function
+     integer(c_int) function sofia_check_neighbor(ctx, nflag) bind(c, name="sofia_check_neighbor")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: nflag
+     end function
+     integer(c_int) function sofia_lj_cut_coul_dsf(ctx, ff, ff64, pot, ecoul, vir) &
+          bind(c, name="sofia_lj_cut_coul_dsf")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, ff64
+       type(*), intent(inout) :: ff(*)
+       real(c_double), intent(inout) :: pot, ecoul, vir(3)
+     end function
+     integer(c_int) function sofia_lj_cut_coul_long(ctx, ff, pot, ecoul, vir) &
+          bind(c, name="sofia_lj_cut_coul_long")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       type(*), intent(inout) :: ff(*)
+       real(c_double), intent(inout) :: pot, ecoul, vir(3)
+     end function
+     integer(c_int) function sofia_correct_bonds(ctx, which, ff, ec, vir) bind(c, name="sofia_correct_bonds")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: which
+       type(*), intent(inout) :: ff(*)
+       real(c_double), intent(inout) :: ec, vir(3)
+     end function
+     integer(c_int) function sofia_nve_v(ctx, dtp, ff, ke) bind(c, name="sofia_nve_v")
+       import :: c_int, c_ptr, c_float, c_double
+       type(c_ptr), value :: ctx
+       real(c_float), value :: dtp
+       type(*), intent(in) :: ff(*)
+       real(c_double), intent(out) :: ke
+     end function
+     integer(c_int) function sofia_nve_x(ctx) bind(c, name="sofia_nve_x")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function
+     integer(c_int) function sofia_pbc(ctx) bind(c, name="sofia_pbc")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function
+     integer(c_int) function sofia_shake(ctx, iflag, ff, vir, maxit) bind(c, name="sofia_shake")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: iflag
+       type(*), intent(inout) :: ff(*)
+       real(c_double), intent(inout) :: vir(3)
+       integer(c_int32_t), intent(out) :: maxit
+     end function
+     integer(c_int) function sofia_last_error(buf, n) bind(c, name="sofia_last_error")
+       import :: c_int, c_char, c_size_t
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_size_t), value :: n
+     end function
+  end interface
+
+contains
+
+  subroutine sofia_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(kind=c_char) :: buf(512)
+    integer(c_int) :: dummy
+    integer :: k
+    character(len=512) :: msg
+    if (rc == 0) return
+    dummy = sofia_last_error(buf, int(512, c_size_t))
+    msg = ' '
+    do k = 1, 512
+       if (buf(k) == c_null_char) exit
+       msg(k:k) = buf(k)
+    end do
+    print *, 'libsofia_hip error in ', where, ': ', trim(msg)
+    stop
+  end subroutine sofia_check
+
+end module sofia_hip_iface
+
+
+module sofia_hip_dropin
+  use iso_c_binding
+  use global
+  use sofia_hip_iface
+  implicit none
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer(c_int32_t), allocatable, save :: tt_buf(:)
+
+contains
+
+  ! Create the device context from module global once the reference has read its inputs
+  ! (constants as in Source/get_started.f90:25-117).
+  subroutine sofia_hip_init(precision)
+    integer, intent(in) :: precision
+    type(sofia_params_t) :: p
+    real(c_double), allocatable :: l14(:,:)
+    integer :: a, b, nt
+    nt = numAtomType
+    p%ntypes = nt; p%rcut = rcut; p%cut_coul = cut_coul; p%pad = neighbor_pad
+    p%coulpre = coulpre; p%dt = dt; p%kcal2amu = kcal2amu
+    p%rcut2 = rcut2; p%cut_coulsq = cut_coulsq; p%rv2 = rv2; p%rvrc2 = rvrc2
+    p%alpha = alpha; p%gewald = gewald; p%e_shift = e_shift; p%f_shift = f_shift
+    p%ewald_p = EWALD_P; p%ewald_f = EWALD_F
+    p%a1 = A1; p%a2 = A2; p%a3 = A3; p%a4 = A4; p%a5 = A5
+    p%my_pis = MY_PIS; p%my_pis_inv = MY_PIS_INV; p%amu2e = amu2e
+    p%fudge_12 = fudge_12; p%fudge_13 = fudge_13; p%fudge_14_coul = fudge_14_coul
+    p%shake_tol = tolerance; p%shake_maxiter = maxiter
+    p%precision = precision; p%coul_flag = coul_flag; p%neigh_alloc0 = neigh_alloc
+    call sofia_check(sofia_ctx_create(p, int(np, c_int32_t), real(box, c_float), ctx), 'ctx_create')
+    ! lj14 tables are (itype, jtype); the ABI wants row-major (itype-1)*nt + jtype
+    allocate(l14(nt*nt, 4))
+    do a = 1, nt
+       do b = 1, nt
+          l14((a-1)*nt + b, 1) = lj14_1(a, b); l14((a-1)*nt + b, 2) = lj14_2(a, b)
+          l14((a-1)*nt + b, 3) = lj14_3(a, b); l14((a-1)*nt + b, 4) = lj14_4(a, b)
+       end do
+    end do
+    call sofia_check(sofia_set_forcefield(ctx, lj1, lj2, lj3, lj4, l14(:,1), l14(:,2), l14(:,3), &
+         l14(:,4), mass), 'set_forcefield')
+    call sofia_check(sofia_set_exclusions(ctx, num1bond, num2bond, num3bond, specbond), 'set_exclusions')
+    call sofia_check(sofia_set_bonded(ctx, bondlist, int(numbond, c_int32_t), c_null_ptr, anglelist, &
+         int(numangle, c_int32_t), c_null_ptr, dihedlist, int(numdihed, c_int32_t), c_null_ptr), &
+         'set_bonded')
+    allocate(tt_buf(np))
+  end subroutine sofia_hip_init
+
+  ! neighbor_wrapper(step): the device sorts and builds the list; the returned tt re-indexes
+  ! the host arrays the device does not hold (mol, integrate_flag), as sort_posit_charge does.
+  subroutine neighbor_wrapper_hip(step)
+    integer :: step, i
+    nn_builds = nn_builds + 1
+    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_neighbor_wrapper(ctx, int(step, c_int32_t), tt_buf), 'neighbor_wrapper')
+    call sofia_check(sofia_get_state(ctx, position, v, global_id), 'get_state')
+    do i = 1, np
+       qs(tt_buf(i)) = q(i)
+       molsort(:, tt_buf(i)) = mol(:, i)
+       integrate_flag_ss(tt_buf(i)) = integrate_flag(i)
+    end do
+    q(1:np) = qs(1:np)
+    mol(:,:) = molsort(:,:)
+    integrate_flag(:) = integrate_flag_ss(:)
+    drx(:) = 0.0; dry(:) = 0.0; drz(:) = 0.0
+  end subroutine neighbor_wrapper_hip
+
+  subroutine check_neighbor_hip(nflag, step)
+    integer :: nflag, step
+    integer(c_int32_t) :: f
+    call sofia_check(sofia_put_displacement(ctx, drx, dry, drz), 'put_displacement')
+    call sofia_check(sofia_check_neighbor(ctx, f), 'check_neighbor')
+    nflag = f
+  end subroutine check_neighbor_hip
+
+  subroutine lj_cut_coul_dsf_nonewton_NOMIC_hip(step)
+    integer :: step
+    real(c_double) :: vir(3)
+    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_lj_cut_coul_dsf(ctx, ff, c_null_ptr, potential, e_coul, vir), 'dsf')
+  end subroutine lj_cut_coul_dsf_nonewton_NOMIC_hip
+
+  subroutine lj_cut_coul_long_nonewton_NOMIC_hip
+    real(c_double) :: vir(3)
+    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_lj_cut_coul_long(ctx, ff, potential, e_coul, vir), 'coul_long')
+    virialx = vir(1); virialy = vir(2); virialz = vir(3)
+  end subroutine lj_cut_coul_long_nonewton_NOMIC_hip
+
+  subroutine correct_12_bonds_hip
+    real(c_double) :: vir(3)
+    vir = (/ v_bondx, v_bondy, v_bondz /)
+    call sofia_check(sofia_correct_bonds(ctx, 12_c_int32_t, ff_bond, e_coul_bond, vir), 'correct_12')
+    v_bondx = vir(1); v_bondy = vir(2); v_bondz = vir(3)
+  end subroutine correct_12_bonds_hip
+
+  subroutine correct_13_bonds_hip
+    real(c_double) :: vir(3)
+    vir = (/ v_anglex, v_angley, v_anglez /)
+    call sofia_check(sofia_correct_bonds(ctx, 13_c_int32_t, ff_angle, e_coul_angle, vir), 'correct_13')
+    v_anglex = vir(1); v_angley = vir(2); v_anglez = vir(3)
+  end subroutine correct_13_bonds_hip
+
+  subroutine correct_14_bonds_hip
+    real(c_double) :: vir(3)
+    vir = (/ v_dihedx, v_dihedy, v_dihedz /)
+    call sofia_check(sofia_correct_bonds(ctx, 14_c_int32_t, ff_dihed, e_coul_dihed, vir), 'correct_14')
+    v_dihedx = vir(1); v_dihedy = vir(2); v_dihedz = vir(3)
+  end subroutine correct_14_bonds_hip
+
+  subroutine nve_v_hip(dt_partial)
+    real*4 :: dt_partial
+    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_nve_v(ctx, dt_partial, ff, ke), 'nve_v')
+    call sofia_check(sofia_get_state(ctx, position, v, global_id), 'get_state')
+  end subroutine nve_v_hip
+
+  subroutine nve_x_hip
+    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_put_displacement(ctx, drx, dry, drz), 'put_displacement')
+    call sofia_check(sofia_nve_x(ctx), 'nve_x')
+    call sofia_check(sofia_get_state(ctx, position, v, global_id), 'get_state')
+  end subroutine nve_x_hip
+
+  subroutine pbc_hip(step)
+    integer :: step
+    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_pbc(ctx), 'pbc')
+    call sofia_check(sofia_get_state(ctx, position, v, global_id), 'get_state')
+  end subroutine pbc_hip
+
+  subroutine shake_hip(iflag)
+    integer :: iflag
+    real(c_double) :: vir(3)
+    integer(c_int32_t) :: its
+    vir = (/ virialx, virialy, virialz /)
+    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_shake(ctx, int(iflag, c_int32_t), ff, vir, its), 'shake')
+    virialx = vir(1); virialy = vir(2); virialz = vir(3)
+  end subroutine shake_hip
+
+end module sofia_hip_dropin

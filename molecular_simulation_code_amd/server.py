@@ -98,6 +98,7 @@ class SegmentServer:
         virtual = 0.0
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         active: list[_Slot] = []
+        in_engine: list[_Slot] = []      # slots held by the current engine, in its replica order
         results: list[Result] = []
         eng = None
         dirty = False
@@ -119,9 +120,10 @@ class SegmentServer:
                 continue
             if dirty:
                 if eng is not None:
-                    self._harvest(eng, active)
+                    self._harvest(eng, in_engine)
                     eng.close()
                 eng = self._engine(active)
+                in_engine = list(active)
                 dirty = False
             nsteps = min(self.segment, min(s.req.nsteps - s.done for s in active))
             eng.run(nsteps)
@@ -145,10 +147,11 @@ class SegmentServer:
                     used -= self.footprint(s.req)
                 keep = [k for k in range(len(active)) if k not in set(finished)]
                 if keep:
-                    self._harvest(eng, active)
+                    self._harvest(eng, in_engine)
                 active = [active[k] for k in keep]
                 eng.close()
                 eng = None
+                in_engine = []
                 dirty = bool(active)
         if eng is not None:
             eng.close()
