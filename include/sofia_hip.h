@@ -192,10 +192,20 @@ int sofia_md_init(sofia_ctx *ctx);
  * SHAKE, nve_v.  use_graph != 0 captures one step in a hipGraph and replays it. */
 int sofia_md_run(sofia_ctx *ctx, int32_t nsteps, int32_t use_graph);
 /* Continue from FP64 state uploaded with sofia_put_state_f64 (and the replica step
- * counters with sofia_set_replica_steps), e.g. after moving replicas into a new batch:
- * rebuilds the lists and re-evaluates the forces of the step that produced the state, so
- * the continuation is the same trajectory as if the replica had never moved. */
+ * counters with sofia_set_replica_steps), e.g. a restart: rebuilds the lists and
+ * re-evaluates the forces at the state, with the Langevin draw of the step that produced
+ * it.  Friction and SHAKE then see the full-step velocity instead of the half-step one;
+ * sofia_md_resume_forces continues bit-exactly. */
 int sofia_md_resume(sofia_ctx *ctx);
+/* Total force on every atom after the last step (3N doubles, the order of
+ * sofia_get_state_f64): the force the next step's first half-kick uses. */
+int sofia_get_forces_f64(sofia_ctx *ctx, double *f);
+/* sofia_md_resume, then the forces of every replica r with have[r] != 0 replaced by f
+ * (3N doubles in sofia_put_state_f64's order, as returned by sofia_get_forces_f64 and
+ * re-ordered with the state).  The state a step carries over is (x, v, F): F was evaluated
+ * with the half-step velocity (Langevin friction, SHAKE), which the state no longer holds,
+ * so carrying F keeps a moved replica on exactly the trajectory it had. */
+int sofia_md_resume_forces(sofia_ctx *ctx, const double *f, const int32_t *have);
 /* Capped steepest descent on the engine state (no SHAKE, no thermostat): each atom
  * moves along its force by min(gamma*|F|, max_disp) A per step; ends with a full force
  * evaluation so sofia_md_run can continue.  For relaxing generated starting structures. */

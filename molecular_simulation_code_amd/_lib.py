@@ -88,6 +88,8 @@ SIGNATURES = {
     "sofia_md_init": [_vp],
     "sofia_md_run": [_vp, _i32, _i32],
     "sofia_md_resume": [_vp],
+    "sofia_get_forces_f64": [_vp, _vp],
+    "sofia_md_resume_forces": [_vp, _vp, _vp],
     "sofia_minimize": [_vp, _i32, _d, _d],
     "sofia_md_energies": [_vp, _vp],
     "sofia_md_rebuilds": [_vp, _vp],

@@ -221,7 +221,7 @@ k_build_list(const float4 *__restrict__ pos4, const int32_t *__restrict__ num3,
              int32_t *__restrict__ maxcnt) {
     const int c = blockIdx.x;
     const int r = cell_rep[c];
-    if (!rebuild[r]) return;
+    if (!rebuild[r] || reps[r].stencil5) return;
     const float box = reps[r].box, ibox = reps[r].ibox;
 
     __shared__ int32_t s_cell[27];
@@ -328,12 +328,15 @@ __global__ void k_check32(int32_t n, const int32_t *__restrict__ rep, const floa
     if (d2 >= rvrc2) atomicOr(&rebuild[rep[i]], 1);
 }
 
+// Engine criterion: per-replica threshold that keeps every pair inside the force cutoff in
+// the list (half the usable skin), so forces do not depend on when the list was built.
 __global__ void k_check64(int32_t n, const int32_t *__restrict__ rep, const double *__restrict__ dr,
-                          double rvrc2, int32_t *__restrict__ rebuild) {
+                          const RepInfo *__restrict__ reps, int32_t *__restrict__ rebuild) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int r = rep[i];
     const double x = dr[i], y = dr[(size_t)n + i], z = dr[2 * (size_t)n + i];
-    if ((x * x + y * y) + z * z >= rvrc2) atomicOr(&rebuild[rep[i]], 1);
+    if ((x * x + y * y) + z * z >= reps[r].chk2) atomicOr(&rebuild[r], 1);
 }
 
 __global__ void k_set_flags(int32_t nrep, int32_t *__restrict__ rebuild, int32_t value) {
@@ -391,6 +394,8 @@ void launch_remap_topology(sofia_ctx *c) {
 }
 
 void launch_build_list(sofia_ctx *c) {
+    if (c->n_stencil5 > 0) launch_build_list_engine(c);
+    if (c->n_stencil5 == c->nrep) return;
     const int w = c->list_waves;
 #define LB(W)                                                                                    \
     hipLaunchKernelGGL(k_build_list<W>, dim3(c->ncell_total), dim3(W * 64), 0, c->stream,       \
@@ -407,7 +412,7 @@ void launch_build_list(sofia_ctx *c) {
 void launch_check_neighbor(sofia_ctx *c, bool engine) {
     if (engine)
         hipLaunchKernelGGL(k_check64, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
-                           c->drd, c->dp.rvrc2, c->rebuild);
+                           c->drd, c->reps_d, c->rebuild);
     else
         hipLaunchKernelGGL(k_check32, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
                            c->dr32, c->dp.rvrc2, c->rebuild);

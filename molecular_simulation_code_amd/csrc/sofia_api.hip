@@ -129,7 +129,8 @@ int ensure_list(sofia_ctx *c, int na) {
     return SOFIA_OK;
 }
 
-// Stage host data through a device scratch buffer.
+// Synchronous copies on the null stream; callers synchronise the context stream first
+// when kernels on it may still use the destination.
 int upload(void *dst, const void *src, size_t bytes) {
     if (!bytes) return SOFIA_OK;
     SOFIA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -887,9 +888,76 @@ int check_engine_flags(sofia_ctx *c) {
 
 namespace {
 
+// The engine's cell grid and rebuild threshold, set once per context.
+//
+// The reference's grid (cells of max(rcut, cut_coul) per side) with its 27-cell stencil
+// lists only pairs closer than the cell size, and its criterion (any atom displaced by
+// rv - rc) lets pairs enter the cutoff unlisted, so its forces depend on when the list was
+// last built.  The engine keeps every pair inside the force cutoff in the list instead:
+//  * cells of at least rv/2 per side with a +-2 stencil (k_build_engine) where the box
+//    holds five of them, else cells of at least rv with the 27-cell stencil where it holds
+//    three, else the reference grid; every pair closer than min(rv, reach) is then listed,
+//    reach being 2 cells or 1 cell;
+//  * rebuild once any atom of the replica has moved by half the guaranteed skin
+//    (min(rv, reach) - rc), less 1e-3 A for float rounding of positions and cell indices.
+// Forces are then a function of the positions alone, which makes a replica's trajectory
+// independent of its rebuild history (resume into a new batch, served alone or in a mix).
+int engine_regrid(sofia_ctx *c) {
+    if (c->engine_grid) return SOFIA_OK;
+    TRY(sync_check(c));
+    const double rc = std::sqrt(std::max(c->p.rcut2, c->p.cut_coulsq));
+    const double rv = std::sqrt(c->p.rv2);
+    int cell0 = 0, n5 = 0;
+    for (auto &R : c->reps_h) {
+        const int nh = (int)((double)R.box / (0.5 * rv));
+        const int n1 = (int)((double)R.box / rv);
+        double reach = R.rn;
+        R.stencil5 = 0;
+        if (nh >= 5 && nh <= 1024) {
+            R.ncellD = nh;
+            R.rn = R.box / (float)nh;
+            R.stencil5 = 1;
+            reach = 2.0 * (double)R.rn;
+            ++n5;
+        } else if (n1 >= 3 && n1 <= R.ncellD) {
+            R.ncellD = n1;
+            R.rn = R.box / (float)n1;
+            reach = R.rn;
+        }
+        R.ncellT = R.ncellD * R.ncellD * R.ncellD;
+        const double half = 0.5 * (std::min(rv, reach) - rc) - 1e-3;
+        R.chk2 = half > 0.0 ? half * half : 0.0;
+        R.cell0 = cell0;
+        cell0 += R.ncellT;
+    }
+    if (cell0 > c->ncell_total) {   // finer cells: re-size the per-cell arrays
+        dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep);
+        TRY(dalloc(&c->ccount, cell0));
+        TRY(dalloc(&c->cstart, cell0));
+        TRY(dalloc(&c->cnum, (size_t)27 * cell0));
+        TRY(dalloc(&c->cell_rep, cell0));
+        SOFIA_HIP(hipMemset(c->ccount, 0, sizeof(int32_t) * cell0));
+        SOFIA_HIP(hipMemset(c->cstart, 0, sizeof(int32_t) * cell0));
+    }
+    c->ncell_total = cell0;
+    c->n_stencil5 = n5;
+    std::vector<int32_t> cnum((size_t)27 * cell0), crep(cell0);
+    for (int r = 0; r < c->nrep; ++r) {
+        const sofia::RepInfo &R = c->reps_h[r];
+        stencil(R.ncellT, R.ncellD, R.cell0, cnum.data() + (size_t)27 * R.cell0);
+        for (int k = 0; k < R.ncellT; ++k) crep[R.cell0 + k] = r;
+    }
+    TRY(upload(c->cnum, cnum.data(), sizeof(int32_t) * cnum.size()));
+    TRY(upload(c->cell_rep, crep.data(), sizeof(int32_t) * crep.size()));
+    TRY(upload(c->reps_d, c->reps_h.data(), sizeof(sofia::RepInfo) * c->nrep));
+    c->engine_grid = true;
+    return SOFIA_OK;
+}
+
 // Build every replica's list with 50 % headroom (so an MD segment never has to regrow it)
 // and reset the engine's counters and flags.
 int engine_lists(sofia_ctx *c) {
+    TRY(engine_regrid(c));
     launch_set_rebuild(c, 1);
     TRY(rebuild_now(c, true, false));
     std::vector<int32_t> nn(c->N);
@@ -937,6 +1005,58 @@ int sofia_md_resume(sofia_ctx *c) {
     launch_step_add(c, 1);
     TRY(collect_energies(c));
     return check_engine_flags(c);
+}
+
+int sofia_get_forces_f64(sofia_ctx *c, double *f) {
+    if (!c || !f) return fail(SOFIA_E_ARG, "null argument");
+    if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
+    TRY(sync_check(c));
+    const size_t n = (size_t)c->N;
+    std::vector<double> nb(3 * n);
+    std::vector<long long> fx(3 * n);
+    TRY(download(nb.data(), c->fnb, sizeof(double) * nb.size()));
+    TRY(download(fx.data(), c->fix, sizeof(long long) * fx.size()));
+    // the sum k_nve_v64 forms (FIX_INV is a power of two, so the product is exact)
+    for (size_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) f[3 * i + d] = nb[d * n + i] + (double)fx[d * n + i] * FIX_INV;
+    return SOFIA_OK;
+}
+
+int sofia_md_resume_forces(sofia_ctx *c, const double *f, const int32_t *have) {
+    if (!c || !f || !have) return fail(SOFIA_E_ARG, "null argument");
+    TRY(sync_check(c));
+    const int n = c->N;
+    // where each (replica, global id) sits in the caller's order, i.e. before the sort
+    std::vector<int32_t> g(n), slot(n, -1);
+    TRY(download(g.data(), c->a.gid, sizeof(int32_t) * n));
+    for (const auto &R : c->reps_h)
+        for (int k = 0; k < R.natom; ++k) {
+            const int gi = g[R.atom0 + k];
+            if (gi < 1 || gi > R.natom || slot[R.atom0 + gi - 1] >= 0)
+                return fail(SOFIA_E_STATE, "global ids of a replica are not a permutation of 1..natom");
+            slot[R.atom0 + gi - 1] = R.atom0 + k;
+        }
+    TRY(sofia_md_resume(c));
+    std::vector<double> nb((size_t)3 * n);
+    std::vector<long long> fx((size_t)3 * n);
+    TRY(download(g.data(), c->a.gid, sizeof(int32_t) * n));
+    TRY(download(nb.data(), c->fnb, sizeof(double) * nb.size()));
+    TRY(download(fx.data(), c->fix, sizeof(long long) * fx.size()));
+    for (int r = 0; r < c->nrep; ++r) {
+        if (!have[r]) continue;
+        const auto &R = c->reps_h[r];
+        for (int k = 0; k < R.natom; ++k) {
+            const int j = R.atom0 + k;
+            const size_t src = (size_t)slot[R.atom0 + g[j] - 1];
+            for (int d = 0; d < 3; ++d) {
+                nb[(size_t)d * n + j] = f[3 * src + d];
+                fx[(size_t)d * n + j] = 0;
+            }
+        }
+    }
+    TRY(upload(c->fnb, nb.data(), sizeof(double) * nb.size()));
+    TRY(upload(c->fix, fx.data(), sizeof(long long) * fx.size()));
+    return SOFIA_OK;
 }
 
 int sofia_md_run(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {

@@ -31,9 +31,11 @@ struct RepInfo {
     int32_t ncellD, ncellT;
     int32_t atom0, natom;
     int32_t cell0;
+    int32_t stencil5; // engine grid of cells >= rv/2 with a +-2 stencil (k_build_engine)
     double fric;      // friction * amu2e (kcal/mol per amu*A/fs)
     double gconst;    // noise amplitude * amu2e
     uint64_t seed;
+    double chk2;      // engine rebuild threshold on |displacement|^2 (complete-list criterion)
 };
 
 // Scalars passed by value to every kernel.
@@ -132,6 +134,8 @@ struct sofia_ctx {
 
     // engine
     bool engine_ready = false;
+    bool engine_grid = false;    // cells re-sized for complete lists (first md_init/md_resume)
+    int32_t n_stencil5 = 0;      // replicas on the half-width engine grid
     uint64_t *step_d = nullptr;  // per-replica step counters (Langevin key)
     uint64_t step_h = 0;
     int lanes = 16, block = 256, list_waves = 4;
@@ -155,6 +159,7 @@ void launch_scatter_sort(sofia_ctx *c);
 void launch_gather(sofia_ctx *c, bool engine);
 void launch_remap_topology(sofia_ctx *c);
 void launch_build_list(sofia_ctx *c);
+void launch_build_list_engine(sofia_ctx *c);
 void launch_check_neighbor(sofia_ctx *c, bool engine);
 void launch_set_rebuild(sofia_ctx *c, int32_t value);
 void launch_finish_rebuild(sofia_ctx *c, bool engine);

@@ -433,12 +433,8 @@ class Engine:
         check(lib().sofia_md_rebuilds(self.ctx, ptr(out)))
         return out
 
-    def state(self):
-        """FP64 positions/velocities and global ids per replica, in each replica's own
-        original atom order (global_id order)."""
-        x = np.empty((self.N, 3), f64)
-        v = np.empty((self.N, 3), f64)
-        check(lib().sofia_get_state_f64(self.ctx, ptr(x), ptr(v)))
+    def _by_gid(self, arrays):
+        """Split device-order (N, 3) arrays per replica, each re-ordered by global id."""
         a = np.empty((self.N, 4), f32)
         gid = np.empty(self.N, i32)
         check(lib().sofia_get_state(self.ctx, ptr(a), None, ptr(gid)))
@@ -447,9 +443,24 @@ class Engine:
         for n in self.natoms:
             sl = slice(off, off + n)
             order = np.argsort(gid[sl])
-            out.append((x[sl][order], v[sl][order]))
+            out.append(tuple(x[sl][order] for x in arrays))
             off += n
         return out
+
+    def state(self):
+        """FP64 positions/velocities per replica, in each replica's own original atom
+        order (global_id order)."""
+        x = np.empty((self.N, 3), f64)
+        v = np.empty((self.N, 3), f64)
+        check(lib().sofia_get_state_f64(self.ctx, ptr(x), ptr(v)))
+        return self._by_gid((x, v))
+
+    def forces(self):
+        """Total force per replica after the last step (global_id order): the force the
+        next step's first half-kick uses."""
+        f = np.empty((self.N, 3), f64)
+        check(lib().sofia_get_forces_f64(self.ctx, ptr(f)))
+        return [t[0] for t in self._by_gid((f,))]
 
     def put_state(self, states):
         """Upload FP64 (x, v) per replica, each in its own global_id order; valid before
@@ -467,8 +478,17 @@ class Engine:
         s = np.ascontiguousarray(steps, dtype=np.uint64)
         check(lib().sofia_set_replica_steps(self.ctx, ptr(s)))
 
-    def resume(self):
-        check(lib().sofia_md_resume(self.ctx))
+    def resume(self, forces=None):
+        """Continue from put_state/set_steps.  `forces` (per replica, global_id order, or
+        None for a replica whose forces are to be recomputed) carries each replica's last
+        total force over, which makes the continuation bit-exact."""
+        if forces is None or all(f is None for f in forces):
+            check(lib().sofia_md_resume(self.ctx))
+            return
+        have = _c([f is not None for f in forces], i32)
+        f = np.ascontiguousarray(np.concatenate(
+            [f if f is not None else np.zeros((n, 3)) for f, n in zip(forces, self.natoms)]), dtype=f64)
+        check(lib().sofia_md_resume_forces(self.ctx, ptr(f), ptr(have)))
 
     def profile_nonbond(self, launches: int = 20):
         """(ms per pair-kernel launch, list entries inside the cutoff)."""

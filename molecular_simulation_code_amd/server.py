@@ -7,9 +7,10 @@ them together in one batched engine, ``segment`` steps at a time.  At each segme
 boundary finished requests leave and waiting ones are admitted while their HBM footprint
 fits the budget -- the batch composition changes as work arrives (continuous batching).
 
-A replica that moves into a new batch is resumed with its exact FP64 state and step
-counter; with the mixed-precision engine (order-independent int64 force sums,
-counter-based Langevin noise) its trajectory is bit-identical to being served alone.
+A replica that moves into a new batch is resumed with its exact FP64 state, last total
+force and step counter; with the mixed-precision engine (order-independent int64 force
+sums, counter-based Langevin noise, lists that hold every pair inside the cutoff) its
+trajectory is bit-identical to being served alone.
 
 Every ``segment`` steps the largest solute cluster (union-find over O-N contacts below
 ``rcontact``) is recorded for each request.
@@ -62,6 +63,7 @@ class _Slot:
     done: int
     admitted: float
     clusters: list
+    f: np.ndarray | None = None      # total force after the last step (None before the first)
 
 
 class SegmentServer:
@@ -87,7 +89,7 @@ class SegmentServer:
         eng = Engine(reps, precision=self.precision, tuning=self.tuning)
         eng.put_state([(s.x, s.v) for s in slots])
         eng.set_steps([s.step for s in slots])
-        eng.resume()
+        eng.resume([s.f for s in slots])
         self.rebatches += 1
         return eng
 
@@ -161,9 +163,11 @@ class SegmentServer:
     @staticmethod
     def _harvest(eng, slots):
         states = eng.state()
+        forces = eng.forces()
         steps = eng.steps()
         for k, s in enumerate(slots):
             s.x, s.v = states[k]
+            s.f = forces[k]
             s.step = int(steps[k])
 
 
