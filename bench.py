@@ -67,6 +67,19 @@ def make_systems(specs, workers):
     return [_make(s) for s in specs]
 
 
+def prepare_engine(eng, minimize, equil, tune=True):
+    """Untimed set-up of a benchmark engine: capped steepest descent, Langevin
+    equilibration at strong friction (gamma 20/ps) back to the replicas' temperatures, then
+    the replicas' own friction and the autotuned skin / lanes.  Returns the tuning."""
+    eng.init()
+    eng.minimize(minimize)
+    if equil > 0:
+        eng.set_thermostat(gammas=[20.0] * len(eng.reps))
+        eng.run(equil)
+        eng.set_thermostat()
+    return eng.autotune() if tune else None
+
+
 def time_engine(eng, steps, warmup):
     eng.run(max(1, warmup))
     eng.synchronize()
@@ -172,6 +185,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--minimize", type=int, default=150)
+    ap.add_argument("--equil", type=int, default=400,
+                    help="untimed Langevin steps at gamma 20/ps after the minimisation")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--workers", type=int, default=0,
                     help="input-generation processes (0 = one per core up to 16; use 1 under a "
                          "profiler, which initialises the GPU before the program starts)")
@@ -217,13 +233,15 @@ def main():
     log(f"[plain] {plain_sys.np} atoms, FP64")
     eng = host.Engine([host.ReplicaSpec(plain_sys, temp=300.0, gamma_ps=1.0, seed=1)],
                       precision=host.PREC_FP64)
-    eng.init()
-    eng.minimize(args.minimize)
+    tune_p = prepare_engine(eng, args.minimize, args.equil, not args.no_tune)
+    log(f"[plain] tuned (skin, lanes, ms/step) = {tune_p}, T = {eng.temperatures()[0]:.1f} K")
     dt_plain = time_engine(eng, K, W)
     plain_rate = plain_sys.np * K / dt_plain
     ms_p, pairs_p = eng.profile_nonbond(10)
+    temp_p = float(eng.temperatures()[0])
+    phases_p = eng.profile_step(30)
     log(f"[plain] {plain_rate / 1e6:.1f} Matom-steps/s  ({1e3 * dt_plain / K:.3f} ms/step, "
-        f"pair kernel {ms_p:.3f} ms, {pairs_p / plain_sys.np:.1f} pairs/atom in cutoff)")
+        f"pair kernel {ms_p:.3f} ms, {pairs_p / plain_sys.np:.1f} pairs/atom in cutoff, T {temp_p:.1f} K)")
     eng.close()
 
     # ---------------------------------------------------------------- batched: 64 replicas mixed
@@ -233,14 +251,20 @@ def main():
     specs = [host.ReplicaSpec(r, temp=float(rng.uniform(280, 320)), gamma_ps=1.0, seed=100 + k)
              for k, r in enumerate(reps)]
     eng = host.Engine(specs, precision=host.PREC_MIXED)
-    eng.init()
-    eng.minimize(args.minimize)
+    tune_b = prepare_engine(eng, args.minimize, args.equil, not args.no_tune)
+    log(f"[batched] tuned (skin, lanes, ms/step) = {tune_b}, "
+        f"T = {eng.temperatures().mean():.1f} K (mean)")
+    b0 = eng.rebuilds()
     dt_b = time_engine(eng, K, W)
     batched_rate = natoms * K / dt_b
+    rebuilds = eng.rebuilds() - b0
+    temps_b = eng.temperatures()
     ms_b, pairs_b = eng.profile_nonbond(10)
-    rebuilds = eng.rebuilds()
+    phases_b = eng.profile_step(30)
+    log("[batched] phases (ms/step, eager): " + ", ".join(f"{k} {v:.3f}" for k, v in phases_b.items()))
     log(f"[batched] {batched_rate / 1e6:.1f} Matom-steps/s ({1e3 * dt_b / K:.3f} ms/step, "
-        f"pair kernel {ms_b:.3f} ms)")
+        f"pair kernel {ms_b:.3f} ms, rebuilds/replica {rebuilds.mean():.1f} over {K + max(1, W)} steps, "
+        f"T {temps_b.mean():.1f} K)")
     eng.close()
 
     # force error of the mixed kernel against FP64 on a replica-sized system
@@ -292,9 +316,17 @@ def main():
                         f"log-uniform ({natoms} atoms), FP32 pair math + int64 fixed-point "
                         "sums, FP64 state, Langevin 1/ps, SHAKE, dt 2 fs, hipGraph steps",
             "replicas": len(reps), "atoms": natoms,
+            "temperature_K_mean": round(float(temps_b.mean()), 1),
+            "phase_ms_eager": {k: round(v, 4) for k, v in phases_b.items()},
+            "tuning": None if tune_b is None else {"skin": tune_b[0], "lanes": tune_b[1]},
+            "preparation": f"minimize {args.minimize}, Langevin gamma 20/ps for {args.equil} steps "
+                           "(untimed)",
             "plain": {"workload": "204,000-atom glycine/water, FP64 pair math and state",
                       "value": round(plain_rate / 1e6, 3), "unit": "Matom-steps/s",
-                      "ms_per_step": round(1e3 * dt_plain / K, 4), "dtype": "f64"},
+                      "ms_per_step": round(1e3 * dt_plain / K, 4), "dtype": "f64",
+                      "temperature_K": round(temp_p, 1),
+                      "phase_ms_eager": {k: round(v, 4) for k, v in phases_p.items()},
+                      "tuning": None if tune_p is None else {"skin": tune_p[0], "lanes": tune_p[1]}},
             "served": {"workload": f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g}s, "
                                    "500-step requests, 3,000-150,000 atoms",
                        "requests": nreq, "p50_ms": p50, "p99_ms": p99,
@@ -306,6 +338,7 @@ def main():
         "p99_ms": p99,
         "force_rel_rms_mixed_vs_fp64": ferr,
         "list_rebuilds_per_replica_mean": float(np.mean(rebuilds)),
+        "list_rebuild_window_steps": K + max(1, W),
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
                      "kernel": "k_nonbond<mixed> pair kernel", "ms_per_launch": ms_b,

@@ -224,9 +224,21 @@ int sofia_get_replica_steps(sofia_ctx *ctx, uint64_t *steps);
  * force cutoff (the algorithmic pairs of one launch). */
 int sofia_profile_nonbond(sofia_ctx *ctx, int32_t launches, double *ms_per_launch,
                           int64_t *pairs_in_cutoff);
+/* Runs nsteps MD steps eagerly with HIP events between the phases of a step and returns
+ * the average ms per step of each phase in phase_ms[SOFIA_NPHASE]: half-kick, drift,
+ * displacement check, cell sort + gather + topology remap, list build, rebuild
+ * bookkeeping, force zeroing, pair kernel, bonded (1-4, bond, angle, torsion), Langevin,
+ * SHAKE, second half-kick.  Energies are not reduced on these steps. */
+enum { SOFIA_NPHASE = 12 };
+int sofia_profile_step(sofia_ctx *ctx, int32_t nsteps, double *phase_ms);
 /* Tuning knobs (autotuner): lanes per atom in the pair kernel (4..64, power of two),
  * threads per block of the pair kernel, waves per block in the list build. */
 int sofia_set_tuning(sofia_ctx *ctx, int32_t lanes_per_atom, int32_t block, int32_t list_waves);
+/* Engine Verlet skin in A (list radius rc + skin; default: the reference's pad).  The
+ * engine's lists hold every pair inside the cutoff whatever the skin, so the skin changes
+ * only speed (longer lists against more frequent rebuilds), never the trajectory.  Takes
+ * effect at once on a running engine (lists are rebuilt). */
+int sofia_set_skin(sofia_ctx *ctx, double skin);
 /* Union-find clusters of solute molecules joined by an O-N contact < rcontact (A):
  * atoms whose type is in o_types/n_types, molecule id per atom in mol (0-based within a
  * replica, -1 = solvent).  largest[r] = largest cluster (molecules) per replica. */

@@ -132,80 +132,58 @@ __global__ void k_pair14(int32_t items, const int32_t *__restrict__ dl, const fl
 }
 
 __device__ inline void dvec64(const double *xd, int n, int a, int b, double box, double ibox, double *o) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const double v = xd[(size_t)c * n + a] - xd[(size_t)c * n + b];
-        o[c] = v - box * rint(v * ibox);
-    }
+    dvec_soa(xd, n, a, b, box, ibox, o);
 }
 
-// bond_harmonic, Source/mod_force_bond.f90:24-45: E = k (r - r0)^2
+// bond_harmonic, Source/mod_force_bond.f90:24-45: E = k (r - r0)^2.  Items whose type is
+// <= 0 belong to a rigid water and are evaluated inside the SHAKE kernel instead.
 __global__ void k_bond(int32_t items, const int32_t *__restrict__ bl, const double *__restrict__ xd,
                        const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
                        const double *__restrict__ kb, const double *__restrict__ r0, int32_t n,
                        long long *__restrict__ fout, long long *__restrict__ eacc) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = m < items;
-    double e = 0.0;
+    const int t = (m < items) ? bl[(size_t)m * 3 + 2] - 1 : -1;
+    const bool live = t >= 0;
+    long long e = 0;
     int r = 0;
     if (live) {
-        const int i1 = bl[(size_t)m * 3], i2 = bl[(size_t)m * 3 + 1], t = bl[(size_t)m * 3 + 2] - 1;
+        const int i1 = bl[(size_t)m * 3], i2 = bl[(size_t)m * 3 + 1];
         r = rep[i1];
-        double d[3];
-        dvec64(xd, n, i1, i2, reps[r].box, reps[r].ibox, d);
-        const double rr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        const double dr = rr - r0[t];
-        const double rk = kb[t] * dr;
-        e = rk * dr;
-        const double force = (rr > 0.0) ? -2.0 * rk / rr : 0.0;
+        long long f[3];
+        e = bond_term(xd, n, i1, i2, reps[r].box, reps[r].ibox, kb[t], r0[t], f);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            atomic_fix(&fout[(size_t)c * n + i1], d[c] * force);
-            atomic_fix(&fout[(size_t)c * n + i2], -d[c] * force);
+            atomicAdd((unsigned long long *)&fout[(size_t)c * n + i1], (unsigned long long)f[c]);
+            atomicAdd((unsigned long long *)&fout[(size_t)c * n + i2], (unsigned long long)(-f[c]));
         }
     }
-    wave_add(to_efix(e), live, r, eacc, E_BOND);
+    wave_add(e, live, r, eacc, E_BOND);
 }
 
-// angle_harmonic, Source/mod_force_angle.f90:29-90: E = k (theta - theta0)^2
+// angle_harmonic, Source/mod_force_angle.f90:29-90: E = k (theta - theta0)^2 (types <= 0:
+// rigid-water angles, evaluated inside the SHAKE kernel)
 __global__ void k_angle(int32_t items, const int32_t *__restrict__ al, const double *__restrict__ xd,
                         const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
                         const double *__restrict__ ka, const double *__restrict__ th0, int32_t n,
                         long long *__restrict__ fout, long long *__restrict__ eacc) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = m < items;
-    double e = 0.0;
+    const int t = (m < items) ? al[(size_t)m * 4 + 3] - 1 : -1;
+    const bool live = t >= 0;
+    long long e = 0;
     int r = 0;
     if (live) {
         const int i1 = al[(size_t)m * 4], i2 = al[(size_t)m * 4 + 1], i3 = al[(size_t)m * 4 + 2];
-        const int t = al[(size_t)m * 4 + 3] - 1;
         r = rep[i1];
-        double d1[3], d2[3];
-        dvec64(xd, n, i1, i2, reps[r].box, reps[r].ibox, d1);
-        dvec64(xd, n, i3, i2, reps[r].box, reps[r].ibox, d2);
-        const double rsq1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
-        const double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
-        const double r1 = sqrt(rsq1), r2 = sqrt(rsq2);
-        double c = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) / (r1 * r2);
-        c = fmin(1.0, fmax(-1.0, c));
-        double s = sqrt(1.0 - c * c);
-        if (s < 0.001) s = 0.001;
-        s = 1.0 / s;
-        const double dth = acos(c) - th0[t];
-        const double tk = ka[t] * dth;
-        e = tk * dth;
-        const double a = 2.0 * tk * s;
-        const double a11 = a * c / rsq1, a12 = -a / (r1 * r2), a22 = a * c / rsq2;
+        long long f1[3], f3[3];
+        e = angle_term(xd, n, i1, i2, i3, reps[r].box, reps[r].ibox, ka[t], th0[t], f1, f3);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double v1 = a11 * d1[k] + a12 * d2[k];
-            const double v2 = a22 * d2[k] + a12 * d1[k];
-            atomic_fix(&fout[(size_t)k * n + i1], -v1);
-            atomic_fix(&fout[(size_t)k * n + i2], v1 + v2);
-            atomic_fix(&fout[(size_t)k * n + i3], -v2);
+            atomicAdd((unsigned long long *)&fout[(size_t)k * n + i1], (unsigned long long)f1[k]);
+            atomicAdd((unsigned long long *)&fout[(size_t)k * n + i2], (unsigned long long)(-f1[k] - f3[k]));
+            atomicAdd((unsigned long long *)&fout[(size_t)k * n + i3], (unsigned long long)f3[k]);
         }
     }
-    wave_add(to_efix(e), live, r, eacc, E_ANGLE);
+    wave_add(e, live, r, eacc, E_ANGLE);
 }
 
 // Torsion E = K (1 + cos(n phi - shift)) (the AMBER form of Source/mod_force_dihed_amber.f90)

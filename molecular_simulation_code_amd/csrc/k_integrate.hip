@@ -374,62 +374,274 @@ __global__ void k_langevin64(int32_t n, const float4 *__restrict__ pos4, const d
     }
 }
 
-__global__ void k_shake64(int32_t ns, const int32_t *__restrict__ num, const int32_t *__restrict__ atoms,
-                          const double *__restrict__ bond, const float4 *__restrict__ pos4,
-                          const double *__restrict__ xd, const double *__restrict__ vd,
-                          const double *__restrict__ fnb, long long *__restrict__ fix,
-                          const double *__restrict__ mass, const int32_t *__restrict__ rep,
-                          const RepInfo *__restrict__ reps, int32_t n, double dt, double dtsq,
-                          double tol, int maxiter, int32_t *__restrict__ err,
-                          int32_t *__restrict__ iters_max, long long *__restrict__ eacc) {
+// Engine SHAKE.  The solver is instantiated per cluster kind so the constraint topology is
+// known at compile time (the arrays stay in registers); the arithmetic is the one of
+// shake_solve.  Cluster atoms are exclusive to their cluster.
+template <int KIND> __device__ constexpr int topo_nc() { return KIND == 1 ? 1 : (KIND == 3 ? 2 : 3); }
+template <int KIND> __device__ constexpr int topo_na() { return KIND == 1 ? 2 : (KIND == 4 ? 4 : 3); }
+template <int KIND> __device__ constexpr int topo_p(int k) { return (KIND == -3 && k == 2) ? 1 : 0; }
+template <int KIND> __device__ constexpr int topo_q(int k) { return (KIND == -3 && k == 2) ? 2 : k + 1; }
+
+template <int KIND>
+__device__ inline bool shake_solve_t(const double *im, const double *bl, const double (*r)[3],
+                                     const double (*s)[3], double tol, int maxiter, double *lam,
+                                     int *iters) {
+    constexpr int NC = topo_nc<KIND>();
+    if constexpr (NC == 1) {
+        const double r01sq = r[0][0] * r[0][0] + r[0][1] * r[0][1] + r[0][2] * r[0][2];
+        const double s01sq = s[0][0] * s[0][0] + s[0][1] * s[0][1] + s[0][2] * s[0][2];
+        const double ims = im[0] + im[1];
+        const double a = ims * ims * r01sq;
+        const double b = 2.0 * ims * (s[0][0] * r[0][0] + s[0][1] * r[0][1] + s[0][2] * r[0][2]);
+        const double cc = s01sq - bl[0] * bl[0];
+        const double det = b * b - 4.0 * a * cc;
+        if (det < 0.0) return false;
+        const double l1 = (-b + sqrt(det)) / (2.0 * a), l2 = (-b - sqrt(det)) / (2.0 * a);
+        lam[0] = (fabs(l1) <= fabs(l2)) ? l1 : l2;
+        *iters = 1;
+        return true;
+    } else {
+        double cpl[NC][NC], A[NC][NC], G[NC][NC], ssq[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            ssq[k] = s[k][0] * s[k][0] + s[k][1] * s[k][1] + s[k][2] * s[k][2];
+#pragma unroll
+            for (int l = 0; l < NC; ++l) {
+                double c = 0.0;
+                if (topo_p<KIND>(k) == topo_p<KIND>(l)) c += im[topo_p<KIND>(k)];
+                if (topo_p<KIND>(k) == topo_q<KIND>(l)) c -= im[topo_p<KIND>(k)];
+                if (topo_q<KIND>(k) == topo_p<KIND>(l)) c -= im[topo_q<KIND>(k)];
+                if (topo_q<KIND>(k) == topo_q<KIND>(l)) c += im[topo_q<KIND>(k)];
+                cpl[k][l] = c;
+                A[k][l] = 2.0 * c * (s[k][0] * r[l][0] + s[k][1] * r[l][1] + s[k][2] * r[l][2]);
+                G[k][l] = r[k][0] * r[l][0] + r[k][1] * r[l][1] + r[k][2] * r[l][2];
+            }
+        }
+        double Ai[NC][NC];
+        if constexpr (NC == 2) {
+            const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+            if (det == 0.0) return false;
+            const double di = 1.0 / det;
+            Ai[0][0] = A[1][1] * di; Ai[0][1] = -A[0][1] * di;
+            Ai[1][0] = -A[1][0] * di; Ai[1][1] = A[0][0] * di;
+        } else {
+            const double det = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] +
+                               A[0][2] * A[1][0] * A[2][1] - A[0][0] * A[1][2] * A[2][1] -
+                               A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
+            if (det == 0.0) return false;
+            const double di = 1.0 / det;
+            Ai[0][0] = di * (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
+            Ai[0][1] = -di * (A[0][1] * A[2][2] - A[0][2] * A[2][1]);
+            Ai[0][2] = di * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+            Ai[1][0] = -di * (A[1][0] * A[2][2] - A[1][2] * A[2][0]);
+            Ai[1][1] = di * (A[0][0] * A[2][2] - A[0][2] * A[2][0]);
+            Ai[1][2] = -di * (A[0][0] * A[1][2] - A[0][2] * A[1][0]);
+            Ai[2][0] = di * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+            Ai[2][1] = -di * (A[0][0] * A[2][1] - A[0][1] * A[2][0]);
+            Ai[2][2] = di * (A[0][0] * A[1][1] - A[0][1] * A[1][0]);
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) lam[k] = 0.0;
+        int it = 0;
+        bool done = false;
+        while (!done && it < maxiter) {
+            double bb[NC], nl[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                double qd = 0.0;
+#pragma unroll
+                for (int l = 0; l < NC; ++l)
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) qd += cpl[k][l] * cpl[k][m] * G[l][m] * lam[l] * lam[m];
+                bb[k] = bl[k] * bl[k] - ssq[k] - qd;
+            }
+            done = true;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                double v = 0.0;
+#pragma unroll
+                for (int l = 0; l < NC; ++l) v += Ai[k][l] * bb[l];
+                nl[k] = v;
+            }
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                if (fabs(nl[k] - lam[k]) > tol) done = false;
+                lam[k] = nl[k];
+            }
+            ++it;
+        }
+        *iters = it;
+        return true;
+    }
+}
+
+struct ShakeArgs {
+    const int32_t *atoms;
+    const double *bond;
+    const int4 *water;        // per cluster: bond types (O-H1, O-H2), angle type, fused flag
+    const float4 *pos4;
+    const double *xd, *vd, *fnb;
+    long long *fix;
+    const double *mass;
+    const int32_t *rep;
+    const RepInfo *reps;
+    const double *bond_k, *bond_r0, *angle_k, *angle_th0;
+    int32_t n;
+    double dt, dtsq, tol;
+    int maxiter;
+    int32_t *err, *iters_max;
+};
+
+// The two O-H bonds and the H-O-H angle of a rigid water (atoms idx = O, H1, H2), added to
+// the atoms' fixed-point sums with the quanta the bond and angle kernels would add.
+__device__ inline void water_bonded(const int4 w, const ShakeArgs &A, const int *idx, double box,
+                                    double ibox, long long (*own)[3], long long &eb, long long &ea) {
+    long long f[3], f1[3], f3[3];
+    eb += bond_term(A.xd, A.n, idx[0], idx[1], box, ibox, A.bond_k[w.x - 1], A.bond_r0[w.x - 1], f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { own[0][c] += f[c]; own[1][c] -= f[c]; }
+    eb += bond_term(A.xd, A.n, idx[0], idx[2], box, ibox, A.bond_k[w.y - 1], A.bond_r0[w.y - 1], f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { own[0][c] += f[c]; own[2][c] -= f[c]; }
+    ea += angle_term(A.xd, A.n, idx[1], idx[0], idx[2], box, ibox, A.angle_k[w.z - 1],
+                     A.angle_th0[w.z - 1], f1, f3);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        own[1][c] += f1[c];
+        own[0][c] -= f1[c] + f3[c];
+        own[2][c] += f3[c];
+    }
+}
+
+// One cluster of kind KIND.  A fused rigid water (KIND -3, water flag set) also evaluates its
+// two O-H bonds and its H-O-H angle here -- with the same fixed-point quanta the bond and
+// angle kernels would add -- and owns its three atoms, so it writes their force sums without
+// atomics.  Returns the cluster's replica; adds to the virial and energy partial sums.
+template <int KIND>
+__device__ inline int shake_cluster(int m, const ShakeArgs &A, double vir[3], long long &eb,
+                                    long long &ea) {
+    constexpr int NA = topo_na<KIND>(), NC = topo_nc<KIND>();
+    int idx[NA];
+    double im[NA], bl[3];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        idx[k] = A.atoms[4 * (size_t)m + k];
+        im[k] = 1.0 / A.mass[__float_as_int(A.pos4[idx[k]].w) - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bl[k] = A.bond[3 * (size_t)m + k];
+    const int rr = A.rep[idx[0]];
+    const double box = A.reps[rr].box, ibox = A.reps[rr].ibox;
+    const int n = A.n;
+    long long own[NA][3];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) own[a][c] = A.fix[(size_t)c * n + idx[a]];
+    bool fused = false;
+    if constexpr (KIND == -3) {
+        const int4 w = A.water[m];
+        fused = w.w != 0;
+        if (fused) water_bonded(w, A, idx, box, ibox, own, eb, ea);
+    }
+    double xs[NA][3];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t o = (size_t)c * n + idx[a];
+            const double f = A.fnb[o] + (double)own[a][c] * FIX_INV;
+            xs[a][c] = A.xd[o] + A.dt * A.vd[o] + A.dtsq * im[a] * f;
+        }
+    double r[NC][3], s[NC][3];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const int pa = topo_p<KIND>(k), qa = topo_q<KIND>(k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double d = A.xd[(size_t)c * n + idx[pa]] - A.xd[(size_t)c * n + idx[qa]];
+            r[k][c] = d - box * rint(d * ibox);
+            const double e = xs[pa][c] - xs[qa][c];
+            s[k][c] = e - box * rint(e * ibox);
+        }
+    }
+    double lam[NC];
+    int its = 0;
+    if (!shake_solve_t<KIND>(im, bl, r, s, A.tol, A.maxiter, lam, &its)) {
+        atomicOr(A.err, 1);
+        return rr;
+    }
+    atomicMax(A.iters_max, its);
+    long long add[NA][3];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) add[a][c] = 0;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const double l = lam[k] / A.dtsq;
+        const int pa = topo_p<KIND>(k), qa = topo_q<KIND>(k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long q = to_fix(l * r[k][c]);
+            add[pa][c] += q;
+            add[qa][c] -= q;
+            vir[c] += r[k][c] * r[k][c] * l;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            long long *p = &A.fix[(size_t)c * n + idx[a]];
+            if (fused) *p = own[a][c] + add[a][c];
+            else atomicAdd((unsigned long long *)p, (unsigned long long)add[a][c]);
+        }
+    return rr;
+}
+
+__global__ void __launch_bounds__(128)
+k_shake64(int32_t ns, const int32_t *__restrict__ num, ShakeArgs A, long long *__restrict__ eacc) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = m < ns;
     double vir[3] = {0.0, 0.0, 0.0};
+    long long eb = 0, ea = 0;
     int rr = 0;
     if (live) {
-        const ShakeCluster S = make_cluster(num[m], atoms + 4 * (size_t)m, bond + 3 * (size_t)m, mass, pos4);
-        rr = rep[S.idx[0]];
-        const double box = reps[rr].box, ibox = reps[rr].ibox;
-        double xs[4][3];
-        for (int a = 0; a < 4; ++a) {
-            if (S.idx[a] < 0) continue;
-            const int i = S.idx[a];
-            for (int c = 0; c < 3; ++c) {
-                const size_t o = (size_t)c * n + i;
-                const double f = fnb[o] + (double)fix[o] * FIX_INV;
-                xs[a][c] = xd[o] + dt * vd[o] + dtsq * S.im[a] * f;
-            }
-        }
-        double r[3][3], s[3][3];
-        for (int k = 0; k < S.nc; ++k) {
-            const int a = S.idx[S.P[k]], b = S.idx[S.Q[k]];
-            for (int c = 0; c < 3; ++c) {
-                const double d = xd[(size_t)c * n + a] - xd[(size_t)c * n + b];
-                r[k][c] = d - box * rint(d * ibox);
-                const double e = xs[S.P[k]][c] - xs[S.Q[k]][c];
-                s[k][c] = e - box * rint(e * ibox);
-            }
-        }
-        double lam[3];
-        int its = 0;
-        if (!shake_solve(S, r, s, tol, maxiter, lam, &its)) {
-            atomicOr(err, 1);
-        } else {
-            atomicMax(iters_max, its);
-            for (int k = 0; k < S.nc; ++k) {
-                const double l = lam[k] / dtsq;
-                const int a = S.idx[S.P[k]], b = S.idx[S.Q[k]];
-                for (int c = 0; c < 3; ++c) {
-                    fix[(size_t)c * n + a] += to_fix(l * r[k][c]);
-                    fix[(size_t)c * n + b] -= to_fix(l * r[k][c]);
-                    vir[c] += r[k][c] * r[k][c] * l;
-                }
-            }
+        switch (num[m]) {   // kinds as in make_cluster: anything but 1, 3, 4 is the triangle
+            case 1: rr = shake_cluster<1>(m, A, vir, eb, ea); break;
+            case 3: rr = shake_cluster<3>(m, A, vir, eb, ea); break;
+            case 4: rr = shake_cluster<4>(m, A, vir, eb, ea); break;
+            default: rr = shake_cluster<-3>(m, A, vir, eb, ea); break;
         }
     }
     wave_add(to_efix(vir[0]), live, rr, eacc, V_X);
     wave_add(to_efix(vir[1]), live, rr, eacc, V_Y);
     wave_add(to_efix(vir[2]), live, rr, eacc, V_Z);
+    wave_add(eb, live, rr, eacc, E_BOND);
+    wave_add(ea, live, rr, eacc, E_ANGLE);
+}
+
+// The fused rigid-water bonded terms alone (steps that run no SHAKE: the minimiser).
+__global__ void k_water_bonded(int32_t ns, const int32_t *__restrict__ num, ShakeArgs A,
+                               long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < ns && num[m] != 1 && num[m] != 3 && num[m] != 4 && A.water[m].w != 0;
+    long long eb = 0, ea = 0;
+    int rr = 0;
+    if (live) {
+        int idx[3];
+        for (int k = 0; k < 3; ++k) idx[k] = A.atoms[4 * (size_t)m + k];
+        rr = A.rep[idx[0]];
+        long long own[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) own[a][c] = A.fix[(size_t)c * A.n + idx[a]];
+        water_bonded(A.water[m], A, idx, A.reps[rr].box, A.reps[rr].ibox, own, eb, ea);
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) A.fix[(size_t)c * A.n + idx[a]] = own[a][c];
+    }
+    wave_add(eb, live, rr, eacc, E_BOND);
+    wave_add(ea, live, rr, eacc, E_ANGLE);
 }
 
 // Capped steepest descent: every atom moves along its force by min(gamma |F|, max_disp).
@@ -544,14 +756,29 @@ void launch_langevin64(sofia_ctx *c) {
                        c->a.vd, c->a.gid, c->rep, c->reps_d, c->mass, c->step_d, c->fix);
 }
 
-void launch_shake64(sofia_ctx *c) {
-    if (!c->ns) return;
+ShakeArgs shake_args(sofia_ctx *c) {
     const double dt = c->dp.dt;
     const double dtsq = dt * dt * (double)c->dp.kcal2amu;
+    ShakeArgs A;
+    A.atoms = c->shake_atoms; A.bond = c->shake_bond; A.water = c->water;
+    A.pos4 = c->a.pos4; A.xd = c->a.xd; A.vd = c->a.vd; A.fnb = c->fnb; A.fix = c->fix;
+    A.mass = c->mass; A.rep = c->rep; A.reps = c->reps_d;
+    A.bond_k = c->bond_k; A.bond_r0 = c->bond_r0; A.angle_k = c->angle_k; A.angle_th0 = c->angle_th0;
+    A.n = c->N; A.dt = dt; A.dtsq = dtsq; A.tol = c->dp.shake_tol; A.maxiter = c->dp.shake_maxiter;
+    A.err = c->shake_err; A.iters_max = c->shake_iters;
+    return A;
+}
+
+void launch_shake64(sofia_ctx *c) {
+    if (!c->ns) return;
     hipLaunchKernelGGL(k_shake64, dim3(nblocks(c->ns, 128)), dim3(128), 0, c->stream, c->ns,
-                       c->shake_num, c->shake_atoms, c->shake_bond, c->a.pos4, c->a.xd, c->a.vd,
-                       c->fnb, c->fix, c->mass, c->rep, c->reps_d, c->N, dt, dtsq, c->dp.shake_tol,
-                       c->dp.shake_maxiter, c->shake_err, c->shake_iters, c->eacc);
+                       c->shake_num, shake_args(c), c->eacc);
+}
+
+void launch_water_bonded(sofia_ctx *c) {
+    if (!c->ns || !c->water_split) return;
+    hipLaunchKernelGGL(k_water_bonded, dim3(nblocks(c->ns, 128)), dim3(128), 0, c->stream, c->ns,
+                       c->shake_num, shake_args(c), c->eacc);
 }
 
 void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp) {

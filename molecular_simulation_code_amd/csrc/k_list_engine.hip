@@ -26,12 +26,16 @@ namespace {
 constexpr int ECAP = 4096;            // staged candidates per pass (64 KiB of LDS)
 constexpr float CULL_MARGIN = 0.01f;  // A
 
-// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run
-// of cells so neighbouring cells share that XCD's L2.
+// Workgroups are dispatched round-robin over the 8 XCDs.  Hand each XCD runs of XCD_RUN
+// consecutive cells (neighbouring cells then share that XCD's L2), the runs dealt out
+// round-robin, so that a single flagged replica -- a contiguous range of cells -- still
+// spreads over every XCD.  The tail beyond a whole number of 8-run rounds maps to itself.
+constexpr int XCD_RUN = 16;
 __device__ inline int xcd_cell(int b, int ncell) {
-    const int q = ncell >> 3, rem = ncell & 7;
+    const int round = 8 * XCD_RUN;
+    if (b >= (ncell / round) * round) return b;
     const int x = b & 7, k = b >> 3;
-    return x * q + min(x, rem) + k;
+    return ((k / XCD_RUN) * 8 + x) * XCD_RUN + (k % XCD_RUN);
 }
 
 template <int WAVES>

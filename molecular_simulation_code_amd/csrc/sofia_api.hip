@@ -179,6 +179,99 @@ int to_global_list(const sofia_ctx *c, const int32_t *src, int items, int width,
     return SOFIA_OK;
 }
 
+int sync_check(sofia_ctx *c);
+
+// Rigid waters (SHAKE kind -3 whose atoms carry exactly two bonds to the first atom and one
+// angle at it, and no other bonded term) get their bonds and angle evaluated inside the
+// engine's SHAKE kernel: the table `water` records the types per cluster and the list items
+// are marked by a non-positive type so the bond/angle kernels skip them.
+int unsplit_water(sofia_ctx *c) {
+    if (!c->water_split) return SOFIA_OK;
+    TRY(sync_check(c));
+    std::vector<int32_t> b((size_t)3 * c->nb), a((size_t)4 * c->na);
+    TRY(download(b.data(), c->bondlist, sizeof(int32_t) * b.size()));
+    TRY(download(a.data(), c->anglelist, sizeof(int32_t) * a.size()));
+    for (int m = 0; m < c->nb; ++m) if (b[3 * (size_t)m + 2] <= 0) b[3 * (size_t)m + 2] = -b[3 * (size_t)m + 2];
+    for (int m = 0; m < c->na; ++m) if (a[4 * (size_t)m + 3] <= 0) a[4 * (size_t)m + 3] = -a[4 * (size_t)m + 3];
+    TRY(upload(c->bondlist, b.data(), sizeof(int32_t) * b.size()));
+    TRY(upload(c->anglelist, a.data(), sizeof(int32_t) * a.size()));
+    if (c->water) SOFIA_HIP(hipMemset(c->water, 0, sizeof(int4) * c->ns));
+    c->water_split = false;
+    return SOFIA_OK;
+}
+
+int split_water(sofia_ctx *c) {
+    if (c->water_split || !c->ns || !c->nb || !c->na || !c->bond_k || !c->angle_k) return SOFIA_OK;
+    TRY(sync_check(c));
+    const int n = c->N;
+    std::vector<int32_t> b((size_t)3 * c->nb), a((size_t)4 * c->na), num(c->ns), at((size_t)4 * c->ns);
+    TRY(download(b.data(), c->bondlist, sizeof(int32_t) * b.size()));
+    TRY(download(a.data(), c->anglelist, sizeof(int32_t) * a.size()));
+    TRY(download(num.data(), c->shake_num, sizeof(int32_t) * num.size()));
+    TRY(download(at.data(), c->shake_atoms, sizeof(int32_t) * at.size()));
+    std::vector<int32_t> nbond(n, 0), nang(n, 0), nother(n, 0), ncl(n, 0), bond_of(n, -1);
+    for (int m = 0; m < c->nb; ++m)
+        for (int k = 0; k < 2; ++k) {
+            const int i = b[3 * (size_t)m + k];
+            if (i >= 0) { ++nbond[i]; bond_of[i] = m; }
+        }
+    std::vector<int32_t> ang_at(n, -1);
+    for (int m = 0; m < c->na; ++m)
+        for (int k = 0; k < 3; ++k) {
+            const int i = a[4 * (size_t)m + k];
+            if (i >= 0) { ++nang[i]; ang_at[i] = m; }
+        }
+    auto touch = [&](const int32_t *lst, int items, int width, int nidx) -> int {
+        if (!items) return SOFIA_OK;
+        std::vector<int32_t> t((size_t)width * items);
+        TRY(download(t.data(), lst, sizeof(int32_t) * t.size()));
+        for (int m = 0; m < items; ++m)
+            for (int k = 0; k < nidx; ++k) {
+                const int i = t[(size_t)width * m + k];
+                if (i >= 0) ++nother[i];
+            }
+        return SOFIA_OK;
+    };
+    TRY(touch(c->dihedlist, c->nd, 5, 4));
+    TRY(touch(c->torsions, c->ntor, 5, 4));
+    for (int m = 0; m < c->ns; ++m)
+        for (int k = 0; k < 4; ++k) {
+            const int i = at[4 * (size_t)m + k];
+            if (i >= 0) ++ncl[i];
+        }
+    std::vector<int4> w(c->ns, int4{0, 0, 0, 0});
+    int fused = 0;
+    for (int m = 0; m < c->ns; ++m) {
+        if (num[m] != -3) continue;
+        const int o = at[4 * (size_t)m], h1 = at[4 * (size_t)m + 1], h2 = at[4 * (size_t)m + 2];
+        if (o < 0 || h1 < 0 || h2 < 0) continue;
+        if (nbond[o] != 2 || nbond[h1] != 1 || nbond[h2] != 1) continue;
+        if (nang[o] != 1 || nang[h1] != 1 || nang[h2] != 1) continue;
+        if (nother[o] || nother[h1] || nother[h2] || ncl[o] != 1 || ncl[h1] != 1 || ncl[h2] != 1) continue;
+        const int b1 = bond_of[h1], b2 = bond_of[h2], an = ang_at[o];
+        auto pair_is = [&](int bm, int x, int y) {
+            const int p = b[3 * (size_t)bm], q = b[3 * (size_t)bm + 1];
+            return (p == x && q == y) || (p == y && q == x);
+        };
+        if (!pair_is(b1, o, h1) || !pair_is(b2, o, h2)) continue;
+        const int e1 = a[4 * (size_t)an], ap = a[4 * (size_t)an + 1], e3 = a[4 * (size_t)an + 2];
+        if (ap != o || !((e1 == h1 && e3 == h2) || (e1 == h2 && e3 == h1))) continue;
+        const int t1 = b[3 * (size_t)b1 + 2], t2 = b[3 * (size_t)b2 + 2], ta = a[4 * (size_t)an + 3];
+        if (t1 < 1 || t1 > c->nbt || t2 < 1 || t2 > c->nbt || ta < 1 || ta > c->nat) continue;
+        w[m] = int4{t1, t2, ta, 1};
+        b[3 * (size_t)b1 + 2] = -t1;
+        b[3 * (size_t)b2 + 2] = -t2;
+        a[4 * (size_t)an + 3] = -ta;
+        ++fused;
+    }
+    if (!fused) return SOFIA_OK;
+    TRY(upload(c->bondlist, b.data(), sizeof(int32_t) * b.size()));
+    TRY(upload(c->anglelist, a.data(), sizeof(int32_t) * a.size()));
+    TRY(upload(c->water, w.data(), sizeof(int4) * w.size()));
+    c->water_split = true;
+    return SOFIA_OK;
+}
+
 void fill_devparams(sofia_ctx *c) {
     const sofia_params &p = c->p;
     sofia::DevParams &d = c->dp;
@@ -223,6 +316,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     }
     c->N = atom0;
     c->ncell_total = cell0;
+    c->ncell_cap = cell0;
     c->last_e.assign((size_t)nrep * 8, 0.0);
     const int n = c->N;
     TRY(alloc_atoms(c->a, n));
@@ -367,7 +461,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep); dfree(c->acell);
     dfree(c->aslot); dfree(c->perm); dfree(c->tt); dfree(c->atombin); dfree(c->sorted_ids);
     dfree(c->nlist); dfree(c->numneigh); dfree(c->maxcnt); dfree(c->rebuild); dfree(c->nbuilds);
-    dfree(c->shake_err); dfree(c->shake_iters); dfree(c->bondlist); dfree(c->anglelist);
+    dfree(c->shake_err); dfree(c->shake_iters); dfree(c->water); dfree(c->bondlist); dfree(c->anglelist);
     dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms);
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
@@ -435,6 +529,7 @@ int sofia_set_bonded(sofia_ctx *c, const int32_t *bl, int32_t nb, const int32_t 
                      const int32_t *al, int32_t na, const int32_t *na_rep, const int32_t *dl,
                      int32_t nd, const int32_t *nd_rep) {
     if (!c || nb < 0 || na < 0 || nd < 0) return fail(SOFIA_E_ARG, "bad argument");
+    TRY(unsplit_water(c));
     std::vector<int32_t> g;
     dfree(c->bondlist); dfree(c->anglelist); dfree(c->dihedlist);
     c->nb = c->na = c->nd = 0;
@@ -464,6 +559,7 @@ int sofia_set_bonded_coeffs(sofia_ctx *c, int32_t nbt, const double *bk, const d
                             const double *tk, const double *tn, const double *ts,
                             const int32_t *tors, int32_t ntor, const int32_t *ntor_rep) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
+    TRY(unsplit_water(c));
     dfree(c->bond_k); dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0);
     dfree(c->tors_k); dfree(c->tors_n); dfree(c->tors_shift); dfree(c->torsions);
     c->ntor = 0;
@@ -492,7 +588,8 @@ int sofia_set_bonded_coeffs(sofia_ctx *c, int32_t nbt, const double *bk, const d
 int sofia_set_shake(sofia_ctx *c, int32_t ns, const int32_t *ns_rep, const int32_t *num,
                     const int32_t *atoms, const double *bond) {
     if (!c || ns < 0) return fail(SOFIA_E_ARG, "bad argument");
-    dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_bond);
+    TRY(unsplit_water(c));
+    dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_bond); dfree(c->water);
     c->ns = 0;
     if (!ns) return SOFIA_OK;
     for (int k = 0; k < ns; ++k)
@@ -511,6 +608,8 @@ int sofia_set_shake(sofia_ctx *c, int32_t ns, const int32_t *ns_rep, const int32
     TRY(upload(c->shake_num, num, sizeof(int32_t) * ns));
     TRY(upload(c->shake_atoms, g.data(), sizeof(int32_t) * g.size()));
     TRY(upload(c->shake_bond, bond, sizeof(double) * 3 * ns));
+    TRY(dalloc(&c->water, ns));
+    SOFIA_HIP(hipMemset(c->water, 0, sizeof(int4) * ns));
     c->ns = ns;
     return SOFIA_OK;
 }
@@ -906,9 +1005,11 @@ int engine_regrid(sofia_ctx *c) {
     if (c->engine_grid) return SOFIA_OK;
     TRY(sync_check(c));
     const double rc = std::sqrt(std::max(c->p.rcut2, c->p.cut_coulsq));
-    const double rv = std::sqrt(c->p.rv2);
+    const double rv = c->skin >= 0.0 ? rc + c->skin : std::sqrt(c->p.rv2);
     int cell0 = 0, n5 = 0;
     for (auto &R : c->reps_h) {
+        // start from the reference grid every time (the skin may have changed)
+        cell_grid(R.box, c->p.rcut, c->p.cut_coul, &R.rn, &R.ncellT, &R.ncellD);
         const int nh = (int)((double)R.box / (0.5 * rv));
         const int n1 = (int)((double)R.box / rv);
         double reach = R.rn;
@@ -930,7 +1031,7 @@ int engine_regrid(sofia_ctx *c) {
         R.cell0 = cell0;
         cell0 += R.ncellT;
     }
-    if (cell0 > c->ncell_total) {   // finer cells: re-size the per-cell arrays
+    if (cell0 > c->ncell_cap) {   // finer cells: re-size the per-cell arrays
         dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep);
         TRY(dalloc(&c->ccount, cell0));
         TRY(dalloc(&c->cstart, cell0));
@@ -938,9 +1039,11 @@ int engine_regrid(sofia_ctx *c) {
         TRY(dalloc(&c->cell_rep, cell0));
         SOFIA_HIP(hipMemset(c->ccount, 0, sizeof(int32_t) * cell0));
         SOFIA_HIP(hipMemset(c->cstart, 0, sizeof(int32_t) * cell0));
+        c->ncell_cap = cell0;
     }
     c->ncell_total = cell0;
     c->n_stencil5 = n5;
+    c->dp.rv2 = rv * rv;   // list radius of both build kernels from here on
     std::vector<int32_t> cnum((size_t)27 * cell0), crep(cell0);
     for (int r = 0; r < c->nrep; ++r) {
         const sofia::RepInfo &R = c->reps_h[r];
@@ -958,6 +1061,7 @@ int engine_regrid(sofia_ctx *c) {
 // and reset the engine's counters and flags.
 int engine_lists(sofia_ctx *c) {
     TRY(engine_regrid(c));
+    TRY(split_water(c));
     launch_set_rebuild(c, 1);
     TRY(rebuild_now(c, true, false));
     std::vector<int32_t> nn(c->N);
@@ -1022,13 +1126,14 @@ int sofia_get_forces_f64(sofia_ctx *c, double *f) {
     return SOFIA_OK;
 }
 
-int sofia_md_resume_forces(sofia_ctx *c, const double *f, const int32_t *have) {
-    if (!c || !f || !have) return fail(SOFIA_E_ARG, "null argument");
-    TRY(sync_check(c));
+namespace {
+
+// Where each (replica, global id) sits in the current device order.
+int gid_slots(sofia_ctx *c, std::vector<int32_t> &slot) {
     const int n = c->N;
-    // where each (replica, global id) sits in the caller's order, i.e. before the sort
-    std::vector<int32_t> g(n), slot(n, -1);
+    std::vector<int32_t> g(n);
     TRY(download(g.data(), c->a.gid, sizeof(int32_t) * n));
+    slot.assign(n, -1);
     for (const auto &R : c->reps_h)
         for (int k = 0; k < R.natom; ++k) {
             const int gi = g[R.atom0 + k];
@@ -1036,18 +1141,26 @@ int sofia_md_resume_forces(sofia_ctx *c, const double *f, const int32_t *have) {
                 return fail(SOFIA_E_STATE, "global ids of a replica are not a permutation of 1..natom");
             slot[R.atom0 + gi - 1] = R.atom0 + k;
         }
-    TRY(sofia_md_resume(c));
+    return SOFIA_OK;
+}
+
+// Set the carried force of every replica r with have[r] (nullptr = all) to f, given in the
+// device order described by `slot_before` (gid_slots taken when f was in that order).
+int put_forces(sofia_ctx *c, const double *f, const std::vector<int32_t> &slot_before,
+               const int32_t *have) {
+    const int n = c->N;
+    std::vector<int32_t> g(n);
     std::vector<double> nb((size_t)3 * n);
     std::vector<long long> fx((size_t)3 * n);
     TRY(download(g.data(), c->a.gid, sizeof(int32_t) * n));
     TRY(download(nb.data(), c->fnb, sizeof(double) * nb.size()));
     TRY(download(fx.data(), c->fix, sizeof(long long) * fx.size()));
     for (int r = 0; r < c->nrep; ++r) {
-        if (!have[r]) continue;
+        if (have && !have[r]) continue;
         const auto &R = c->reps_h[r];
         for (int k = 0; k < R.natom; ++k) {
             const int j = R.atom0 + k;
-            const size_t src = (size_t)slot[R.atom0 + g[j] - 1];
+            const size_t src = (size_t)slot_before[R.atom0 + g[j] - 1];
             for (int d = 0; d < 3; ++d) {
                 nb[(size_t)d * n + j] = f[3 * src + d];
                 fx[(size_t)d * n + j] = 0;
@@ -1057,6 +1170,17 @@ int sofia_md_resume_forces(sofia_ctx *c, const double *f, const int32_t *have) {
     TRY(upload(c->fnb, nb.data(), sizeof(double) * nb.size()));
     TRY(upload(c->fix, fx.data(), sizeof(long long) * fx.size()));
     return SOFIA_OK;
+}
+
+}  // namespace
+
+int sofia_md_resume_forces(sofia_ctx *c, const double *f, const int32_t *have) {
+    if (!c || !f || !have) return fail(SOFIA_E_ARG, "null argument");
+    TRY(sync_check(c));
+    std::vector<int32_t> slot;
+    TRY(gid_slots(c, slot));          // the caller's order: before the sort
+    TRY(sofia_md_resume(c));
+    return put_forces(c, f, slot, have);
 }
 
 int sofia_md_run(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
@@ -1094,6 +1218,7 @@ int sofia_minimize(sofia_ctx *c, int32_t nsteps, double gamma, double max_disp) 
         launch_zero_forces(c, c->fix, false);
         launch_nonbond(c, nb_mode(c), false);
         launch_bonded_engine(c);
+        launch_water_bonded(c);
         launch_minimize_step(c, gamma, max_disp);
         launch_check_neighbor(c, true);
         launch_cell_reset(c, false);
@@ -1178,6 +1303,64 @@ int sofia_set_tuning(sofia_ctx *c, int32_t lanes, int32_t block, int32_t list_wa
     c->block = block;
     c->list_waves = list_waves;
     if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return SOFIA_OK;
+}
+
+int sofia_profile_step(sofia_ctx *c, int32_t nsteps, double *phase_ms) {
+    if (!c || !phase_ms || nsteps < 1) return fail(SOFIA_E_ARG, "bad argument");
+    if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
+    constexpr int NP = SOFIA_NPHASE;
+    hipEvent_t ev[NP + 1];
+    for (int k = 0; k <= NP; ++k) SOFIA_HIP(hipEventCreate(&ev[k]));
+    std::vector<double> acc(NP, 0.0);
+    int rc = SOFIA_OK;
+    for (int s = 0; s < nsteps && rc == SOFIA_OK; ++s) {
+        int k = 0;
+        auto mark = [&]() { (void)hipEventRecord(ev[k++], c->stream); };
+        mark();
+        launch_nve_v64(c, false); mark();
+        launch_nve_x64(c); mark();
+        launch_check_neighbor(c, true); mark();
+        launch_cell_reset(c, false); launch_bin(c); launch_cell_scan(c); launch_scatter_sort(c);
+        launch_gather(c, true); launch_remap_topology(c); mark();
+        launch_build_list(c); mark();
+        launch_finish_rebuild(c, true); mark();
+        launch_zero_forces(c, c->fix, false); mark();
+        launch_nonbond(c, nb_mode(c), false); mark();
+        launch_bonded_engine(c); mark();
+        if (c->has_thermostat) launch_langevin64(c);
+        mark();
+        launch_shake64(c); mark();
+        launch_nve_v64(c, false); launch_step_advance(c); mark();
+        rc = sync_check(c);
+        for (int p = 0; p < NP && rc == SOFIA_OK; ++p) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[p], ev[p + 1]) != hipSuccess) rc = fail(SOFIA_E_HIP, "event timing");
+            acc[p] += ms;
+        }
+    }
+    for (int k = 0; k <= NP; ++k) (void)hipEventDestroy(ev[k]);
+    TRY(rc);
+    c->step_h += nsteps;
+    for (int p = 0; p < NP; ++p) phase_ms[p] = acc[p] / nsteps;
+    return check_engine_flags(c);
+}
+
+int sofia_set_skin(sofia_ctx *c, double skin) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!(skin > 0.05) || skin > 10.0) return fail(SOFIA_E_ARG, "skin must be in (0.05, 10] A");
+    c->skin = skin;
+    c->engine_grid = false;
+    if (c->engine_ready) {
+        // the re-sort permutes the atoms but not the carried forces: take them along
+        TRY(sync_check(c));
+        std::vector<double> f((size_t)3 * c->N);
+        std::vector<int32_t> slot;
+        TRY(sofia_get_forces_f64(c, f.data()));
+        TRY(gid_slots(c, slot));
+        TRY(engine_lists(c));   // rebuild counters restart from zero
+        TRY(put_forces(c, f.data(), slot, nullptr));
+    }
     return SOFIA_OK;
 }
 

@@ -53,9 +53,64 @@ __device__ inline void wave_add(long long v, bool active, int r, long long *slot
     }
 }
 
-
 __device__ inline void atomic_fix(long long *p, double v) {
     atomicAdd((unsigned long long *)p, (unsigned long long)to_fix(v));
+}
+
+// Minimum-image difference x[a] - x[b] of the FP64 SoA positions (x | y | z, n each).
+__device__ inline void dvec_soa(const double *xd, int n, int a, int b, double box, double ibox,
+                                double *o) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double v = xd[(size_t)c * n + a] - xd[(size_t)c * n + b];
+        o[c] = v - box * rint(v * ibox);
+    }
+}
+
+// Harmonic bond a-b, E = k (r - r0)^2 (Source/mod_force_bond.f90:24-45).  f receives the
+// force on a in fixed-point quanta (b gets -f); returns E in energy quanta.  Swapping a and
+// b negates d exactly, so the integers do not depend on the order of the pair.
+__device__ inline long long bond_term(const double *xd, int n, int a, int b, double box, double ibox,
+                                      double k, double r0, long long f[3]) {
+    double d[3];
+    dvec_soa(xd, n, a, b, box, ibox, d);
+    const double rr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double dr = rr - r0;
+    const double rk = k * dr;
+    const double force = (rr > 0.0) ? -2.0 * rk / rr : 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = to_fix(d[c] * force);
+    return to_efix(rk * dr);
+}
+
+// Harmonic angle a-b-c with apex b, E = k (theta - theta0)^2
+// (Source/mod_force_angle.f90:29-90).  f1 / f3 receive the forces on a / c in fixed-point
+// quanta, the apex gets -(f1 + f3) (exact momentum balance); returns E in energy quanta.
+// The expression is symmetric in a and c.
+__device__ inline long long angle_term(const double *xd, int n, int a, int b, int c, double box,
+                                       double ibox, double k, double th0, long long f1[3],
+                                       long long f3[3]) {
+    double d1[3], d2[3];
+    dvec_soa(xd, n, a, b, box, ibox, d1);
+    dvec_soa(xd, n, c, b, box, ibox, d2);
+    const double rsq1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+    const double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+    const double r1 = sqrt(rsq1), r2 = sqrt(rsq2);
+    double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) / (r1 * r2);
+    cs = fmin(1.0, fmax(-1.0, cs));
+    double sn = sqrt(1.0 - cs * cs);
+    if (sn < 0.001) sn = 0.001;
+    sn = 1.0 / sn;
+    const double dth = acos(cs) - th0;
+    const double tk = k * dth;
+    const double aa = 2.0 * tk * sn;
+    const double a11 = aa * cs / rsq1, a12 = -aa / (r1 * r2), a22 = aa * cs / rsq2;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        f1[q] = to_fix(-(a11 * d1[q] + a12 * d2[q]));
+        f3[q] = to_fix(-(a22 * d2[q] + a12 * d1[q]));
+    }
+    return to_efix(tk * dth);
 }
 
 // Philox4x32-10 (Salmon et al., SC'11): counter-based generator, so a random number is a

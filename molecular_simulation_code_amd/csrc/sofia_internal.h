@@ -118,6 +118,8 @@ struct sofia_ctx {
     int32_t *torsions = nullptr, ntor = 0;
     int32_t *shake_num = nullptr, *shake_atoms = nullptr, ns = 0;
     double *shake_bond = nullptr;
+    int4 *water = nullptr;       // per SHAKE cluster: rigid-water bond/angle types + fused flag
+    bool water_split = false;    // water bonds/angles moved into the SHAKE kernel
 
     // force field
     float4 *ljtab = nullptr;    // (lj1, lj2, lj3, lj4) per type pair
@@ -136,6 +138,7 @@ struct sofia_ctx {
     bool engine_ready = false;
     bool engine_grid = false;    // cells re-sized for complete lists (first md_init/md_resume)
     int32_t n_stencil5 = 0;      // replicas on the half-width engine grid
+    double skin = -1.0;          // engine skin (A); < 0: the reference's pad
     uint64_t *step_d = nullptr;  // per-replica step counters (Langevin key)
     uint64_t step_h = 0;
     int lanes = 16, block = 256, list_waves = 4;
@@ -189,6 +192,7 @@ void launch_nve_v64(sofia_ctx *c, bool ke);
 void launch_nve_x64(sofia_ctx *c);
 void launch_langevin64(sofia_ctx *c);
 void launch_shake64(sofia_ctx *c);
+void launch_water_bonded(sofia_ctx *c);
 void launch_step_advance(sofia_ctx *c);
 void launch_step_add(sofia_ctx *c, long long delta);
 void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp);

@@ -410,6 +410,55 @@ class Engine:
     def set_tuning(self, lanes: int, block: int, list_waves: int):
         check(lib().sofia_set_tuning(self.ctx, lanes, block, list_waves))
 
+    def autotune(self, steps: int = 80, skins=(1.0, 1.3, 1.6, 2.0, 2.5), lanes=(4, 8, 16, 32)):
+        """Pick the Verlet skin and the pair kernel's lanes per atom on the current state.
+        For each skin the lanes are chosen by timing the pair kernel alone; skins are
+        compared on whole MD steps (list length against rebuild rate).  Either knob changes
+        speed only.  Returns (skin, lanes, ms/step)."""
+        import time
+        best = None
+        for sk in skins:
+            self.set_skin(sk)
+            t_l = {}
+            for L in lanes:
+                self.set_tuning(L, 256, 4)
+                t_l[L] = self.profile_nonbond(8)[0]
+            L = min(t_l, key=t_l.get)
+            self.set_tuning(L, 256, 4)
+            self.run(5)
+            self.synchronize()
+            t0 = time.perf_counter()
+            self.run(steps)
+            self.synchronize()
+            ms = 1e3 * (time.perf_counter() - t0) / steps
+            if best is None or ms < best[2]:
+                best = (sk, L, ms)
+        self.set_skin(best[0])
+        self.set_tuning(best[1], 256, 4)
+        return best
+
+    def set_skin(self, skin: float):
+        """Engine Verlet skin (A); changes speed only, never the trajectory."""
+        check(lib().sofia_set_skin(self.ctx, float(skin)))
+
+    def set_thermostat(self, temps=None, gammas=None, seeds=None):
+        """Per-replica Langevin temperature (K), friction (1/ps) and seed; None keeps the
+        replica's spec.  A replica's noise stream stays keyed by (seed, step, atom)."""
+        n = len(self.reps)
+        temps = [r.temp for r in self.reps] if temps is None else list(temps)
+        gammas = [r.gamma_ps for r in self.reps] if gammas is None else list(gammas)
+        seeds = [r.seed for r in self.reps] if seeds is None else list(seeds)
+        th = (SofiaThermostat * n)(*[SofiaThermostat(float(t), float(g), int(s))
+                                     for t, g, s in zip(temps, gammas, seeds)])
+        check(lib().sofia_set_thermostat(self.ctx, th))
+
+    def temperatures(self) -> np.ndarray:
+        """Kinetic temperature per replica (K): 2 KE / (k_B (3N - constraints - 3))."""
+        ke = self.energies()[:, 7]
+        dof = np.array([3 * r.system.np - len(S.constraint_pairs(r.system)[0]) - 3
+                        for r in self.reps], float)
+        return 2.0 * ke / (float(S.KBOLTZ) * dof)
+
     def init(self):
         check(lib().sofia_md_init(self.ctx))
 
@@ -496,6 +545,15 @@ class Engine:
         pairs = C.c_int64()
         check(lib().sofia_profile_nonbond(self.ctx, launches, C.byref(ms), C.byref(pairs)))
         return ms.value, pairs.value
+
+    PHASES = ("half-kick", "drift", "check", "sort+gather", "list build", "rebuild flags",
+              "zero forces", "pair kernel", "bonded", "langevin", "shake", "half-kick 2")
+
+    def profile_step(self, steps: int = 50) -> dict:
+        """Average ms per step of each phase of an MD step (eager launches, HIP events)."""
+        out = np.zeros(len(self.PHASES), f64)
+        check(lib().sofia_profile_step(self.ctx, steps, ptr(out)))
+        return dict(zip(self.PHASES, out.tolist()))
 
     def largest_cluster(self, rcontact: float = 3.5) -> np.ndarray:
         o = _c([4, 5], i32)

@@ -100,6 +100,55 @@ def test_engine_list_holds_every_pair_inside_the_cutoff():
         e.close()
 
 
+def test_engine_bonded_energies_match_oracle():
+    """Bond and angle energies of the engine (rigid waters evaluated inside the SHAKE kernel,
+    the rest in the bond/angle kernels) against the oracle on the same positions, also in a
+    batch where the replicas' lists are concatenated."""
+    host = _engine()
+    from oracle import oracle as O
+    systems = [S.make_system(5, 700, seed=41), S.make_system(3, 900, seed=42)]
+    e = host.Engine(systems, precision=host.PREC_FP64)
+    try:
+        e.init()
+        E = e.energies()
+        for k, s in enumerate(systems):
+            d = s.derived()
+            kb = np.array([c[0] for c in S.BOND_COEFF]); r0 = np.array([c[1] for c in S.BOND_COEFF])
+            ka = np.array([c[0] for c in S.ANGLE_COEFF]); th = np.radians([c[1] for c in S.ANGLE_COEFF])
+            _, eb = O.bond_harmonic(s.bondlist, s.pos, s.types, kb, r0, d["box"], d["ibox"])
+            _, ea = O.angle_harmonic(s.anglelist, s.pos, s.types, ka, th, d["box"], d["ibox"])
+            say(f"[engine bonded] replica {k}: bond {E[k][4]:.6f} vs {eb:.6f}, angle {E[k][5]:.6f} vs {ea:.6f}")
+            np.testing.assert_allclose(E[k][4], eb, rtol=1e-6, atol=1e-5)
+            np.testing.assert_allclose(E[k][5], ea, rtol=1e-6, atol=1e-5)
+    finally:
+        e.close()
+
+
+def test_skin_changes_speed_not_trajectory():
+    """Mixed precision: the skin (and lanes per atom) only change speed -- the trajectory is
+    bit-identical because the lists always hold every pair inside the cutoff."""
+    host = _engine()
+    s = S.make_system(6, 1800, seed=21)
+    outs = []
+    for plan in ([], [(0, 0.6, 8), (30, 1.5, 32)]):
+        say("[engine skin] plan", plan)
+        e = host.Engine([host.ReplicaSpec(s, temp=300.0, gamma_ps=1.0, seed=9)], precision=host.PREC_MIXED)
+        try:
+            e.init()
+            done = 0
+            for at, skin, lanes in plan + [(60, None, None)]:
+                if at > done:
+                    e.run(at - done)
+                done = at
+                if skin is not None:
+                    e.set_skin(skin)
+                    e.set_tuning(lanes, 256, 4)
+            outs.append(e.state()[0])
+        finally:
+            e.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 def test_minimize_then_nve_energy_is_stable():
     host = _engine()
     s = _sys(100, 600, 31)
