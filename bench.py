@@ -5,8 +5,11 @@ Metric: aggregate atom-steps/s of batched replicas vs the plain one-system-per-l
 path, at a mixed-precision force error <= 1e-5 (relative RMS vs FP64), plus p50/p99
 latency of served MD-segment requests.
 
-* plain   -- 1,200 glycine + 64,000 water (204,000 atoms), FP64 pair arithmetic, FP64
-             state, one system per launch.
+* plain   -- the same 64 replicas advanced one system per launch (a single-replica engine
+             each, same precision and tuning, resumed from the batched run's state):
+             `batched_over_plain` is the gain of batching.
+* plain_c2 -- 1,200 glycine + 64,000 water (204,000 atoms), FP64 pair arithmetic, FP64
+             state; `plain_over_cpu` compares it with the CPU baseline.
 * batched -- 64 replicas, 5,000-60,000 atoms (log-uniform, seeded), glycine mole
              fraction 0.01-0.04, FP32 pair arithmetic with int64 fixed-point sums, FP64
              state; all replicas advance in one launch sequence (hipGraph-captured step).
@@ -36,6 +39,18 @@ METRIC = "Matom-steps/s batched vs plain @ force rel.err≤1e-5; p50/p99 segment
 FLOP_PER_PAIR = 42.0          # per ordered in-cutoff list entry (84 per unordered pair)
 PEAK_FP32 = 157.3e12          # MI355X FP32 vector/matrix peak, FLOP/s
 PEAK_FP64 = 78.6e12
+
+
+def traffic_bytes():
+    """HBM bytes per pair-kernel launch on the batched workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by the calibration copy), or
+    None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", "pair_kernel_batched64_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def log(*a):
@@ -173,6 +188,29 @@ def cpu_baseline(system, budget_s=12.0):
     return s.np * n / el, n, threads
 
 
+def profile_run(args, workers):
+    """The batched workload for a profiler: kernels of a few MD steps, 5 launches of the pair
+    kernel and 5 of the 1 GiB streaming copy whose traffic is known (2 GiB per launch)."""
+    from molecular_simulation_code_amd import system as S
+    comps = S.replica_sizes(args.replicas)
+    reps = make_systems([(g, w, 20261210 + k) for k, (g, w) in enumerate(comps)], workers)
+    from molecular_simulation_code_amd import host
+    rng = np.random.default_rng(5)
+    specs = [host.ReplicaSpec(r, temp=float(rng.uniform(280, 320)), gamma_ps=1.0, seed=100 + k)
+             for k, r in enumerate(reps)]
+    eng = host.Engine(specs, precision=host.PREC_MIXED)
+    try:
+        eng.init()
+        eng.minimize(args.minimize)
+        eng.run(max(1, args.steps))
+        ms, pairs = eng.profile_nonbond(5)
+        ms_copy = eng.profile_copy(1 << 30, 5)
+        log(f"[profile-run] {sum(r.np for r in reps)} atoms; pair kernel {ms:.4f} ms/launch, "
+            f"{pairs} entries in cutoff; copy 1 GiB {ms_copy:.4f} ms/launch")
+    finally:
+        eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +226,9 @@ def main():
     ap.add_argument("--equil", type=int, default=400,
                     help="untimed Langevin steps at gamma 20/ps after the minimisation")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run of the batched workload for rocprofv3: a few steps, the pair "
+                         "kernel and the 1 GiB calibration copy; no JSON line")
     ap.add_argument("--workers", type=int, default=0,
                     help="input-generation processes (0 = one per core up to 16; use 1 under a "
                          "profiler, which initialises the GPU before the program starts)")
@@ -201,6 +242,10 @@ def main():
     from molecular_simulation_code_amd import system as S
     workers = args.workers if args.workers > 0 else min(16, os.cpu_count() or 1)
     K, W = args.steps, args.warmup
+
+    if args.profile_run:
+        profile_run(args, workers)
+        return
 
     # ---------------------------------------------------------------- inputs (CPU, before any GPU use)
     t = time.perf_counter()
@@ -260,12 +305,34 @@ def main():
     rebuilds = eng.rebuilds() - b0
     temps_b = eng.temperatures()
     ms_b, pairs_b = eng.profile_nonbond(10)
+    ms_copy = eng.profile_copy(1 << 30, 10)
+    snap = (eng.state(), eng.forces(), eng.steps())
     phases_b = eng.profile_step(30)
     log("[batched] phases (ms/step, eager): " + ", ".join(f"{k} {v:.3f}" for k, v in phases_b.items()))
     log(f"[batched] {batched_rate / 1e6:.1f} Matom-steps/s ({1e3 * dt_b / K:.3f} ms/step, "
         f"pair kernel {ms_b:.3f} ms, rebuilds/replica {rebuilds.mean():.1f} over {K + max(1, W)} steps, "
         f"T {temps_b.mean():.1f} K)")
     eng.close()
+
+    # ---------------------------------------------------------------- plain: the same replicas, one per launch
+    log(f"[plain/replica] the {len(reps)} replicas one system per launch, mixed precision")
+    kp, wp = max(1, K // 2), max(1, W // 2)
+    t_rep, as_rep = 0.0, 0
+    for k, spec in enumerate(specs):
+        e1 = host.Engine([spec], precision=host.PREC_MIXED)
+        try:
+            e1.put_state([snap[0][k]])
+            e1.set_steps([snap[2][k]])
+            if tune_b is not None:
+                e1.set_skin(tune_b[0])
+                e1.set_tuning(tune_b[1], 256, tune_b[3])
+            e1.resume([snap[1][k]])
+            t_rep += time_engine(e1, kp, wp)
+            as_rep += spec.system.np * kp
+        finally:
+            e1.close()
+    per_launch_rate = as_rep / t_rep
+    log(f"[plain/replica] {per_launch_rate / 1e6:.1f} Matom-steps/s ({kp} steps per replica)")
 
     # force error of the mixed kernel against FP64 on a replica-sized system
     ferr = force_error_mixed_vs_fp64(reps[0])
@@ -318,29 +385,37 @@ def main():
             "replicas": len(reps), "atoms": natoms,
             "temperature_K_mean": round(float(temps_b.mean()), 1),
             "phase_ms_eager": {k: round(v, 4) for k, v in phases_b.items()},
-            "tuning": None if tune_b is None else {"skin": tune_b[0], "lanes": tune_b[1]},
+            "tuning": None if tune_b is None else {"skin": tune_b[0], "lanes": tune_b[1],
+                                                   "list_waves": tune_b[3]},
             "preparation": f"minimize {args.minimize}, Langevin gamma 20/ps for {args.equil} steps "
                            "(untimed)",
-            "plain": {"workload": "204,000-atom glycine/water, FP64 pair math and state",
+            "plain": {"workload": f"the same {len(reps)} replicas, one system per launch (one engine "
+                                  f"each, hipGraph steps, mixed precision, batched tuning), {kp} "
+                                  "timed steps each",
+                      "value": round(per_launch_rate / 1e6, 3), "unit": "Matom-steps/s"},
+            "plain_c2": {"workload": "204,000-atom glycine/water, FP64 pair math and state",
                       "value": round(plain_rate / 1e6, 3), "unit": "Matom-steps/s",
                       "ms_per_step": round(1e3 * dt_plain / K, 4), "dtype": "f64",
                       "temperature_K": round(temp_p, 1),
                       "phase_ms_eager": {k: round(v, 4) for k, v in phases_p.items()},
-                      "tuning": None if tune_p is None else {"skin": tune_p[0], "lanes": tune_p[1]}},
+                      "tuning": None if tune_p is None else {"skin": tune_p[0], "lanes": tune_p[1],
+                                                             "list_waves": tune_p[3]}},
             "served": {"workload": f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g}s, "
                                    "500-step requests, 3,000-150,000 atoms",
                        "requests": nreq, "p50_ms": p50, "p99_ms": p99,
                        "value": None if served_rate is None else round(served_rate / 1e6, 3)},
         },
-        "batched_over_plain": round(batched_rate / plain_rate, 3),
+        "batched_over_plain": round(batched_rate / per_launch_rate, 3),
+        "batched_over_plain_c2_fp64": round(batched_rate / plain_rate, 3),
         "plain_over_cpu": None if cpu is None else round(plain_rate / 1e6 / cpu["value"], 1),
         "p50_ms": p50,
         "p99_ms": p99,
         "force_rel_rms_mixed_vs_fp64": ferr,
         "list_rebuilds_per_replica_mean": float(np.mean(rebuilds)),
         "list_rebuild_window_steps": K + max(1, W),
+        "hbm_copy_GBps": round(2.0 * (1 << 30) / (ms_copy * 1e-3) / 1e9, 1),
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic_bytes(),
                      "kernel": "k_nonbond<mixed> pair kernel", "ms_per_launch": ms_b,
                      "pairs_in_cutoff_per_launch": pairs_b, "flop_per_pair": FLOP_PER_PAIR},
         "cpu_baseline": cpu,

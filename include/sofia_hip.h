@@ -224,13 +224,18 @@ int sofia_get_replica_steps(sofia_ctx *ctx, uint64_t *steps);
  * force cutoff (the algorithmic pairs of one launch). */
 int sofia_profile_nonbond(sofia_ctx *ctx, int32_t launches, double *ms_per_launch,
                           int64_t *pairs_in_cutoff);
+/* Times `launches` plain device-to-device copies of `bytes` (a multiple of 16) with the
+ * library's streaming copy kernel k_copy16: the HBM reference for achieved bandwidth and
+ * for scaling traffic counters (read `bytes`, write `bytes` per launch). */
+int sofia_profile_copy(sofia_ctx *ctx, int64_t bytes, int32_t launches, double *ms_per_launch);
 /* Runs nsteps MD steps eagerly with HIP events between the phases of a step and returns
  * the average ms per step of each phase in phase_ms[SOFIA_NPHASE]: half-kick, drift,
  * displacement check, cell sort + gather + topology remap, list build, rebuild
  * bookkeeping, force zeroing, pair kernel, bonded (1-4, bond, angle, torsion), Langevin,
- * SHAKE, second half-kick.  Energies are not reduced on these steps. */
+ * SHAKE, second half-kick.  full_rebuild != 0 makes every replica rebuild on every step
+ * (the cost of one full rebuild).  Energies are not reduced on these steps. */
 enum { SOFIA_NPHASE = 12 };
-int sofia_profile_step(sofia_ctx *ctx, int32_t nsteps, double *phase_ms);
+int sofia_profile_step(sofia_ctx *ctx, int32_t nsteps, int32_t full_rebuild, double *phase_ms);
 /* Tuning knobs (autotuner): lanes per atom in the pair kernel (4..64, power of two),
  * threads per block of the pair kernel, waves per block in the list build. */
 int sofia_set_tuning(sofia_ctx *ctx, int32_t lanes_per_atom, int32_t block, int32_t list_waves);

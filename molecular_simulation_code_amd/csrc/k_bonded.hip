@@ -264,14 +264,16 @@ void launch_bonded_engine(sofia_ctx *c) {
         hipLaunchKernelGGL(k_pair14<true>, dim3(nblocks(c->nd, 256)), dim3(256), 0, c->stream, c->nd,
                            c->dihedlist, c->a.pos4, c->a.xd, c->a.q, c->rep, c->reps_d, c->lj14,
                            c->dp, n, c->fix, c->eacc);
-    if (c->nb && c->bond_k)
-        hipLaunchKernelGGL(k_bond, dim3(nblocks(c->nb, 256)), dim3(256), 0, c->stream, c->nb,
-                           c->bondlist, c->a.xd, c->rep, c->reps_d, c->bond_k, c->bond_r0, n,
-                           c->fix, c->eacc);
-    if (c->na && c->angle_k)
-        hipLaunchKernelGGL(k_angle, dim3(nblocks(c->na, 256)), dim3(256), 0, c->stream, c->na,
-                           c->anglelist, c->a.xd, c->rep, c->reps_d, c->angle_k, c->angle_th0, n,
-                           c->fix, c->eacc);
+    // with the rigid waters fused into SHAKE, the remaining items only
+    const int nb = c->water_split ? c->nbe : c->nb, na = c->water_split ? c->nae : c->na;
+    const int32_t *bl = c->water_split ? c->ebond : c->bondlist;
+    const int32_t *al = c->water_split ? c->eangle : c->anglelist;
+    if (nb && c->bond_k)
+        hipLaunchKernelGGL(k_bond, dim3(nblocks(nb, 256)), dim3(256), 0, c->stream, nb, bl, c->a.xd,
+                           c->rep, c->reps_d, c->bond_k, c->bond_r0, n, c->fix, c->eacc);
+    if (na && c->angle_k)
+        hipLaunchKernelGGL(k_angle, dim3(nblocks(na, 256)), dim3(256), 0, c->stream, na, al, c->a.xd,
+                           c->rep, c->reps_d, c->angle_k, c->angle_th0, n, c->fix, c->eacc);
     if (c->ntor && c->tors_k)
         hipLaunchKernelGGL(k_torsion, dim3(nblocks(c->ntor, 256)), dim3(256), 0, c->stream, c->ntor,
                            c->torsions, c->a.xd, c->rep, c->reps_d, c->tors_k, c->tors_n,

@@ -622,6 +622,13 @@ k_shake64(int32_t ns, const int32_t *__restrict__ num, ShakeArgs A, long long *_
     wave_add(ea, live, rr, eacc, E_ANGLE);
 }
 
+// Plain streaming copy (16 B per thread per iteration): the HBM reference for traffic
+// counters and for the achieved-bandwidth figure of the profiling helpers.
+__global__ void k_copy16(size_t n16, const float4 *__restrict__ src, float4 *__restrict__ dst) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride) dst[k] = src[k];
+}
+
 // The fused rigid-water bonded terms alone (steps that run no SHAKE: the minimiser).
 __global__ void k_water_bonded(int32_t ns, const int32_t *__restrict__ num, ShakeArgs A,
                                long long *__restrict__ eacc) {
@@ -773,6 +780,11 @@ void launch_shake64(sofia_ctx *c) {
     if (!c->ns) return;
     hipLaunchKernelGGL(k_shake64, dim3(nblocks(c->ns, 128)), dim3(128), 0, c->stream, c->ns,
                        c->shake_num, shake_args(c), c->eacc);
+}
+
+void launch_copy16(sofia_ctx *c, size_t n16, const void *src, void *dst) {
+    hipLaunchKernelGGL(k_copy16, dim3(2048), dim3(256), 0, c->stream, n16, (const float4 *)src,
+                       (float4 *)dst);
 }
 
 void launch_water_bonded(sofia_ctx *c) {

@@ -18,25 +18,13 @@
 // over in numneigh (each atom stays with the same wavefront).
 #include <cmath>
 
-#include "sofia_internal.h"
+#include "sofia_device.h"
 
 namespace sofia {
 namespace {
 
 constexpr int ECAP = 4096;            // staged candidates per pass (64 KiB of LDS)
 constexpr float CULL_MARGIN = 0.01f;  // A
-
-// Workgroups are dispatched round-robin over the 8 XCDs.  Hand each XCD runs of XCD_RUN
-// consecutive cells (neighbouring cells then share that XCD's L2), the runs dealt out
-// round-robin, so that a single flagged replica -- a contiguous range of cells -- still
-// spreads over every XCD.  The tail beyond a whole number of 8-run rounds maps to itself.
-constexpr int XCD_RUN = 16;
-__device__ inline int xcd_cell(int b, int ncell) {
-    const int round = 8 * XCD_RUN;
-    if (b >= (ncell / round) * round) return b;
-    const int x = b & 7, k = b >> 3;
-    return ((k / XCD_RUN) * 8 + x) * XCD_RUN + (k % XCD_RUN);
-}
 
 template <int WAVES>
 __global__ void __launch_bounds__(WAVES * 64)
@@ -46,7 +34,7 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
                const RepInfo *__restrict__ reps, const int32_t *__restrict__ rebuild, float rv,
                float rv2, int32_t na, int32_t *__restrict__ nlist, int32_t *__restrict__ numneigh,
                int32_t *__restrict__ maxcnt) {
-    const int c = xcd_cell(blockIdx.x, ncell);
+    const int c = xcd_block(blockIdx.x, ncell);   // runs of cells per XCD
     const int r = cell_rep[c];
     if (!rebuild[r]) return;
     const RepInfo R = reps[r];
@@ -196,9 +184,15 @@ inline int nblk(int n, int b) { return (n + b - 1) / b; }
 void launch_build_list_engine(sofia_ctx *c) {
     const float rv2 = (float)c->dp.rv2;
     const float rv = std::sqrt(rv2);
-    hipLaunchKernelGGL(k_build_engine<4>, dim3(c->ncell_total), dim3(256), 0, c->stream, c->ncell_total,
-                       c->a.pos4, c->a.num3, c->a.spec, c->cstart, c->ccount, c->cell_rep, c->reps_d,
-                       c->rebuild, rv, rv2, c->neigh_alloc, c->nlist, c->numneigh, c->maxcnt);
+#define LBE(W)                                                                                   \
+    hipLaunchKernelGGL(k_build_engine<W>, dim3(c->ncell_total), dim3(W * 64), 0, c->stream,      \
+                       c->ncell_total, c->a.pos4, c->a.num3, c->a.spec, c->cstart, c->ccount,    \
+                       c->cell_rep, c->reps_d, c->rebuild, rv, rv2, c->neigh_alloc, c->nlist,    \
+                       c->numneigh, c->maxcnt)
+    if (c->list_waves >= 16) LBE(16);
+    else if (c->list_waves >= 8) LBE(8);
+    else LBE(4);
+#undef LBE
 }
 
 }  // namespace sofia

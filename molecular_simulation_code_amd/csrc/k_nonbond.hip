@@ -15,7 +15,9 @@
 //
 // Layout: L lanes cooperate on one atom i (L = 4..64, chosen by the autotuner); lane s
 // walks list entries s, s+L, ... so each group reads a contiguous run of the atom's list,
-// and the L partial sums are combined with cross-lane shuffles.  Positions are the
+// and the L partial sums are combined with cross-lane shuffles.  The loop is software
+// pipelined (the next entry's position and charge are in flight during this entry's math),
+// and blocks are dealt to the XCDs in runs so that an XCD's L2 holds one region of atoms.  Positions are the
 // cell-sorted float4 (x, y, z, type) array, so the j gathers of neighbouring atoms hit
 // the same cache lines.  No atomics on forces: every atom's sum is owned by one group.
 //
@@ -41,7 +43,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
     for (int t = threadIdx.x; t < nt2; t += blockDim.x) s_lj[t] = ljtab[t];
     __syncthreads();
 
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = gt / L;
     const int sub = gt % L;
     const bool live = i < n;
@@ -65,9 +67,16 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
         const double qc = (double)P.coulpre * (double)q[ii];
         const double two_a_pis = (2.0 * P.alpha) * P.my_pis_inv;
         const double a2 = P.alpha * P.alpha, pa = P.ewald_p * P.alpha;
+        int jnn = (sub + L < nn) ? lst[sub + L] : ii;
+        float4 pn = pos4[(sub < nn) ? lst[sub] : ii];
+        float qn = q[(sub < nn) ? lst[sub] : ii];
         for (int k = sub; k < nn; k += L) {
-            const int j = lst[k];
-            const float4 pj = pos4[j];
+            // software pipeline: this entry's data arrived during the previous iteration
+            const float4 pj = pn;
+            const float qj = qn;
+            pn = pos4[jnn];
+            qn = q[jnn];
+            jnn = (k + 2 * L < nn) ? lst[k + 2 * L] : ii;
             double dx = x1 - (double)pj.x, dy = y1 - (double)pj.y, dz = z1 - (double)pj.z;
             dx -= bx * __builtin_rint(dx * ib);
             dy -= bx * __builtin_rint(dy * ib);
@@ -80,7 +89,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
             if (dr2 > P.rcut2) dr6i = 0.0;
             const float4 lj = ljrow[__float_as_int(pj.w)];
             const double flj = dr6i * ((double)lj.x * dr6i - (double)lj.y);
-            double pref = qc * (double)q[j] * dri;
+            double pref = qc * (double)qj * dri;
             if (dr2 > P.cut_coulsq) pref = 0.0;
             const double erfcd = exp(-a2 * dr2);
             const double t = rcp_f64(1.0 + pa * r);
@@ -109,9 +118,16 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
         const float A1 = (float)P.a1, A2 = (float)P.a2, A3 = (float)P.a3, A4 = (float)P.a4,
                     A5 = (float)P.a5;
         const float fsh = (float)P.f_shift, esh = (float)P.e_shift;
+        int jnn = (sub + L < nn) ? lst[sub + L] : ii;
+        float4 pn = pos4[(sub < nn) ? lst[sub] : ii];
+        float qn = q[(sub < nn) ? lst[sub] : ii];
         for (int k = sub; k < nn; k += L) {
-            const int j = lst[k];
-            const float4 pj = pos4[j];
+            // software pipeline: this entry's data arrived during the previous iteration
+            const float4 pj = pn;
+            const float qj = qn;
+            pn = pos4[jnn];
+            qn = q[jnn];
+            jnn = (k + 2 * L < nn) ? lst[k + 2 * L] : ii;
             float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
             dx -= box * __builtin_rintf(dx * ibox);
             dy -= box * __builtin_rintf(dy * ibox);
@@ -124,7 +140,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
             if (dr2 > rcut2) dr6i = 0.0f;
             const float4 lj = ljrow[__float_as_int(pj.w)];
             const float flj = dr6i * (lj.x * dr6i - lj.y);
-            float pref = qc * q[j] * dri;
+            float pref = qc * qj * dri;
             if (dr2 > cut_coulsq) pref = 0.0f;
             const float erfcd = __builtin_amdgcn_exp2f(nal2e * dr2);
             const float t = __builtin_amdgcn_rcpf(1.0f + pa * r);

@@ -183,18 +183,13 @@ int sync_check(sofia_ctx *c);
 
 // Rigid waters (SHAKE kind -3 whose atoms carry exactly two bonds to the first atom and one
 // angle at it, and no other bonded term) get their bonds and angle evaluated inside the
-// engine's SHAKE kernel: the table `water` records the types per cluster and the list items
-// are marked by a non-positive type so the bond/angle kernels skip them.
+// engine's SHAKE kernel: the table `water` records the types per cluster, and the engine's
+// bond and angle kernels run over copies of the lists without those items.
 int unsplit_water(sofia_ctx *c) {
     if (!c->water_split) return SOFIA_OK;
     TRY(sync_check(c));
-    std::vector<int32_t> b((size_t)3 * c->nb), a((size_t)4 * c->na);
-    TRY(download(b.data(), c->bondlist, sizeof(int32_t) * b.size()));
-    TRY(download(a.data(), c->anglelist, sizeof(int32_t) * a.size()));
-    for (int m = 0; m < c->nb; ++m) if (b[3 * (size_t)m + 2] <= 0) b[3 * (size_t)m + 2] = -b[3 * (size_t)m + 2];
-    for (int m = 0; m < c->na; ++m) if (a[4 * (size_t)m + 3] <= 0) a[4 * (size_t)m + 3] = -a[4 * (size_t)m + 3];
-    TRY(upload(c->bondlist, b.data(), sizeof(int32_t) * b.size()));
-    TRY(upload(c->anglelist, a.data(), sizeof(int32_t) * a.size()));
+    dfree(c->ebond); dfree(c->eangle);
+    c->nbe = c->nae = 0;
     if (c->water) SOFIA_HIP(hipMemset(c->water, 0, sizeof(int4) * c->ns));
     c->water_split = false;
     return SOFIA_OK;
@@ -259,14 +254,23 @@ int split_water(sofia_ctx *c) {
         const int t1 = b[3 * (size_t)b1 + 2], t2 = b[3 * (size_t)b2 + 2], ta = a[4 * (size_t)an + 3];
         if (t1 < 1 || t1 > c->nbt || t2 < 1 || t2 > c->nbt || ta < 1 || ta > c->nat) continue;
         w[m] = int4{t1, t2, ta, 1};
-        b[3 * (size_t)b1 + 2] = -t1;
+        b[3 * (size_t)b1 + 2] = -t1;     // marks only; the copies below leave these items out
         b[3 * (size_t)b2 + 2] = -t2;
         a[4 * (size_t)an + 3] = -ta;
         ++fused;
     }
     if (!fused) return SOFIA_OK;
-    TRY(upload(c->bondlist, b.data(), sizeof(int32_t) * b.size()));
-    TRY(upload(c->anglelist, a.data(), sizeof(int32_t) * a.size()));
+    std::vector<int32_t> eb, ea;
+    for (int m = 0; m < c->nb; ++m)
+        if (b[3 * (size_t)m + 2] > 0) eb.insert(eb.end(), &b[3 * (size_t)m], &b[3 * (size_t)m] + 3);
+    for (int m = 0; m < c->na; ++m)
+        if (a[4 * (size_t)m + 3] > 0) ea.insert(ea.end(), &a[4 * (size_t)m], &a[4 * (size_t)m] + 4);
+    c->nbe = (int32_t)(eb.size() / 3);
+    c->nae = (int32_t)(ea.size() / 4);
+    TRY(dalloc(&c->ebond, eb.size()));
+    TRY(dalloc(&c->eangle, ea.size()));
+    TRY(upload(c->ebond, eb.data(), sizeof(int32_t) * eb.size()));
+    TRY(upload(c->eangle, ea.data(), sizeof(int32_t) * ea.size()));
     TRY(upload(c->water, w.data(), sizeof(int4) * w.size()));
     c->water_split = true;
     return SOFIA_OK;
@@ -461,7 +465,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep); dfree(c->acell);
     dfree(c->aslot); dfree(c->perm); dfree(c->tt); dfree(c->atombin); dfree(c->sorted_ids);
     dfree(c->nlist); dfree(c->numneigh); dfree(c->maxcnt); dfree(c->rebuild); dfree(c->nbuilds);
-    dfree(c->shake_err); dfree(c->shake_iters); dfree(c->water); dfree(c->bondlist); dfree(c->anglelist);
+    dfree(c->shake_err); dfree(c->shake_iters); dfree(c->water); dfree(c->ebond); dfree(c->eangle); dfree(c->bondlist); dfree(c->anglelist);
     dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms);
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
@@ -1280,6 +1284,35 @@ int sofia_profile_nonbond(sofia_ctx *c, int32_t launches, double *ms_per_launch,
     return sync_check(c);
 }
 
+int sofia_profile_copy(sofia_ctx *c, int64_t bytes, int32_t launches, double *ms_per_launch) {
+    if (!c || bytes < 16 || bytes % 16 || launches < 1 || !ms_per_launch) return fail(SOFIA_E_ARG, "bad argument");
+    char *a = nullptr, *b = nullptr;
+    TRY(dalloc(&a, (size_t)bytes));
+    int rc = dalloc(&b, (size_t)bytes);
+    if (rc != SOFIA_OK) { dfree(a); return rc; }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    float ms = 0.0f;
+    if (hipMemsetAsync(a, 1, (size_t)bytes, c->stream) != hipSuccess ||
+        hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) {
+        rc = fail(SOFIA_E_HIP, "copy set-up failed");
+    } else {
+        launch_copy16(c, (size_t)bytes / 16, a, b);   // warm
+        (void)hipEventRecord(t0, c->stream);
+        for (int k = 0; k < launches; ++k) launch_copy16(c, (size_t)bytes / 16, a, b);
+        (void)hipEventRecord(t1, c->stream);
+        if (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess)
+            rc = fail(SOFIA_E_HIP, "copy timing failed");
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(a);
+    dfree(b);
+    TRY(rc);
+    *ms_per_launch = (double)ms / launches;
+    return SOFIA_OK;
+}
+
 int sofia_md_energies(sofia_ctx *c, double *out) {
     if (!c || !out) return fail(SOFIA_E_ARG, "null argument");
     std::memcpy(out, c->last_e.data(), sizeof(double) * c->last_e.size());
@@ -1306,7 +1339,7 @@ int sofia_set_tuning(sofia_ctx *c, int32_t lanes, int32_t block, int32_t list_wa
     return SOFIA_OK;
 }
 
-int sofia_profile_step(sofia_ctx *c, int32_t nsteps, double *phase_ms) {
+int sofia_profile_step(sofia_ctx *c, int32_t nsteps, int32_t full_rebuild, double *phase_ms) {
     if (!c || !phase_ms || nsteps < 1) return fail(SOFIA_E_ARG, "bad argument");
     if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
     constexpr int NP = SOFIA_NPHASE;
@@ -1320,6 +1353,7 @@ int sofia_profile_step(sofia_ctx *c, int32_t nsteps, double *phase_ms) {
         mark();
         launch_nve_v64(c, false); mark();
         launch_nve_x64(c); mark();
+        if (full_rebuild) launch_set_rebuild(c, 1);
         launch_check_neighbor(c, true); mark();
         launch_cell_reset(c, false); launch_bin(c); launch_cell_scan(c); launch_scatter_sort(c);
         launch_gather(c, true); launch_remap_topology(c); mark();

@@ -57,6 +57,18 @@ __device__ inline void atomic_fix(long long *p, double v) {
     atomicAdd((unsigned long long *)p, (unsigned long long)to_fix(v));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs.  xcd_block maps dispatch order to
+// a logical block so that each XCD gets runs of XCD_RUN consecutive logical blocks (which
+// then share that XCD's L2) while any contiguous range still spreads over all XCDs.  A
+// bijection on [0, nblocks); the tail beyond whole rounds maps to itself.
+constexpr int XCD_RUN = 16;
+__device__ inline int xcd_block(int b, int nblocks) {
+    const int round = 8 * XCD_RUN;
+    if (b >= (nblocks / round) * round) return b;
+    const int x = b & 7, k = b >> 3;
+    return ((k / XCD_RUN) * 8 + x) * XCD_RUN + (k % XCD_RUN);
+}
+
 // Minimum-image difference x[a] - x[b] of the FP64 SoA positions (x | y | z, n each).
 __device__ inline void dvec_soa(const double *xd, int n, int a, int b, double box, double ibox,
                                 double *o) {

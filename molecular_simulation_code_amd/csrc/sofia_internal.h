@@ -120,6 +120,8 @@ struct sofia_ctx {
     double *shake_bond = nullptr;
     int4 *water = nullptr;       // per SHAKE cluster: rigid-water bond/angle types + fused flag
     bool water_split = false;    // water bonds/angles moved into the SHAKE kernel
+    int32_t *ebond = nullptr, nbe = 0;    // engine bond / angle lists without the fused waters
+    int32_t *eangle = nullptr, nae = 0;
 
     // force field
     float4 *ljtab = nullptr;    // (lj1, lj2, lj3, lj4) per type pair
@@ -193,6 +195,7 @@ void launch_nve_x64(sofia_ctx *c);
 void launch_langevin64(sofia_ctx *c);
 void launch_shake64(sofia_ctx *c);
 void launch_water_bonded(sofia_ctx *c);
+void launch_copy16(sofia_ctx *c, size_t n16, const void *src, void *dst);
 void launch_step_advance(sofia_ctx *c);
 void launch_step_add(sofia_ctx *c, long long delta);
 void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp);

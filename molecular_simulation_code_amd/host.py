@@ -410,11 +410,13 @@ class Engine:
     def set_tuning(self, lanes: int, block: int, list_waves: int):
         check(lib().sofia_set_tuning(self.ctx, lanes, block, list_waves))
 
-    def autotune(self, steps: int = 80, skins=(1.0, 1.3, 1.6, 2.0, 2.5), lanes=(4, 8, 16, 32)):
+    def autotune(self, steps: int = 80, skins=(1.0, 1.3, 1.6, 2.0, 2.5), lanes=(4, 8, 16, 32),
+                 waves=(4, 8, 16)):
         """Pick the Verlet skin and the pair kernel's lanes per atom on the current state.
         For each skin the lanes are chosen by timing the pair kernel alone; skins are
-        compared on whole MD steps (list length against rebuild rate).  Either knob changes
-        speed only.  Returns (skin, lanes, ms/step)."""
+        compared on whole MD steps (list length against rebuild rate); then the list build's
+        waves per workgroup on full rebuilds.  The knobs change speed only.  Returns
+        (skin, lanes, ms/step, list waves)."""
         import time
         best = None
         for sk in skins:
@@ -435,7 +437,13 @@ class Engine:
                 best = (sk, L, ms)
         self.set_skin(best[0])
         self.set_tuning(best[1], 256, 4)
-        return best
+        t_w = {}
+        for w in waves:
+            self.set_tuning(best[1], 256, w)
+            t_w[w] = self.profile_step(3, full_rebuild=True)["list build"]
+        w = min(t_w, key=t_w.get)
+        self.set_tuning(best[1], 256, w)
+        return best[0], best[1], best[2], w
 
     def set_skin(self, skin: float):
         """Engine Verlet skin (A); changes speed only, never the trajectory."""
@@ -546,13 +554,20 @@ class Engine:
         check(lib().sofia_profile_nonbond(self.ctx, launches, C.byref(ms), C.byref(pairs)))
         return ms.value, pairs.value
 
+    def profile_copy(self, nbytes: int = 1 << 30, launches: int = 10) -> float:
+        """ms per launch of a plain device copy of nbytes (read nbytes + write nbytes)."""
+        ms = C.c_double()
+        check(lib().sofia_profile_copy(self.ctx, int(nbytes), launches, C.byref(ms)))
+        return ms.value
+
     PHASES = ("half-kick", "drift", "check", "sort+gather", "list build", "rebuild flags",
               "zero forces", "pair kernel", "bonded", "langevin", "shake", "half-kick 2")
 
-    def profile_step(self, steps: int = 50) -> dict:
-        """Average ms per step of each phase of an MD step (eager launches, HIP events)."""
+    def profile_step(self, steps: int = 50, full_rebuild: bool = False) -> dict:
+        """Average ms per step of each phase of an MD step (eager launches, HIP events);
+        full_rebuild makes every replica rebuild its list on every profiled step."""
         out = np.zeros(len(self.PHASES), f64)
-        check(lib().sofia_profile_step(self.ctx, steps, ptr(out)))
+        check(lib().sofia_profile_step(self.ctx, steps, 1 if full_rebuild else 0, ptr(out)))
         return dict(zip(self.PHASES, out.tolist()))
 
     def largest_cluster(self, rcontact: float = 3.5) -> np.ndarray:

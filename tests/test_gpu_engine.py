@@ -106,7 +106,7 @@ def test_engine_bonded_energies_match_oracle():
     batch where the replicas' lists are concatenated."""
     host = _engine()
     from oracle import oracle as O
-    systems = [S.make_system(5, 700, seed=41), S.make_system(3, 900, seed=42)]
+    systems = [S.make_system(5, 1000, seed=41), S.make_system(3, 1100, seed=42)]
     e = host.Engine(systems, precision=host.PREC_FP64)
     try:
         e.init()
@@ -118,8 +118,10 @@ def test_engine_bonded_energies_match_oracle():
             _, eb = O.bond_harmonic(s.bondlist, s.pos, s.types, kb, r0, d["box"], d["ibox"])
             _, ea = O.angle_harmonic(s.anglelist, s.pos, s.types, ka, th, d["box"], d["ibox"])
             say(f"[engine bonded] replica {k}: bond {E[k][4]:.6f} vs {eb:.6f}, angle {E[k][5]:.6f} vs {ea:.6f}")
-            np.testing.assert_allclose(E[k][4], eb, rtol=1e-6, atol=1e-5)
-            np.testing.assert_allclose(E[k][5], ea, rtol=1e-6, atol=1e-5)
+            # the oracle follows the reference's real*4 coordinate differences; the engine
+            # differences FP64 positions, so the sums agree to float32 rounding
+            np.testing.assert_allclose(E[k][4], eb, rtol=2e-5, atol=1e-4)
+            np.testing.assert_allclose(E[k][5], ea, rtol=2e-5, atol=1e-4)
     finally:
         e.close()
 
