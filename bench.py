@@ -192,8 +192,18 @@ def profile_run(args, workers):
     """The batched workload for a profiler: kernels of a few MD steps, 5 launches of the pair
     kernel and 5 of the 1 GiB streaming copy whose traffic is known (2 GiB per launch)."""
     from molecular_simulation_code_amd import system as S
-    comps = S.replica_sizes(args.replicas)
-    reps = make_systems([(g, w, 20261210 + k) for k, (g, w) in enumerate(comps)], workers)
+    import pickle
+    if args.cache and os.path.exists(args.cache):
+        with open(args.cache, "rb") as f:      # written by this script
+            reps = pickle.load(f)
+    else:
+        comps = S.replica_sizes(args.replicas)
+        reps = make_systems([(g, w, 20261210 + k) for k, (g, w) in enumerate(comps)], workers)
+        if args.cache:
+            with open(args.cache, "wb") as f:
+                pickle.dump(reps, f)
+    if args.generate_only:
+        return
     from molecular_simulation_code_amd import host
     rng = np.random.default_rng(5)
     specs = [host.ReplicaSpec(r, temp=float(rng.uniform(280, 320)), gamma_ps=1.0, seed=100 + k)
@@ -226,6 +236,10 @@ def main():
     ap.add_argument("--equil", type=int, default=400,
                     help="untimed Langevin steps at gamma 20/ps after the minimisation")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--cache", default=None,
+                    help="profile run: pickle of the generated replicas (written if absent)")
+    ap.add_argument("--generate-only", action="store_true",
+                    help="profile run: only generate the replicas into --cache")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run of the batched workload for rocprofv3: a few steps, the pair "
                          "kernel and the 1 GiB calibration copy; no JSON line")
