@@ -29,18 +29,24 @@
 namespace sofia {
 namespace {
 
-constexpr int LJ_MAX = 1024;   // ntypes^2 entries held in LDS
-
 template <int L, int MODE, bool ENERGY>
 __global__ void __launch_bounds__(256)
 k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ q,
           const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
           const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh, int32_t na,
-          const float4 *__restrict__ ljtab, DevParams P, double *__restrict__ fnb,
-          float *__restrict__ ff32, long long *__restrict__ eacc) {
-    __shared__ float4 s_lj[LJ_MAX];
+          const float4 *__restrict__ ljtab, const float4 *__restrict__ tab, DevParams P,
+          double *__restrict__ fnb, float *__restrict__ ff32, long long *__restrict__ eacc) {
+    // LDS: the LJ table, then (mode 1) the DSF Coulomb force table and, with energies, the
+    // energy table -- sized at launch
+    extern __shared__ float4 s_dyn[];
+    float4 *s_lj = s_dyn;
     const int nt2 = P.ntypes * P.ntypes;
+    float4 *s_tab = s_dyn + nt2;
     for (int t = threadIdx.x; t < nt2; t += blockDim.x) s_lj[t] = ljtab[t];
+    if (MODE == 1) {
+        const int ntab = ENERGY ? 2 * P.tab_n : P.tab_n;
+        for (int t = threadIdx.x; t < ntab; t += blockDim.x) s_tab[t] = tab[t];
+    }
     __syncthreads();
 
     const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -117,7 +123,8 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
         const float c2 = (float)(MODE == 1 ? (2.0 * P.alpha) * P.my_pis_inv : P.ewald_f);
         const float A1 = (float)P.a1, A2 = (float)P.a2, A3 = (float)P.a3, A4 = (float)P.a4,
                     A5 = (float)P.a5;
-        const float fsh = (float)P.f_shift, esh = (float)P.e_shift;
+        const float tr0 = P.tab_r0, tinv = P.tab_inv_h;
+        const int tn = P.tab_n;
         int jnn = (sub + L < nn) ? lst[sub + L] : ii;
         float4 pn = pos4[(sub < nn) ? lst[sub] : ii];
         float qn = q[(sub < nn) ? lst[sub] : ii];
@@ -140,20 +147,31 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
             if (dr2 > rcut2) dr6i = 0.0f;
             const float4 lj = ljrow[__float_as_int(pj.w)];
             const float flj = dr6i * (lj.x * dr6i - lj.y);
-            float pref = qc * qj * dri;
-            if (dr2 > cut_coulsq) pref = 0.0f;
-            const float erfcd = __builtin_amdgcn_exp2f(nal2e * dr2);
-            const float t = __builtin_amdgcn_rcpf(1.0f + pa * r);
-            const float erfcc = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * erfcd;
             float fc, ec;
             if (MODE == 1) {
-                fc = pref * (erfcc * dri + c2 * erfcd + r * fsh) * r;
-                ec = pref * (erfcc - r * esh - dr2 * fsh);
+                // DSF Coulomb from the cubic table in r: force/r = C qi qj g(r)
+                float qq = qc * qj;
+                if (dr2 > cut_coulsq) qq = 0.0f;
+                const float u = (r - tr0) * tinv;
+                const int kb = min(max((int)u, 0), tn - 1);
+                const float t = u - (float)kb;
+                const float4 cg = s_tab[kb];
+                fc = qq * (cg.x + t * (cg.y + t * (cg.z + t * cg.w)));
+                if (ENERGY) {
+                    const float4 ce = s_tab[tn + kb];
+                    ec = qq * (ce.x + t * (ce.y + t * (ce.z + t * ce.w)));
+                }
             } else {
+                float pref = qc * qj * dri;
+                if (dr2 > cut_coulsq) pref = 0.0f;
+                const float erfcd = __builtin_amdgcn_exp2f(nal2e * dr2);
+                const float t = __builtin_amdgcn_rcpf(1.0f + pa * r);
+                const float erfcc = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * erfcd;
                 fc = pref * (erfcc + c2 * (alpha * r) * erfcd);
                 ec = pref * erfcc;
             }
-            const float force = (fc + flj) * dr2i;
+            // mode 1: fc is already force/r; mode 2: fc is force*r
+            const float force = (MODE == 1) ? fc + flj * dr2i : (fc + flj) * dr2i;
             if (MODE == 1) {
                 const float fs = force * PAIR_QUANT;
                 const float px = dx * fs, py = dy * fs, pz = dz * fs;
@@ -226,14 +244,16 @@ void launch_t(sofia_ctx *c, bool energies, float *ff32) {
     const int block = c->block;
     const long long threads = (long long)c->N * L;
     const int grid = (int)((threads + block - 1) / block);
+    const int nt2 = c->dp.ntypes * c->dp.ntypes;
+    const size_t lds = sizeof(float4) * (nt2 + (MODE == 1 ? (energies ? 2 : 1) * c->dp.tab_n : 0));
     if (energies)
-        hipLaunchKernelGGL((k_nonbond<L, MODE, true>), dim3(grid), dim3(block), 0, c->stream, c->N,
+        hipLaunchKernelGGL((k_nonbond<L, MODE, true>), dim3(grid), dim3(block), lds, c->stream, c->N,
                            c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
-                           c->neigh_alloc, c->ljtab, c->dp, c->fnb, ff32, c->eacc);
+                           c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
     else
-        hipLaunchKernelGGL((k_nonbond<L, MODE, false>), dim3(grid), dim3(block), 0, c->stream, c->N,
+        hipLaunchKernelGGL((k_nonbond<L, MODE, false>), dim3(grid), dim3(block), lds, c->stream, c->N,
                            c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
-                           c->neigh_alloc, c->ljtab, c->dp, c->fnb, ff32, c->eacc);
+                           c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
 }
 
 template <int MODE>

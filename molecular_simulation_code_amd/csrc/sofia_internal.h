@@ -51,7 +51,11 @@ struct DevParams {
     int32_t nrep;
     int32_t natom;
     int32_t neigh_alloc;
+    float tab_r0, tab_inv_h;     // DSF Coulomb table of the mixed kernel: r0 + k / inv_h
+    int32_t tab_n;               // bins (force table, then energy table)
 };
+
+constexpr int DSF_TAB_BINS = 1024;
 
 // Per-atom arrays that the cell sort permutes (one set; the context keeps two).
 struct AtomArrays {
@@ -129,6 +133,7 @@ struct sofia_ctx {
     double *mass = nullptr;     // per type
     double *bond_k = nullptr, *bond_r0 = nullptr, *angle_k = nullptr, *angle_th0 = nullptr;
     double *tors_k = nullptr, *tors_n = nullptr, *tors_shift = nullptr;
+    float4 *dsf_tab = nullptr;  // cubic pieces of the DSF Coulomb force / energy (mixed kernel)
     int32_t nbt = 0, nat = 0, ntt = 0;
     bool have_ff = false;
 
