@@ -188,6 +188,60 @@ def cpu_baseline(system, budget_s=12.0):
     return s.np * n / el, n, threads
 
 
+def drift_run(args, workers):
+    """6,000 glycine + 320,000 water (1,020,000 atoms), mixed precision, SHAKE, dt 2 fs:
+    minimise, Langevin-equilibrate, then --drift-steps NVE steps with the total energy
+    sampled every 250 steps, then --drift-steps NVT steps (Langevin 1/ps).  Drift is the
+    least-squares slope of the total energy, per ns and per atom."""
+    from molecular_simulation_code_amd import system as S
+    from molecular_simulation_code_amd import host
+    t = time.perf_counter()
+    s = make_systems([(6000, 320000, 20261204)], 1)[0]
+    log(f"[drift] {s.np} atoms, box {float(s.box):.2f} A, generated in {time.perf_counter() - t:.1f}s")
+    eng = host.Engine([host.ReplicaSpec(s, temp=300.0, gamma_ps=20.0, seed=11)], precision=host.PREC_MIXED)
+    out = {"workload": f"{s.np}-atom glycine/water (6,000 Gly + 320,000 water), mixed precision, "
+                       "SHAKE, dt 2 fs", "atoms": s.np}
+    try:
+        eng.init()
+        eng.minimize(args.minimize)
+        eng.run(max(0, args.equil))                      # Langevin at 20/ps
+        out["tuning"] = list(eng.autotune(steps=40))
+        eng.set_thermostat(gammas=[0.0])                 # NVE
+        sample, series = 250, []
+        eng.run(1)
+        e = eng.energies()[0]
+        series.append((0, float(e.sum()), float(e[7])))
+        t0 = time.perf_counter()
+        for k in range(args.drift_steps // sample):
+            eng.run(sample)
+            e = eng.energies()[0]
+            series.append(((k + 1) * sample, float(e.sum()), float(e[7])))
+            log(f"[drift] NVE step {(k + 1) * sample}: E {e.sum():.3f} kcal/mol, KE {e[7]:.3f}")
+        t_nve = time.perf_counter() - t0
+        st = np.array([x[0] for x in series], float)
+        et = np.array([x[1] for x in series], float)
+        ke = np.array([x[2] for x in series], float)
+        dt_ns = float(s.derived()["dt"]) * 1e-6
+        slope = np.polyfit(st * dt_ns, et, 1)[0]         # kcal/mol per ns
+        out["nve"] = {"steps": int(st[-1]), "sample_every": sample,
+                      "E_start": et[0], "E_end": et[-1], "KE_mean": float(ke.mean()),
+                      "drift_kcal_per_mol_per_ns": slope,
+                      "drift_kcal_per_mol_per_ns_per_atom": slope / s.np,
+                      "drift_per_ns_relative_to_KE": slope / float(ke.mean()),
+                      "E_rms_fluctuation": float(np.std(et - np.polyval(np.polyfit(st, et, 1), st))),
+                      "temperature_K_end": float(eng.temperatures()[0]),
+                      "ms_per_step": 1e3 * t_nve / max(1, int(st[-1]))}
+        eng.set_thermostat(temps=[300.0], gammas=[1.0])  # NVT
+        t0 = time.perf_counter()
+        eng.run(args.drift_steps)
+        out["nvt"] = {"steps": args.drift_steps, "temperature_K_end": float(eng.temperatures()[0]),
+                      "ms_per_step": 1e3 * (time.perf_counter() - t0) / args.drift_steps}
+        out["series"] = series
+    finally:
+        eng.close()
+    print(json.dumps(out))
+
+
 def profile_run(args, workers):
     """The batched workload for a profiler: kernels of a few MD steps, 5 launches of the pair
     kernel and 5 of the 1 GiB streaming copy whose traffic is known (2 GiB per launch)."""
@@ -240,6 +294,10 @@ def main():
                     help="profile run: pickle of the generated replicas (written if absent)")
     ap.add_argument("--generate-only", action="store_true",
                     help="profile run: only generate the replicas into --cache")
+    ap.add_argument("--drift", action="store_true",
+                    help="energy-conservation run on the 1.02 M-atom mixed system: NVE segment of "
+                         "--drift-steps steps, then as many NVT steps; prints one JSON line")
+    ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--profile-run", action="store_true",
                     help="short run of the batched workload for rocprofv3: a few steps, the pair "
                          "kernel and the 1 GiB calibration copy; no JSON line")
@@ -259,6 +317,9 @@ def main():
 
     if args.profile_run:
         profile_run(args, workers)
+        return
+    if args.drift:
+        drift_run(args, workers)
         return
 
     # ---------------------------------------------------------------- inputs (CPU, before any GPU use)

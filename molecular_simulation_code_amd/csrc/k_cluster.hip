@@ -59,7 +59,7 @@ __global__ void k_uf_contacts(int32_t n, const float4 *__restrict__ pos4, const 
     if (ti < 1 || ti > 32 || !((nmask >> (ti - 1)) & 1u)) return;
     const RepInfo &R = reps[rep[i]];
     const int32_t *lst = nlist + (size_t)i * na;
-    const int nn = numneigh[i];
+    const int nn = min(numneigh[i], na);   // a row never reads past its stride
     for (int k = 0; k < nn; ++k) {
         const int j = lst[k];
         const int mj = mol[j];

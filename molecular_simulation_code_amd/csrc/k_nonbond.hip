@@ -59,7 +59,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
     const int ti = __float_as_int(xi.w) - 1;
     const int r = rep[ii];
     const float box = reps[r].box, ibox = reps[r].ibox;
-    const int nn = live ? numneigh[ii] : 0;
+    const int nn = live ? min(numneigh[ii], na) : 0;   // a row never reads past its stride
     const int32_t *lst = nlist + (size_t)ii * na;
     const float4 *ljrow = s_lj + ti * P.ntypes - 1;   // index by jtype (1-based)
 
@@ -239,6 +239,182 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
     }
 }
 
+// Mixed DSF kernel with in-cutoff compaction, one atom per wavefront.
+//
+// Phase A walks the atom's list 64 entries at a time (loads software pipelined one chunk
+// ahead), forms the minimum-image separation and appends the entries inside the cutoff to
+// a per-wavefront ring in LDS (dx, dy, dz, q_j and type_j).  Phase B takes 64 queued
+// entries at a time and runs the force math with every lane busy, so no lane computes a
+// pair beyond the cutoff (about 42 % of a list with a 2 A skin).  Each pair is quantised
+// as in k_nonbond<L, 1>, so per-atom sums are exact and independent of the queue order.
+constexpr int QRING = 128;
+
+template <bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_nonbond_compact(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ q,
+                  const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
+                  const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh,
+                  int32_t na, const float4 *__restrict__ ljtab, const float4 *__restrict__ tab,
+                  DevParams P, double *__restrict__ fnb, float *__restrict__ ff32,
+                  long long *__restrict__ eacc) {
+    extern __shared__ float4 s_dyn[];
+    const int nt2 = P.ntypes * P.ntypes;
+    const int tn = P.tab_n;
+    float4 *s_lj = s_dyn;
+    float4 *s_tab = s_dyn + nt2;
+    float4 *s_ring = s_tab + (ENERGY ? 2 : 1) * tn;
+    int *s_rtype = (int *)(s_ring + 4 * QRING);
+    for (int t = threadIdx.x; t < nt2; t += blockDim.x) s_lj[t] = ljtab[t];
+    for (int t = threadIdx.x; t < (ENERGY ? 2 : 1) * tn; t += blockDim.x) s_tab[t] = tab[t];
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+    if (i >= n) return;   // whole wavefront; no barrier follows
+    float4 *ring = s_ring + wave * QRING;
+    int *rtype = s_rtype + wave * QRING;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    const float4 xi = pos4[i];
+    const int ti = __float_as_int(xi.w) - 1;
+    const RepInfo &R = reps[rep[i]];
+    const float box = R.box, ibox = R.ibox;
+    const int nn = min(numneigh[i], na);
+    const int32_t *lst = nlist + (size_t)i * na;
+    const float4 *ljrow = s_lj + ti * P.ntypes - 1;
+    const float rcut2 = (float)P.rcut2, cut_coulsq = (float)P.cut_coulsq;
+    const float cut2 = fmaxf(rcut2, cut_coulsq);
+    const float qc = P.coulpre * q[i];
+    const float tr0 = P.tab_r0, tinv = P.tab_inv_h;
+
+    long long fxi = 0, fyi = 0, fzi = 0;
+    double pot = 0.0, ecl = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+    int head = 0, tail = 0;
+
+    auto consume = [&](int cnt) {
+        if (lane < cnt) {
+            const int slot = (head + lane) & (QRING - 1);
+            const float4 e = ring[slot];
+            const int tj = rtype[slot];
+            const float dx = e.x, dy = e.y, dz = e.z;
+            const float dr2 = dx * dx + dy * dy + dz * dz;
+            const float dri = __builtin_amdgcn_rsqf(dr2);
+            const float r = dr2 * dri;
+            const float dr2i = dri * dri;
+            float dr6i = dr2i * dr2i * dr2i;
+            if (dr2 > rcut2) dr6i = 0.0f;
+            const float4 lj = ljrow[tj];
+            const float flj = dr6i * (lj.x * dr6i - lj.y);
+            float qq = qc * e.w;
+            if (dr2 > cut_coulsq) qq = 0.0f;
+            const float u = (r - tr0) * tinv;
+            const int kb = min(max((int)u, 0), tn - 1);
+            const float t = u - (float)kb;
+            const float4 cg = s_tab[kb];
+            const float force = qq * (cg.x + t * (cg.y + t * (cg.z + t * cg.w))) + flj * dr2i;
+            const float fs = force * PAIR_QUANT;
+            const float px = dx * fs, py = dy * fs, pz = dz * fs;
+            const float big = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
+            if (__builtin_expect(__any(big >= 2147483520.0f), 0)) {
+                fxi += (long long)(double)px;
+                fyi += (long long)(double)py;
+                fzi += (long long)(double)pz;
+            } else {
+                fxi += (long long)(int)px;
+                fyi += (long long)(int)py;
+                fzi += (long long)(int)pz;
+            }
+            if (ENERGY) {
+                const float4 ce = s_tab[tn + kb];
+                pot += (double)(dr6i * (dr6i * lj.z - lj.w));
+                ecl += (double)(qq * (ce.x + t * (ce.y + t * (ce.z + t * ce.w))));
+                vx += (double)(dx * dx * force);
+                vy += (double)(dy * dy * force);
+                vz += (double)(dz * dz * force);
+            }
+        }
+        head += cnt;
+    };
+
+    if (nn > 0) {
+        const int last = nn - 1;
+        int jc = lst[min(lane, last)];
+        float4 pn = pos4[jc];
+        float qn = q[jc];
+        int jnn = lst[min(64 + lane, last)];
+        for (int base = 0; base < nn; base += 64) {
+            const int k = base + lane;
+            const float4 pj = pn;
+            const float qj = qn;
+            pn = pos4[jnn];
+            qn = q[jnn];
+            jnn = lst[min(base + 128 + lane, last)];
+            float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
+            dx -= box * __builtin_rintf(dx * ibox);
+            dy -= box * __builtin_rintf(dy * ibox);
+            dz -= box * __builtin_rintf(dz * ibox);
+            const float dr2 = dx * dx + dy * dy + dz * dz;
+            const bool in = k < nn && dr2 <= cut2;
+            const uint64_t m = __ballot(in);
+            if (in) {
+                const int slot = (tail + __popcll(m & lt_mask)) & (QRING - 1);
+                ring[slot] = make_float4(dx, dy, dz, qj);
+                rtype[slot] = __float_as_int(pj.w);
+            }
+            tail += __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            if (tail - head >= 64) consume(64);
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (tail > head) consume(tail - head);
+    }
+
+    fxi = group_sum<64>(fxi);
+    fyi = group_sum<64>(fyi);
+    fzi = group_sum<64>(fzi);
+    const double qinv = 1.0 / (double)PAIR_QUANT;
+    const double Fx = (double)fxi * qinv, Fy = (double)fyi * qinv, Fz = (double)fzi * qinv;
+    const bool owner = lane == 0;
+    if (owner) {
+        fnb[i] = Fx;
+        fnb[(size_t)n + i] = Fy;
+        fnb[2 * (size_t)n + i] = Fz;
+        if (ff32) {
+            ff32[3 * (size_t)i + 0] = (float)Fx;
+            ff32[3 * (size_t)i + 1] = (float)Fy;
+            ff32[3 * (size_t)i + 2] = (float)Fz;
+        }
+    }
+    if (ENERGY) {
+        const int r = rep[i];
+        pot = group_sum<64>(pot);
+        ecl = group_sum<64>(ecl);
+        vx = group_sum<64>(vx);
+        vy = group_sum<64>(vy);
+        vz = group_sum<64>(vz);
+        wave_add(to_efix(0.5 * pot), owner, r, eacc, E_LJ);
+        wave_add(to_efix(0.5 * ecl), owner, r, eacc, E_COUL);
+        wave_add(to_efix(0.5 * vx), owner, r, eacc, V_X);
+        wave_add(to_efix(0.5 * vy), owner, r, eacc, V_Y);
+        wave_add(to_efix(0.5 * vz), owner, r, eacc, V_Z);
+    }
+}
+
+void launch_compact(sofia_ctx *c, bool energies, float *ff32) {
+    const int grid = (c->N + 3) / 4;
+    const int nt2 = c->dp.ntypes * c->dp.ntypes;
+    const size_t lds = sizeof(float4) * (nt2 + (energies ? 2 : 1) * c->dp.tab_n + 4 * QRING) +
+                       sizeof(int) * 4 * QRING;
+    if (energies)
+        hipLaunchKernelGGL(k_nonbond_compact<true>, dim3(grid), dim3(256), lds, c->stream, c->N,
+                           c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
+                           c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
+    else
+        hipLaunchKernelGGL(k_nonbond_compact<false>, dim3(grid), dim3(256), lds, c->stream, c->N,
+                           c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
+                           c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
+}
+
 template <int L, int MODE>
 void launch_t(sofia_ctx *c, bool energies, float *ff32) {
     const int block = c->block;
@@ -258,6 +434,10 @@ void launch_t(sofia_ctx *c, bool energies, float *ff32) {
 
 template <int MODE>
 void launch_mode(sofia_ctx *c, bool energies, float *ff32) {
+    if (MODE == 1 && c->lanes == 0) {
+        launch_compact(c, energies, ff32);
+        return;
+    }
     switch (c->lanes) {
         case 4: launch_t<4, MODE>(c, energies, ff32); break;
         case 8: launch_t<8, MODE>(c, energies, ff32); break;
@@ -290,7 +470,8 @@ __global__ void k_count_pairs(int32_t n, const float4 *__restrict__ pos4, const 
         const float4 xi = pos4[i];
         const RepInfo &R = reps[rep[i]];
         const int32_t *lst = nlist + (size_t)i * na;
-        for (int k = 0; k < numneigh[i]; ++k) {
+        const int nn = min(numneigh[i], na);
+        for (int k = 0; k < nn; ++k) {
             const float4 pj = pos4[lst[k]];
             float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
             dx -= R.box * __builtin_rintf(dx * R.ibox);

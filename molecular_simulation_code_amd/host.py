@@ -245,6 +245,10 @@ class SofiaGPU:
         """FP32 pair arithmetic with int64 fixed-point force sums."""
         return self._nonbond(lib().sofia_lj_cut_coul_dsf_mixed)
 
+    def set_tuning(self, lanes: int = 16, block: int = 256, list_waves: int = 4):
+        """Pair-kernel lanes per atom (0: the compacting mixed kernel), block, list waves."""
+        check(lib().sofia_set_tuning(self.ctx, lanes, block, list_waves))
+
     def lj_cut_coul_long_nonewton(self, step: int = 0):
         """lj_cut_coul_long_nonewton_NOMIC, Source/mod_force_nonbond.f90:300-400."""
         return self._nonbond(lib().sofia_lj_cut_coul_long, with64=False)
@@ -410,7 +414,7 @@ class Engine:
     def set_tuning(self, lanes: int, block: int, list_waves: int):
         check(lib().sofia_set_tuning(self.ctx, lanes, block, list_waves))
 
-    def autotune(self, steps: int = 80, skins=(1.0, 1.3, 1.6, 2.0, 2.5), lanes=(4, 8, 16, 32),
+    def autotune(self, steps: int = 80, skins=(1.0, 1.3, 1.6, 2.0, 2.5), lanes=(0, 8, 16, 32),
                  waves=(4, 8, 16)):
         """Pick the Verlet skin and the pair kernel's lanes per atom on the current state.
         For each skin the lanes are chosen by timing the pair kernel alone; skins are

@@ -135,6 +135,25 @@ def test_dsf_mixed(case, gpu_sys):
     assert g.e_coul == pytest.approx(ec, rel=1e-5)
 
 
+def test_dsf_mixed_kernels_agree(case, gpu_sys):
+    """Every lanes-per-atom layout of the mixed kernel, and the compacting kernel (0), give
+    the same per-atom sums: each pair is quantised to 2^-24 before an exact integer sum."""
+    name, s, d, ref = case
+    g, _ = gpu_sys
+    out = {}
+    for lanes in (16, 4, 64, 0):
+        say(f"[gpu {name}] dsf mixed lanes={lanes}")
+        g.set_tuning(lanes)
+        g.potential = g.e_coul = 0.0
+        g.lj_cut_coul_dsf_nonewton_mixed(0)
+        out[lanes] = (g.ff64.copy(), g.potential, g.e_coul)
+    g.set_tuning(16)
+    for lanes in (4, 64, 0):
+        assert np.array_equal(out[lanes][0], out[16][0]), lanes
+        assert out[lanes][1] == pytest.approx(out[16][1], rel=1e-9)
+        assert out[lanes][2] == pytest.approx(out[16][2], rel=1e-9)
+
+
 def test_coul_long(case, gpu_sys):
     name, s, d, ref = case
     g, _ = gpu_sys
