@@ -372,6 +372,8 @@ def main():
              for k, r in enumerate(reps)]
     eng = host.Engine(specs, precision=host.PREC_MIXED)
     tune_b = prepare_engine(eng, args.minimize, args.equil, not args.no_tune)
+    if tune_b is not None:
+        log(f"[batched] autotune log: {eng.tune_log}")
     log(f"[batched] tuned (skin, lanes, ms/step) = {tune_b}, "
         f"T = {eng.temperatures().mean():.1f} K (mean)")
     b0 = eng.rebuilds()

@@ -337,18 +337,23 @@ k_nonbond_compact(int32_t n, const float4 *__restrict__ pos4, const float *__res
     };
 
     if (nn > 0) {
+        // two-deep pipeline: chunk c uses positions gathered two chunks earlier; the index
+        // load is issued before the gathers so waiting for it leaves the gathers in flight
         const int last = nn - 1;
-        int jc = lst[min(lane, last)];
-        float4 pn = pos4[jc];
-        float qn = q[jc];
-        int jnn = lst[min(64 + lane, last)];
+        const int j0 = lst[min(lane, last)], j1 = lst[min(64 + lane, last)];
+        int j2 = lst[min(128 + lane, last)];
+        float4 p0 = pos4[j0], p1 = pos4[j1];
+        float q0 = q[j0], q1 = q[j1];
         for (int base = 0; base < nn; base += 64) {
             const int k = base + lane;
-            const float4 pj = pn;
-            const float qj = qn;
-            pn = pos4[jnn];
-            qn = q[jnn];
-            jnn = lst[min(base + 128 + lane, last)];
+            const float4 pj = p0;
+            const float qj = q0;
+            p0 = p1;
+            q0 = q1;
+            const int j3 = lst[min(base + 192 + lane, last)];
+            p1 = pos4[j2];
+            q1 = q[j2];
+            j2 = j3;
             float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
             dx -= box * __builtin_rintf(dx * ibox);
             dy -= box * __builtin_rintf(dy * ibox);

@@ -423,12 +423,14 @@ class Engine:
         (skin, lanes, ms/step, list waves)."""
         import time
         best = None
+        self.tune_log = {"pair_ms": {}, "step_ms": {}, "list_ms": {}}
         for sk in skins:
             self.set_skin(sk)
             t_l = {}
             for L in lanes:
                 self.set_tuning(L, 256, 4)
                 t_l[L] = self.profile_nonbond(8)[0]
+            self.tune_log["pair_ms"][sk] = t_l
             L = min(t_l, key=t_l.get)
             self.set_tuning(L, 256, 4)
             self.run(5)
@@ -437,6 +439,7 @@ class Engine:
             self.run(steps)
             self.synchronize()
             ms = 1e3 * (time.perf_counter() - t0) / steps
+            self.tune_log["step_ms"][sk] = ms
             if best is None or ms < best[2]:
                 best = (sk, L, ms)
         self.set_skin(best[0])
@@ -445,6 +448,7 @@ class Engine:
         for w in waves:
             self.set_tuning(best[1], 256, w)
             t_w[w] = self.profile_step(3, full_rebuild=True)["list build"]
+        self.tune_log["list_ms"] = t_w
         w = min(t_w, key=t_w.get)
         self.set_tuning(best[1], 256, w)
         return best[0], best[1], best[2], w
