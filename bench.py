@@ -423,13 +423,15 @@ def main():
         SV.relax(pool, args.minimize)
         reqs = SV.poisson_stream(pool, args.serve_rate, args.serve_seconds)
         nreq = len(reqs)
-        srv = SV.SegmentServer(segment=100)
+        tuning = None if tune_b is None else (tune_b[0], tune_b[1], tune_b[3])
+        srv = SV.SegmentServer(segment=100, tuning=tuning)
         res, wall, atom_steps = srv.serve(reqs)
         lat = np.array([r.latency_ms for r in res])
         p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
         served_rate = atom_steps / wall
         log(f"[serve] {nreq} requests, p50 {p50:.0f} ms, p99 {p99:.0f} ms, "
-            f"{served_rate / 1e6:.1f} Matom-steps/s, {srv.rebatches} re-batches")
+            f"{served_rate / 1e6:.1f} Matom-steps/s, {srv.cohorts_started} cohorts, "
+            f"{srv.rebatches} re-batches, {srv.segments_run} segments")
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None

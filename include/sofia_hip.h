@@ -191,6 +191,11 @@ int sofia_md_init(sofia_ctx *ctx);
  * nve_v, nve_x, pbc, check_neighbor, list rebuild when flagged, forces, Langevin,
  * SHAKE, nve_v.  use_graph != 0 captures one step in a hipGraph and replays it. */
 int sofia_md_run(sofia_ctx *ctx, int32_t nsteps, int32_t use_graph);
+/* sofia_md_run in two halves: md_launch enqueues the steps on the context's stream and
+ * returns at once; md_finish waits for them, reduces the energies and reports a list
+ * overflow or SHAKE failure.  Several contexts launched back to back run concurrently. */
+int sofia_md_launch(sofia_ctx *ctx, int32_t nsteps, int32_t use_graph);
+int sofia_md_finish(sofia_ctx *ctx);
 /* Continue from FP64 state uploaded with sofia_put_state_f64 (and the replica step
  * counters with sofia_set_replica_steps), e.g. a restart: rebuilds the lists and
  * re-evaluates the forces at the state, with the Langevin draw of the step that produced

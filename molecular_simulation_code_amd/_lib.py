@@ -87,6 +87,8 @@ SIGNATURES = {
     "sofia_set_thermostat": [_vp, _vp],
     "sofia_md_init": [_vp],
     "sofia_md_run": [_vp, _i32, _i32],
+    "sofia_md_launch": [_vp, _i32, _i32],
+    "sofia_md_finish": [_vp],
     "sofia_md_resume": [_vp],
     "sofia_get_forces_f64": [_vp, _vp],
     "sofia_md_resume_forces": [_vp, _vp, _vp],

@@ -187,12 +187,14 @@ __global__ void k_angle(int32_t items, const int32_t *__restrict__ al, const dou
 }
 
 // Torsion E = K (1 + cos(n phi - shift)) (the AMBER form of Source/mod_force_dihed_amber.f90)
-// with the Blondel-Karplus gradient of phi.
+// with the Blondel-Karplus gradient of phi.  For an integer multiplicity 1..6 the angle
+// functions come from cos(phi), sin(phi) (dot and triple products) by the multiple-angle
+// recurrence and a per-type (cos shift, sin shift): no FP64 atan2/cos/sin per torsion.
 __global__ void k_torsion(int32_t items, const int32_t *__restrict__ tl, const double *__restrict__ xd,
                           const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
                           const double *__restrict__ kk, const double *__restrict__ mult,
-                          const double *__restrict__ shift_deg, int32_t n,
-                          long long *__restrict__ fout, long long *__restrict__ eacc) {
+                          const double *__restrict__ shift_deg, const double *__restrict__ shift_cs,
+                          int32_t n, long long *__restrict__ fout, long long *__restrict__ eacc) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = m < items;
     double e = 0.0;
@@ -214,11 +216,30 @@ __global__ void k_torsion(int32_t items, const int32_t *__restrict__ tl, const d
         if (A2 > 0.0 && B2 > 0.0 && G2 > 0.0) {
             const double Gn = sqrt(G2);
             const double BxA[3] = {B[1] * A[2] - B[2] * A[1], B[2] * A[0] - B[0] * A[2], B[0] * A[1] - B[1] * A[0]};
-            const double phi = atan2((BxA[0] * G[0] + BxA[1] * G[1] + BxA[2] * G[2]) / Gn,
-                                     A[0] * B[0] + A[1] * B[1] + A[2] * B[2]);
-            const double nn = mult[t], dl = shift_deg[t] * (3.141592653589793 / 180.0);
-            e = kk[t] * (1.0 + cos(nn * phi - dl));
-            const double dEdphi = -kk[t] * nn * sin(nn * phi - dl);
+            const double y = (BxA[0] * G[0] + BxA[1] * G[1] + BxA[2] * G[2]) / Gn;
+            const double x = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
+            const double nn = mult[t];
+            const int ni = (int)nn;
+            double cnd, snd;                     // cos(n phi - shift), sin(n phi - shift)
+            if ((double)ni == nn && ni >= 1 && ni <= 6) {
+                const double inv = 1.0 / sqrt(A2 * B2);   // |A||B| = sqrt(x^2 + y^2)
+                const double c1 = x * inv, s1 = y * inv;
+                double cn = c1, sn = s1;
+                for (int k = 1; k < ni; ++k) {
+                    const double c2 = cn * c1 - sn * s1;
+                    sn = sn * c1 + cn * s1;
+                    cn = c2;
+                }
+                const double cd = shift_cs[2 * t], sd = shift_cs[2 * t + 1];
+                cnd = cn * cd + sn * sd;
+                snd = sn * cd - cn * sd;
+            } else {
+                const double ph = nn * atan2(y, x) - shift_deg[t] * (3.141592653589793 / 180.0);
+                cnd = cos(ph);
+                snd = sin(ph);
+            }
+            e = kk[t] * (1.0 + cnd);
+            const double dEdphi = -kk[t] * nn * snd;
             const double FG = F[0] * G[0] + F[1] * G[1] + F[2] * G[2];
             const double HG = H[0] * G[0] + H[1] * G[1] + H[2] * G[2];
 #pragma unroll
@@ -277,7 +298,7 @@ void launch_bonded_engine(sofia_ctx *c) {
     if (c->ntor && c->tors_k)
         hipLaunchKernelGGL(k_torsion, dim3(nblocks(c->ntor, 256)), dim3(256), 0, c->stream, c->ntor,
                            c->torsions, c->a.xd, c->rep, c->reps_d, c->tors_k, c->tors_n,
-                           c->tors_shift, n, c->fix, c->eacc);
+                           c->tors_shift, c->tors_cs, n, c->fix, c->eacc);
 }
 
 }  // namespace sofia

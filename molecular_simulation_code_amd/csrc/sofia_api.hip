@@ -509,7 +509,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms);
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
-    dfree(c->tors_shift); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
+    dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SOFIA_OK;
@@ -605,7 +605,7 @@ int sofia_set_bonded_coeffs(sofia_ctx *c, int32_t nbt, const double *bk, const d
     if (!c) return fail(SOFIA_E_ARG, "null context");
     TRY(unsplit_water(c));
     dfree(c->bond_k); dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0);
-    dfree(c->tors_k); dfree(c->tors_n); dfree(c->tors_shift); dfree(c->torsions);
+    dfree(c->tors_k); dfree(c->tors_n); dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->torsions);
     c->ntor = 0;
     if (nbt > 0) {
         TRY(dalloc(&c->bond_k, nbt)); TRY(dalloc(&c->bond_r0, nbt));
@@ -619,6 +619,14 @@ int sofia_set_bonded_coeffs(sofia_ctx *c, int32_t nbt, const double *bk, const d
         TRY(dalloc(&c->tors_k, ntt)); TRY(dalloc(&c->tors_n, ntt)); TRY(dalloc(&c->tors_shift, ntt));
         TRY(upload(c->tors_k, tk, sizeof(double) * ntt)); TRY(upload(c->tors_n, tn, sizeof(double) * ntt));
         TRY(upload(c->tors_shift, ts, sizeof(double) * ntt));
+        std::vector<double> cs((size_t)2 * ntt);
+        for (int t = 0; t < ntt; ++t) {
+            const double d = ts[t] * (3.141592653589793 / 180.0);
+            cs[2 * (size_t)t] = std::cos(d);
+            cs[2 * (size_t)t + 1] = std::sin(d);
+        }
+        TRY(dalloc(&c->tors_cs, cs.size()));
+        TRY(upload(c->tors_cs, cs.data(), sizeof(double) * cs.size()));
         std::vector<int32_t> g;
         TRY(to_global_list(c, tors, ntor, 5, 4, ntor_rep, g));
         TRY(dalloc(&c->torsions, g.size()));
@@ -1227,7 +1235,7 @@ int sofia_md_resume_forces(sofia_ctx *c, const double *f, const int32_t *have) {
     return put_forces(c, f, slot, have);
 }
 
-int sofia_md_run(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
+int sofia_md_launch(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
     if (nsteps <= 0) return SOFIA_OK;
@@ -1236,8 +1244,9 @@ int sofia_md_run(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
         SOFIA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         step_sequence(c, false);
         SOFIA_HIP(hipStreamEndCapture(c->stream, &g));
-        SOFIA_HIP(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+        if (c->graph_raw) (void)hipGraphDestroy(c->graph_raw);
         c->graph_raw = g;
+        SOFIA_HIP(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
     }
     for (int s = 0; s < nsteps - 1; ++s) {
         if (use_graph) {
@@ -1248,8 +1257,21 @@ int sofia_md_run(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
     }
     step_sequence(c, true);   // last step also reduces energies
     c->step_h += nsteps;
+    return SOFIA_OK;
+}
+
+int sofia_md_finish(sofia_ctx *c) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
     TRY(collect_energies(c));
     return check_engine_flags(c);
+}
+
+int sofia_md_run(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (nsteps <= 0) return c->engine_ready ? SOFIA_OK : fail(SOFIA_E_STATE, "call sofia_md_init first");
+    TRY(sofia_md_launch(c, nsteps, use_graph));
+    return sofia_md_finish(c);
 }
 
 int sofia_minimize(sofia_ctx *c, int32_t nsteps, double gamma, double max_disp) {

@@ -133,6 +133,7 @@ struct sofia_ctx {
     double *mass = nullptr;     // per type
     double *bond_k = nullptr, *bond_r0 = nullptr, *angle_k = nullptr, *angle_th0 = nullptr;
     double *tors_k = nullptr, *tors_n = nullptr, *tors_shift = nullptr;
+    double *tors_cs = nullptr;  // per torsion type: cos(shift), sin(shift)
     float4 *dsf_tab = nullptr;  // cubic pieces of the DSF Coulomb force / energy (mixed kernel)
     int32_t nbt = 0, nat = 0, ntt = 0;
     bool have_ff = false;

@@ -484,6 +484,14 @@ class Engine:
     def run(self, nsteps: int, use_graph: bool = True):
         check(lib().sofia_md_run(self.ctx, nsteps, 1 if use_graph else 0))
 
+    def launch(self, nsteps: int, use_graph: bool = True):
+        """Enqueue nsteps steps and return at once (finish() waits); engines launched back to
+        back run concurrently on their own streams."""
+        check(lib().sofia_md_launch(self.ctx, nsteps, 1 if use_graph else 0))
+
+    def finish(self):
+        check(lib().sofia_md_finish(self.ctx))
+
     def synchronize(self):
         check(lib().sofia_synchronize(self.ctx))
 

@@ -2,10 +2,20 @@
 
 Requests arrive over time, each carrying a system (coordinates and topology), a seed, a
 temperature and a step count.  The server keeps every admitted request's state (FP64
-positions and velocities, Langevin step counter) resident on the GPU and advances all of
-them together in one batched engine, ``segment`` steps at a time.  At each segment
-boundary finished requests leave and waiting ones are admitted while their HBM footprint
-fits the budget -- the batch composition changes as work arrives (continuous batching).
+positions and velocities, last force, Langevin step counter) resident on the GPU and
+advances them ``segment`` steps at a time.  At each segment boundary finished requests leave
+and waiting ones are admitted while their HBM footprint fits the budget -- the set of
+requests on the GPU changes as work arrives (continuous batching).
+
+Two policies:
+
+* ``cohort`` (default): the requests admitted at one segment boundary form a batched engine
+  of their own; every segment, all cohorts' steps are enqueued back to back on their own
+  streams and run concurrently, so admitting work never rebuilds the batch of the requests
+  already running.  A cohort whose requests partly finish is re-batched with the survivors;
+  more than ``max_cohorts`` cohorts are merged into one.
+* ``rebatch``: one engine holds every active request and is rebuilt whenever the set
+  changes.
 
 A replica that moves into a new batch is resumed with its exact FP64 state, last total
 force and step counter; with the mixed-precision engine (order-independent int64 force
@@ -69,7 +79,8 @@ class _Slot:
 class SegmentServer:
     def __init__(self, hbm_budget_gb: float = 64.0, segment: int = 100,
                  precision: int = PREC_MIXED, bytes_per_atom: int = 6000,
-                 rcontact: float = 3.5, keep_state: bool = False, tuning=None):
+                 rcontact: float = 3.5, keep_state: bool = False, tuning=None,
+                 policy: str = "cohort", max_cohorts: int = 12):
         self.budget = hbm_budget_gb * 1e9
         self.segment = segment
         self.precision = precision
@@ -79,6 +90,11 @@ class SegmentServer:
         self.tuning = tuning
         self.segments_run = 0
         self.rebatches = 0
+        if policy not in ("cohort", "rebatch"):
+            raise ValueError("policy must be 'cohort' or 'rebatch'")
+        self.policy = policy
+        self.max_cohorts = max_cohorts
+        self.cohorts_started = 0
 
     def footprint(self, req: Request) -> float:
         return req.system.np * self.bytes_per_atom
@@ -86,15 +102,99 @@ class SegmentServer:
     def _engine(self, slots):
         reps = [ReplicaSpec(s.req.system, temp=s.req.temp, gamma_ps=s.req.gamma_ps, seed=s.req.seed)
                 for s in slots]
-        eng = Engine(reps, precision=self.precision, tuning=self.tuning)
+        eng = Engine(reps, precision=self.precision)
+        if self.tuning:
+            skin, lanes, waves = self.tuning
+            eng.set_skin(skin)
+            eng.set_tuning(lanes, 256, waves)
         eng.put_state([(s.x, s.v) for s in slots])
         eng.set_steps([s.step for s in slots])
         eng.resume([s.f for s in slots])
-        self.rebatches += 1
         return eng
 
     def serve(self, requests, realtime: bool = True):
-        """Run the stream; returns (results, wall seconds, atom-steps)."""
+        """Run the stream; returns (results, wall seconds, atom-steps).  With realtime=False
+        the clock is virtual: one unit per segment."""
+        if self.policy == "rebatch":
+            return self._serve_rebatch(requests, realtime)
+        return self._serve_cohorts(requests, realtime)
+
+    def _result(self, s, now, energies, state):
+        return Result(s.req.rid, s.req.system.np, s.req.arrival, s.admitted, now, s.clusters,
+                      energies, state)
+
+    def _serve_cohorts(self, requests, realtime: bool):
+        t0 = time.perf_counter()
+        clock = (lambda: time.perf_counter() - t0) if realtime else None
+        virtual = 0.0
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        cohorts = []                     # [engine, slots]
+        results = []
+        used = 0.0
+        atom_steps = 0
+        while pending or cohorts:
+            now = clock() if realtime else virtual
+            new = []
+            while pending and pending[0].arrival <= now and used + self.footprint(pending[0]) <= self.budget:
+                r = pending.popleft()
+                new.append(_Slot(r, r.system.pos.astype(np.float64), r.system.v.astype(np.float64),
+                                 0, 0, now, []))
+                used += self.footprint(r)
+            if new:
+                cohorts.append([self._engine(new), new])
+                self.cohorts_started += 1
+            if len(cohorts) > self.max_cohorts:
+                slots = [s for _, sl in cohorts for s in sl]
+                for eng, sl in cohorts:
+                    self._harvest(eng, sl)
+                    eng.close()
+                cohorts = [[self._engine(slots), slots]]
+                self.rebatches += 1          # merged
+            if not cohorts:
+                if realtime:
+                    time.sleep(max(0.0, pending[0].arrival - clock()))
+                else:
+                    virtual = pending[0].arrival
+                continue
+            steps = [min(self.segment, min(s.req.nsteps - s.done for s in sl)) for _, sl in cohorts]
+            for (eng, _), n in zip(cohorts, steps):
+                eng.launch(n)
+            for eng, _ in cohorts:
+                eng.finish()
+            self.segments_run += 1
+            for (eng, sl), n in zip(cohorts, steps):
+                atom_steps += n * sum(s.req.system.np for s in sl)
+                clusters = eng.largest_cluster(self.rcontact)
+                for k, s in enumerate(sl):
+                    s.done += n
+                    s.clusters.append(int(clusters[k]))
+            if not realtime:
+                virtual += 1.0
+            now = clock() if realtime else virtual
+            keep = []
+            for eng, sl in cohorts:
+                fin = [k for k, s in enumerate(sl) if s.done >= s.req.nsteps]
+                if not fin:
+                    keep.append([eng, sl])
+                    continue
+                energies = eng.energies()
+                states = eng.state() if self.keep_state else None
+                for k in fin:
+                    results.append(self._result(sl[k], now, energies[k], states[k] if states else None))
+                    used -= self.footprint(sl[k].req)
+                rest = [s for k, s in enumerate(sl) if k not in set(fin)]
+                if rest:
+                    self._harvest(eng, sl)
+                    eng.close()
+                    keep.append([self._engine(rest), rest])
+                    self.rebatches += 1
+                else:
+                    eng.close()
+            cohorts = keep
+        wall = (clock() if realtime else virtual)
+        return sorted(results, key=lambda r: r.rid), wall, atom_steps
+
+    def _serve_rebatch(self, requests, realtime: bool):
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else None
         virtual = 0.0
@@ -125,6 +225,7 @@ class SegmentServer:
                     self._harvest(eng, in_engine)
                     eng.close()
                 eng = self._engine(active)
+                self.rebatches += 1
                 in_engine = list(active)
                 dirty = False
             nsteps = min(self.segment, min(s.req.nsteps - s.done for s in active))

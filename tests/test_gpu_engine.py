@@ -101,9 +101,9 @@ def test_engine_list_holds_every_pair_inside_the_cutoff():
 
 
 def test_engine_bonded_energies_match_oracle():
-    """Bond and angle energies of the engine (rigid waters evaluated inside the SHAKE kernel,
-    the rest in the bond/angle kernels) against the oracle on the same positions, also in a
-    batch where the replicas' lists are concatenated."""
+    """Bond, angle and torsion energies of the engine (rigid waters evaluated inside the SHAKE
+    kernel, the rest in the bonded kernels; torsions by the multiple-angle recurrence) against
+    the oracle on the same positions, in a batch where the replicas' lists are concatenated."""
     host = _engine()
     from oracle import oracle as O
     systems = [S.make_system(5, 1000, seed=41), S.make_system(3, 1100, seed=42)]
@@ -117,7 +117,12 @@ def test_engine_bonded_energies_match_oracle():
             ka = np.array([c[0] for c in S.ANGLE_COEFF]); th = np.radians([c[1] for c in S.ANGLE_COEFF])
             _, eb = O.bond_harmonic(s.bondlist, s.pos, s.types, kb, r0, d["box"], d["ibox"])
             _, ea = O.angle_harmonic(s.anglelist, s.pos, s.types, ka, th, d["box"], d["ibox"])
-            say(f"[engine bonded] replica {k}: bond {E[k][4]:.6f} vs {eb:.6f}, angle {E[k][5]:.6f} vs {ea:.6f}")
+            tk = np.array([c[0] for c in S.DIHED_COEFF]); tm = np.array([c[1] for c in S.DIHED_COEFF], float)
+            ts = np.array([c[2] for c in S.DIHED_COEFF], float)
+            _, et = O.torsion_cos(host.torsion_list(s), s.pos, s.types, tk, tm, ts, d["box"], d["ibox"])
+            say(f"[engine bonded] replica {k}: bond {E[k][4]:.6f} vs {eb:.6f}, angle {E[k][5]:.6f} vs {ea:.6f}, "
+                f"torsion {E[k][6]:.6f} vs {et:.6f}")
+            np.testing.assert_allclose(E[k][6], et, rtol=2e-5, atol=1e-4)
             # the oracle follows the reference's real*4 coordinate differences; the engine
             # differences FP64 positions, so the sums agree to float32 rounding
             np.testing.assert_allclose(E[k][4], eb, rtol=2e-5, atol=1e-4)
@@ -299,13 +304,19 @@ def test_served_request_same_alone_and_in_mix():
 
     srv = SV.SegmentServer(segment=100, keep_state=True)
     alone, _, _ = srv.serve([req(0, 0, 0.0, 101, 300.0)], realtime=False)
+    a = alone[0]
     mix_reqs = [req(0, 0, 0.0, 101, 300.0), req(1, 1, 0.0, 202, 285.0), req(2, 2, 1.5, 303, 315.0),
                 req(3, 1, 2.5, 404, 295.0)]
-    srv2 = SV.SegmentServer(segment=100, keep_state=True)
-    mix, _, _ = srv2.serve(mix_reqs, realtime=False)
-    assert srv2.rebatches >= 3                       # the batch really changed under request 0
-    a, m = alone[0], mix[0]
-    assert np.array_equal(a.state[0], m.state[0]) and np.array_equal(a.state[1], m.state[1])
-    assert a.clusters == m.clusters
-    np.testing.assert_allclose(a.energies, m.energies, rtol=1e-12, atol=1e-9)
+    for policy in ("rebatch", "cohort"):
+        say(f"[serve] policy {policy}")
+        srv2 = SV.SegmentServer(segment=100, keep_state=True, policy=policy)
+        mix, _, _ = srv2.serve(mix_reqs, realtime=False)
+        if policy == "rebatch":
+            assert srv2.rebatches >= 3                   # the batch really changed under request 0
+        else:
+            assert srv2.cohorts_started >= 3             # requests 2 and 3 started their own cohorts
+        m = mix[0]
+        assert np.array_equal(a.state[0], m.state[0]) and np.array_equal(a.state[1], m.state[1])
+        assert a.clusters == m.clusters
+        np.testing.assert_allclose(a.energies, m.energies, rtol=1e-12, atol=1e-9)
     assert all(r.finished >= r.arrival for r in mix)
