@@ -21,9 +21,10 @@
 // cell-sorted float4 (x, y, z, type) array, so the j gathers of neighbouring atoms hit
 // the same cache lines.  No atomics on forces: every atom's sum is owned by one group.
 //
-// Mode 1 turns each pair force into an integer number of 2^-24 kcal/mol/A (truncation
-// of the exact float product; a 64-bit conversion when |f| >= 128), so per-atom sums are
-// exact and independent of L, of list order and of how replicas are batched.
+// Mode 1 turns each pair force into an integer number of 2^-24 kcal/mol/A (truncation of
+// the float product) and sums those integers in FP64, exactly while a sum stays below 2^53
+// quanta (5e8 kcal/mol/A), so per-atom sums are independent of L, of list order and of how
+// replicas are batched.
 #include "sofia_device.h"
 
 namespace sofia {
@@ -64,7 +65,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
     const float4 *ljrow = s_lj + ti * P.ntypes - 1;   // index by jtype (1-based)
 
     double fxd = 0.0, fyd = 0.0, fzd = 0.0;           // modes 0, 2
-    long long fxi = 0, fyi = 0, fzi = 0;              // mode 1
+    double fxq = 0.0, fyq = 0.0, fzq = 0.0;           // mode 1: sums of integer quanta
     double pot = 0.0, ecl = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
 
     if (MODE == 0) {
@@ -73,16 +74,21 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
         const double qc = (double)P.coulpre * (double)q[ii];
         const double two_a_pis = (2.0 * P.alpha) * P.my_pis_inv;
         const double a2 = P.alpha * P.alpha, pa = P.ewald_p * P.alpha;
-        int jnn = (sub + L < nn) ? lst[sub + L] : ii;
-        float4 pn = pos4[(sub < nn) ? lst[sub] : ii];
-        float qn = q[(sub < nn) ? lst[sub] : ii];
+        // software pipeline: an entry's data is loaded one iteration ahead; loads past the
+        // end of the row re-read its last entry (clamped index), so no branch is needed
+        const int last = max(nn - 1, 0);
+        const uint32_t *ulst = (const uint32_t *)lst;
+        const uint32_t j0 = (nn > 0) ? ulst[min(sub, last)] : (uint32_t)ii;
+        uint32_t jnn = (nn > 0) ? ulst[min(sub + L, last)] : (uint32_t)ii;
+        float4 pn = pos4[j0];
+        float qn = q[j0];
+#pragma unroll 2
         for (int k = sub; k < nn; k += L) {
-            // software pipeline: this entry's data arrived during the previous iteration
             const float4 pj = pn;
             const float qj = qn;
             pn = pos4[jnn];
             qn = q[jnn];
-            jnn = (k + 2 * L < nn) ? lst[k + 2 * L] : ii;
+            jnn = ulst[min(k + 2 * L, last)];
             double dx = x1 - (double)pj.x, dy = y1 - (double)pj.y, dz = z1 - (double)pj.z;
             dx -= bx * __builtin_rint(dx * ib);
             dy -= bx * __builtin_rint(dy * ib);
@@ -125,16 +131,21 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
                     A5 = (float)P.a5;
         const float tr0 = P.tab_r0, tinv = P.tab_inv_h;
         const int tn = P.tab_n;
-        int jnn = (sub + L < nn) ? lst[sub + L] : ii;
-        float4 pn = pos4[(sub < nn) ? lst[sub] : ii];
-        float qn = q[(sub < nn) ? lst[sub] : ii];
+        // software pipeline: an entry's data is loaded one iteration ahead; loads past the
+        // end of the row re-read its last entry (clamped index), so no branch is needed
+        const int last = max(nn - 1, 0);
+        const uint32_t *ulst = (const uint32_t *)lst;
+        const uint32_t j0 = (nn > 0) ? ulst[min(sub, last)] : (uint32_t)ii;
+        uint32_t jnn = (nn > 0) ? ulst[min(sub + L, last)] : (uint32_t)ii;
+        float4 pn = pos4[j0];
+        float qn = q[j0];
+#pragma unroll 2
         for (int k = sub; k < nn; k += L) {
-            // software pipeline: this entry's data arrived during the previous iteration
             const float4 pj = pn;
             const float qj = qn;
             pn = pos4[jnn];
             qn = q[jnn];
-            jnn = (k + 2 * L < nn) ? lst[k + 2 * L] : ii;
+            jnn = ulst[min(k + 2 * L, last)];
             float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
             dx -= box * __builtin_rintf(dx * ibox);
             dy -= box * __builtin_rintf(dy * ibox);
@@ -173,18 +184,11 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
             // mode 1: fc is already force/r; mode 2: fc is force*r
             const float force = (MODE == 1) ? fc + flj * dr2i : (fc + flj) * dr2i;
             if (MODE == 1) {
+                // truncated quanta are integers; FP64 sums of integers below 2^53 are exact
                 const float fs = force * PAIR_QUANT;
-                const float px = dx * fs, py = dy * fs, pz = dz * fs;
-                const float big = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
-                if (__builtin_expect(__any(big >= 2147483520.0f), 0)) {
-                    fxi += (long long)(double)px;
-                    fyi += (long long)(double)py;
-                    fzi += (long long)(double)pz;
-                } else {
-                    fxi += (long long)(int)px;
-                    fyi += (long long)(int)py;
-                    fzi += (long long)(int)pz;
-                }
+                fxq += (double)__builtin_truncf(dx * fs);
+                fyq += (double)__builtin_truncf(dy * fs);
+                fzq += (double)__builtin_truncf(dz * fs);
             } else {
                 fxd += (double)(dx * force);
                 fyd += (double)(dy * force);
@@ -202,13 +206,10 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
 
     double Fx, Fy, Fz;
     if (MODE == 1) {
-        fxi = group_sum<L>(fxi);
-        fyi = group_sum<L>(fyi);
-        fzi = group_sum<L>(fzi);
         const double qinv = 1.0 / (double)PAIR_QUANT;
-        Fx = (double)fxi * qinv;
-        Fy = (double)fyi * qinv;
-        Fz = (double)fzi * qinv;
+        Fx = group_sum<L>(fxq) * qinv;
+        Fy = group_sum<L>(fyq) * qinv;
+        Fz = group_sum<L>(fzq) * qinv;
     } else {
         Fx = group_sum<L>(fxd);
         Fy = group_sum<L>(fyd);
@@ -287,7 +288,7 @@ k_nonbond_compact(int32_t n, const float4 *__restrict__ pos4, const float *__res
     const float qc = P.coulpre * q[i];
     const float tr0 = P.tab_r0, tinv = P.tab_inv_h;
 
-    long long fxi = 0, fyi = 0, fzi = 0;
+    double fxq = 0.0, fyq = 0.0, fzq = 0.0;   // exact sums of integer quanta
     double pot = 0.0, ecl = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
     int head = 0, tail = 0;
 
@@ -313,17 +314,9 @@ k_nonbond_compact(int32_t n, const float4 *__restrict__ pos4, const float *__res
             const float4 cg = s_tab[kb];
             const float force = qq * (cg.x + t * (cg.y + t * (cg.z + t * cg.w))) + flj * dr2i;
             const float fs = force * PAIR_QUANT;
-            const float px = dx * fs, py = dy * fs, pz = dz * fs;
-            const float big = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
-            if (__builtin_expect(__any(big >= 2147483520.0f), 0)) {
-                fxi += (long long)(double)px;
-                fyi += (long long)(double)py;
-                fzi += (long long)(double)pz;
-            } else {
-                fxi += (long long)(int)px;
-                fyi += (long long)(int)py;
-                fzi += (long long)(int)pz;
-            }
+            fxq += (double)__builtin_truncf(dx * fs);
+            fyq += (double)__builtin_truncf(dy * fs);
+            fzq += (double)__builtin_truncf(dz * fs);
             if (ENERGY) {
                 const float4 ce = s_tab[tn + kb];
                 pot += (double)(dr6i * (dr6i * lj.z - lj.w));
@@ -374,11 +367,9 @@ k_nonbond_compact(int32_t n, const float4 *__restrict__ pos4, const float *__res
         if (tail > head) consume(tail - head);
     }
 
-    fxi = group_sum<64>(fxi);
-    fyi = group_sum<64>(fyi);
-    fzi = group_sum<64>(fzi);
     const double qinv = 1.0 / (double)PAIR_QUANT;
-    const double Fx = (double)fxi * qinv, Fy = (double)fyi * qinv, Fz = (double)fzi * qinv;
+    const double Fx = group_sum<64>(fxq) * qinv, Fy = group_sum<64>(fyq) * qinv,
+                 Fz = group_sum<64>(fzq) * qinv;
     const bool owner = lane == 0;
     if (owner) {
         fnb[i] = Fx;
