@@ -148,6 +148,16 @@ int sofia_lj_cut_coul_dsf(sofia_ctx *ctx, sofia_coords *ff, double *ff64, double
 /* Mixed-precision variant: FP32 pair math, int64 fixed-point force sums. */
 int sofia_lj_cut_coul_dsf_mixed(sofia_ctx *ctx, sofia_coords *ff, double *ff64,
                                 double *potential, double *e_coul, double *virial);
+/* Asynchronous form of the pair-kernel drop-in, for hosts that overlap the force call with
+ * their own work as the reference's MIC offload does (`offload ... signal(sig)` /
+ * `offload_wait`, Source/mod_force.f90:66-156): sofia_lj_cut_coul_dsf_async enqueues the DSF
+ * pair kernel (mixed != 0: the mixed-precision kernel) and the copy of its results into
+ * pinned host memory and returns; sofia_nonbond_wait waits and then delivers ff (np coords,
+ * overwritten), adds to *potential / *e_coul and sets virial[3], like sofia_lj_cut_coul_dsf.
+ * One call may be in flight per context. */
+int sofia_lj_cut_coul_dsf_async(sofia_ctx *ctx, int32_t mixed);
+int sofia_nonbond_wait(sofia_ctx *ctx, sofia_coords *ff, double *potential, double *e_coul,
+                       double *virial);
 /* lj_cut_coul_long_nonewton_NOMIC, Source/mod_force_nonbond.f90:300-400 (PME real space) */
 int sofia_lj_cut_coul_long(sofia_ctx *ctx, sofia_coords *ff, double *potential,
                            double *e_coul, double *virial);

@@ -77,6 +77,8 @@ SIGNATURES = {
     "sofia_lj_cut_coul_dsf": [_vp] * 6,
     "sofia_lj_cut_coul_dsf_mixed": [_vp] * 6,
     "sofia_lj_cut_coul_long": [_vp] * 5,
+    "sofia_lj_cut_coul_dsf_async": [_vp, _i32],
+    "sofia_nonbond_wait": [_vp] * 5,
     "sofia_correct_bonds": [_vp, _i32, _vp, _vp, _vp],
     "sofia_pair14_dsf": [_vp] * 4,
     "sofia_nve_v": [_vp, _f, _vp, _vp],

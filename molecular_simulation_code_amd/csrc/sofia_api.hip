@@ -510,6 +510,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
     dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SOFIA_OK;
@@ -841,6 +842,48 @@ static int nonbond_dropin(sofia_ctx *c, int mode, sofia_coords *ff, double *ff64
     }
     if (pot) *pot += sp;
     if (ecoul) *ecoul += se;
+    if (vir) { vir[0] = vx; vir[1] = vy; vir[2] = vz; }
+    return SOFIA_OK;
+}
+
+int sofia_lj_cut_coul_dsf_async(sofia_ctx *c, int32_t mixed) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no Verlet list: call sofia_neighbor_wrapper first");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    if (c->async_mode >= 0) return fail(SOFIA_E_STATE, "a pair-kernel call is already in flight");
+    const size_t fbytes = sizeof(float) * 3 * (size_t)c->N, ebytes = sizeof(long long) * c->nrep * NE;
+    if (c->pinned_bytes < fbytes + ebytes) {
+        if (c->pinned) (void)hipHostFree(c->pinned);
+        c->pinned = nullptr;
+        c->pinned_bytes = 0;
+        SOFIA_HIP(hipHostMalloc(&c->pinned, fbytes + ebytes, hipHostMallocDefault));
+        c->pinned_bytes = fbytes + ebytes;
+    }
+    const int mode = mixed ? 1 : 0;
+    launch_zero_forces(c, c->fscratch, true);
+    launch_nonbond_out(c, mode, true, staging_f32(c));
+    SOFIA_HIP(hipMemcpyAsync(c->pinned, staging_f32(c), fbytes, hipMemcpyDeviceToHost, c->stream));
+    SOFIA_HIP(hipMemcpyAsync((char *)c->pinned + fbytes, c->eacc, ebytes, hipMemcpyDeviceToHost, c->stream));
+    c->async_mode = mode;
+    return SOFIA_OK;
+}
+
+int sofia_nonbond_wait(sofia_ctx *c, sofia_coords *ff, double *pot, double *ec, double *vir) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (c->async_mode < 0) return fail(SOFIA_E_STATE, "no pair-kernel call in flight");
+    c->async_mode = -1;
+    TRY(sync_check(c));
+    const size_t fbytes = sizeof(float) * 3 * (size_t)c->N;
+    if (ff) std::memcpy(ff, c->pinned, fbytes);
+    const long long *e = (const long long *)((const char *)c->pinned + fbytes);
+    double sp = 0, se = 0, vx = 0, vy = 0, vz = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        const long long *o = e + (size_t)r * NE;
+        sp += (double)o[E_LJ] * EFIX_INV; se += (double)o[E_COUL] * EFIX_INV;
+        vx += (double)o[V_X] * EFIX_INV; vy += (double)o[V_Y] * EFIX_INV; vz += (double)o[V_Z] * EFIX_INV;
+    }
+    if (pot) *pot += sp;
+    if (ec) *ec += se;
     if (vir) { vir[0] = vx; vir[1] = vy; vir[2] = vz; }
     return SOFIA_OK;
 }

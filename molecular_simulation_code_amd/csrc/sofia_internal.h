@@ -154,9 +154,10 @@ struct sofia_ctx {
     hipGraph_t graph_raw = nullptr;
     bool has_thermostat = false;
 
-    // host staging
+    // host staging (pinned) for the asynchronous pair-kernel drop-in
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    int async_mode = -1;         // pair-kernel mode in flight, -1 = none
 };
 
 namespace sofia {

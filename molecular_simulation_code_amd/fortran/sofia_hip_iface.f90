@@ -115,6 +115,19 @@ module sofia_hip_iface
        type(*), intent(inout) :: ff(*)
        real(c_double), intent(inout) :: pot, ecoul, vir(3)
      end function
+     integer(c_int) function sofia_lj_cut_coul_dsf_async(ctx, mixed) &
+          bind(c, name="sofia_lj_cut_coul_dsf_async")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: mixed
+     end function
+     integer(c_int) function sofia_nonbond_wait(ctx, ff, pot, ecoul, vir) &
+          bind(c, name="sofia_nonbond_wait")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       type(*), intent(inout) :: ff(*)
+       real(c_double), intent(inout) :: pot, ecoul, vir(3)
+     end function
      integer(c_int) function sofia_lj_cut_coul_long(ctx, ff, pot, ecoul, vir) &
           bind(c, name="sofia_lj_cut_coul_long")
        import :: c_int, c_ptr, c_double
@@ -261,6 +274,19 @@ contains
     call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
     call sofia_check(sofia_lj_cut_coul_dsf(ctx, ff, c_null_ptr, potential, e_coul, vir), 'dsf')
   end subroutine lj_cut_coul_dsf_nonewton_NOMIC_hip
+
+  ! The offload pattern of Source/mod_force.f90:66-156: start the pair kernel, do host work,
+  ! then wait (offload ... signal(sig) / offload_wait).
+  subroutine lj_cut_coul_dsf_nonewton_MIC_signal_hip(step)
+    integer :: step
+    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_lj_cut_coul_dsf_async(ctx, 0_c_int32_t), 'dsf_async')
+  end subroutine lj_cut_coul_dsf_nonewton_MIC_signal_hip
+
+  subroutine lj_cut_coul_dsf_nonewton_MIC_wait_hip
+    real(c_double) :: vir(3)
+    call sofia_check(sofia_nonbond_wait(ctx, ff, potential, e_coul, vir), 'nonbond_wait')
+  end subroutine lj_cut_coul_dsf_nonewton_MIC_wait_hip
 
   subroutine lj_cut_coul_long_nonewton_NOMIC_hip
     real(c_double) :: vir(3)

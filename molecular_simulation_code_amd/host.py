@@ -245,6 +245,24 @@ class SofiaGPU:
         """FP32 pair arithmetic with int64 fixed-point force sums."""
         return self._nonbond(lib().sofia_lj_cut_coul_dsf_mixed)
 
+    def lj_cut_coul_dsf_nonewton_async(self, mixed: bool = False):
+        """Start the DSF pair kernel and return at once (the reference's offload ... signal);
+        nonbond_wait() delivers ff, potential, e_coul and the virial."""
+        self._put_pos()
+        check(lib().sofia_lj_cut_coul_dsf_async(self.ctx, 1 if mixed else 0))
+
+    def nonbond_wait(self):
+        """Wait for the pair kernel started by lj_cut_coul_dsf_nonewton_async (offload_wait)."""
+        ff = np.empty((self.np, 3), f32)
+        pe, ec = C.c_double(0.0), C.c_double(0.0)
+        vir = np.zeros(3, f64)
+        check(lib().sofia_nonbond_wait(self.ctx, ptr(ff), C.byref(pe), C.byref(ec), ptr(vir)))
+        self.ff = ff
+        self.potential += pe.value
+        self.e_coul += ec.value
+        self.virial = vir
+        return ff
+
     def set_tuning(self, lanes: int = 16, block: int = 256, list_waves: int = 4):
         """Pair-kernel lanes per atom (0: the compacting mixed kernel), block, list waves."""
         check(lib().sofia_set_tuning(self.ctx, lanes, block, list_waves))

@@ -154,6 +154,23 @@ def test_dsf_mixed_kernels_agree(case, gpu_sys):
         assert out[lanes][2] == pytest.approx(out[16][2], rel=1e-9)
 
 
+def test_dsf_async_equals_sync(case, gpu_sys):
+    """The signal/wait form of the pair-kernel drop-in delivers what the blocking call does."""
+    name, s, d, ref = case
+    g, _ = gpu_sys
+    for mixed in (False, True):
+        say(f"[gpu {name}] dsf async mixed={mixed}")
+        g.potential = g.e_coul = 0.0
+        (g.lj_cut_coul_dsf_nonewton_mixed if mixed else g.lj_cut_coul_dsf_nonewton)(0)
+        ff, pot, ec, vir = g.ff.copy(), g.potential, g.e_coul, g.virial.copy()
+        g.potential = g.e_coul = 0.0
+        g.lj_cut_coul_dsf_nonewton_async(mixed)
+        g.nonbond_wait()
+        assert np.array_equal(g.ff, ff)
+        assert g.potential == pot and g.e_coul == ec
+        np.testing.assert_array_equal(g.virial, vir)
+
+
 def test_coul_long(case, gpu_sys):
     name, s, d, ref = case
     g, _ = gpu_sys
