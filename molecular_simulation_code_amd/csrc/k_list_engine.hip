@@ -140,32 +140,37 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
             uint64_t rows = __ballot(rhi > rlo);
             int cnt = (g0 == 0) ? 0 : numneigh[i];
             int32_t *out = nlist + (size_t)i * na;
+            // test one staged candidate per lane; hits appended in staged order
+            auto sweep = [&](const float4 pj, const bool valid) {
+                const int z = __float_as_int(pj.w);
+                const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
+                bool pred = valid && (dx * dx + dy * dy + dz * dz < rv2) && (z != i);
+                const bool maybe = pred && z >= exmin && z <= exmax;
+                if (__any(maybe)) {
+                    if (maybe) {
+#pragma unroll
+                        for (int l = 0; l < SPEC; ++l)
+                            if (ex[l] == z) pred = false;
+                    }
+                }
+                const uint64_t mask = __ballot(pred);
+                const int pos = cnt + __popcll(mask & lt_mask);
+                if (pred && pos < na) out[pos] = z;
+                cnt += __popcll(mask);
+            };
             while (rows) {
                 const int k = __builtin_ctzll(rows);
                 rows &= rows - 1;
-                const int lo = __shfl(rlo, k, 64) - g0, hi = __shfl(rhi, k, 64) - g0;
-                for (int base = lo; base < hi; base += 64) {
-                    const int t = base + lane;
-                    bool pred = false;
-                    int z = -1;
-                    if (t < hi) {
-                        const float4 pj = s_j[t];
-                        z = __float_as_int(pj.w);
-                        const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
-                        pred = (dx * dx + dy * dy + dz * dz < rv2) && (z != i);
-                    }
-                    const bool maybe = pred && z >= exmin && z <= exmax;
-                    if (__any(maybe)) {
-                        if (maybe) {
-#pragma unroll
-                            for (int l = 0; l < SPEC; ++l)
-                                if (ex[l] == z) pred = false;
-                        }
-                    }
-                    const uint64_t mask = __ballot(pred);
-                    const int pos = cnt + __popcll(mask & lt_mask);
-                    if (pred && pos < na) out[pos] = z;
-                    cnt += __popcll(mask);
+                const int lo = __builtin_amdgcn_readlane(rlo, k) - g0;
+                const int hi = __builtin_amdgcn_readlane(rhi, k) - g0;
+                // two chunks per iteration, both LDS reads issued before either is used;
+                // indices past the row re-read its last slot and are masked off
+                for (int base = lo; base < hi; base += 128) {
+                    const int t0 = base + lane, t1 = base + 64 + lane;
+                    const float4 p0 = s_j[min(t0, hi - 1)];
+                    const float4 p1 = s_j[min(t1, hi - 1)];
+                    sweep(p0, t0 < hi);
+                    if (base + 64 < hi) sweep(p1, t1 < hi);
                 }
             }
             if (lane == 0) {
