@@ -175,6 +175,13 @@ int sofia_pair14_dsf(sofia_ctx *ctx, sofia_coords *ffh, double *e_coul_14, doubl
 int sofia_nve_v(sofia_ctx *ctx, float dt_partial, const sofia_coords *ff, double *ke);
 int sofia_nve_x(sofia_ctx *ctx);
 int sofia_pbc(sofia_ctx *ctx);
+/* berendsen, Source/mod_ensemble.f90:167-205, given the scale factor the host computed
+ * (mu = (1 - (dt/tau)(ptarget - pressure))^(1/3), one per replica): positions scaled by mu,
+ * drx/dry/drz accumulate the displacement, box = box*mu and ibox = 1.0/box in real*4
+ * (box_out, optional, receives the new boxes).  The cell grid is re-derived from the new
+ * box, as check_allocation does, so the next call is sofia_neighbor_wrapper.  Drop-in only
+ * (the engine runs at fixed volume). */
+int sofia_berendsen(sofia_ctx *ctx, const double *mu, float *box_out);
 /* shake(iflag), Source/mod_shake.f90:8-85: adds constraint forces to ff (np, in/out) and
  * to virial (3).  max_iter (optional) receives the largest iteration count. */
 int sofia_shake(sofia_ctx *ctx, int32_t iflag, sofia_coords *ff, double *virial,

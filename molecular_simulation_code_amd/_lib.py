@@ -84,6 +84,7 @@ SIGNATURES = {
     "sofia_nve_v": [_vp, _f, _vp, _vp],
     "sofia_nve_x": [_vp],
     "sofia_pbc": [_vp],
+    "sofia_berendsen": [_vp, _vp, _vp],
     "sofia_shake": [_vp, _i32, _vp, _vp, _vp],
     "sofia_langevin": [_vp, _f, _f, _u64, _u64, _vp],
     "sofia_set_thermostat": [_vp, _vp],

@@ -90,6 +90,26 @@ __global__ void k_pbc32(int32_t n, float4 *__restrict__ pos4, const int32_t *__r
     pos4[i] = p;
 }
 
+// berendsen, Source/mod_ensemble.f90:167-205: position = position*mu (real*4 * real*8, stored
+// real*4); dx = new - dble(old); drx = drx + dx (real*8 sum stored real*4).
+__global__ void k_berendsen32(int32_t n, float4 *__restrict__ pos4, float *__restrict__ dr,
+                              const int32_t *__restrict__ rep, const double *__restrict__ mu) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = mu[rep[i]];
+    float4 p = pos4[i];
+    float *c[3] = {&p.x, &p.y, &p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double old = (double)*c[d];
+        const float nw = (float)(old * m);
+        *c[d] = nw;
+        float *r = dr + (size_t)d * n + i;
+        *r = (float)((double)*r + ((double)nw - old));
+    }
+    pos4[i] = p;
+}
+
 __global__ void k_langevin32(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ v,
                              const int32_t *__restrict__ gid, const double *__restrict__ mass,
                              float fric, float gconst, uint64_t seed, uint64_t step,
@@ -714,6 +734,11 @@ void launch_nve_v32(sofia_ctx *c, float dt_partial, const float *ff) {
 void launch_nve_x32(sofia_ctx *c) {
     hipLaunchKernelGGL(k_nve_x32, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, c->dp.dt,
                        c->a.v32, c->a.pos4, c->dr32);
+}
+
+void launch_berendsen32(sofia_ctx *c, const double *mu_dev) {
+    hipLaunchKernelGGL(k_berendsen32, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, c->a.pos4,
+                       c->dr32, c->rep, mu_dev);
 }
 
 void launch_pbc32(sofia_ctx *c) {

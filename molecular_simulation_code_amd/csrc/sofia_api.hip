@@ -961,6 +961,36 @@ int sofia_nve_x(sofia_ctx *c) {
     return sync_check(c);
 }
 
+namespace {
+int reference_regrid(sofia_ctx *c);
+}
+
+int sofia_berendsen(sofia_ctx *c, const double *mu, float *box_out) {
+    if (!c || !mu) return fail(SOFIA_E_ARG, "null argument");
+    if (c->engine_ready) return fail(SOFIA_E_STATE, "berendsen is a drop-in; the engine runs at fixed volume");
+    for (int r = 0; r < c->nrep; ++r)
+        if (!(mu[r] > 0.5 && mu[r] < 2.0)) return fail(SOFIA_E_ARG, "box scale factor mu out of (0.5, 2)");
+    double *mu_d = nullptr;
+    TRY(dalloc(&mu_d, c->nrep));
+    int rc = upload(mu_d, mu, sizeof(double) * c->nrep);
+    if (rc == SOFIA_OK) {
+        launch_berendsen32(c, mu_d);
+        rc = sync_check(c);
+    }
+    dfree(mu_d);
+    TRY(rc);
+    // box = box*mu (real*4); ibox = 1.0/box (real*4 division), Source/mod_ensemble.f90:175-178;
+    // the next neighbor_wrapper re-derives the grid from the new box (check_allocation)
+    for (int r = 0; r < c->nrep; ++r) {
+        auto &R = c->reps_h[r];
+        R.box = (float)((double)R.box * mu[r]);
+        R.ibox = 1.0f / R.box;
+        if (box_out) box_out[r] = R.box;
+    }
+    TRY(reference_regrid(c));
+    return SOFIA_OK;
+}
+
 int sofia_pbc(sofia_ctx *c) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     launch_pbc32(c);
@@ -1081,6 +1111,44 @@ int check_engine_flags(sofia_ctx *c) {
 }  // namespace
 
 namespace {
+
+// check_allocation after a box change (Source/mod_neighbor.f90:1434-1526): the reference grid
+// of every replica from its current box, per-cell arrays re-sized when the cell count grows.
+int reference_regrid(sofia_ctx *c) {
+    TRY(sync_check(c));
+    int cell0 = 0;
+    for (auto &R : c->reps_h) {
+        cell_grid(R.box, c->p.rcut, c->p.cut_coul, &R.rn, &R.ncellT, &R.ncellD);
+        if (R.ncellD < 3 || R.ncellD * R.ncellD * R.ncellD != R.ncellT)
+            return fail(SOFIA_E_ARG, "box too small for the cell grid after the box change");
+        R.stencil5 = 0;
+        R.cell0 = cell0;
+        cell0 += R.ncellT;
+    }
+    if (cell0 > c->ncell_cap) {
+        dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep);
+        TRY(dalloc(&c->ccount, cell0));
+        TRY(dalloc(&c->cstart, cell0));
+        TRY(dalloc(&c->cnum, (size_t)27 * cell0));
+        TRY(dalloc(&c->cell_rep, cell0));
+        SOFIA_HIP(hipMemset(c->ccount, 0, sizeof(int32_t) * cell0));
+        SOFIA_HIP(hipMemset(c->cstart, 0, sizeof(int32_t) * cell0));
+        c->ncell_cap = cell0;
+    }
+    c->ncell_total = cell0;
+    c->n_stencil5 = 0;
+    std::vector<int32_t> cnum((size_t)27 * cell0), crep(cell0);
+    for (int r = 0; r < c->nrep; ++r) {
+        const sofia::RepInfo &R = c->reps_h[r];
+        stencil(R.ncellT, R.ncellD, R.cell0, cnum.data() + (size_t)27 * R.cell0);
+        for (int k = 0; k < R.ncellT; ++k) crep[R.cell0 + k] = r;
+    }
+    TRY(upload(c->cnum, cnum.data(), sizeof(int32_t) * cnum.size()));
+    TRY(upload(c->cell_rep, crep.data(), sizeof(int32_t) * crep.size()));
+    TRY(upload(c->reps_d, c->reps_h.data(), sizeof(sofia::RepInfo) * c->nrep));
+    c->have_list = false;   // the sort and list refer to the old grid
+    return SOFIA_OK;
+}
 
 // The engine's cell grid and rebuild threshold, set once per context.
 //

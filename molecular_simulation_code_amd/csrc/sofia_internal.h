@@ -192,6 +192,7 @@ void launch_f64_to_coords(sofia_ctx *c, const double *f, float *out_aos);
 void launch_nve_v32(sofia_ctx *c, float dt_partial, const float *ff_aos);
 void launch_nve_x32(sofia_ctx *c);
 void launch_pbc32(sofia_ctx *c);
+void launch_berendsen32(sofia_ctx *c, const double *mu_dev);
 void launch_shake32(sofia_ctx *c, int iflag, float *ff_aos, double *vir_dev);
 void launch_langevin32(sofia_ctx *c, float fric, float gconst, uint64_t seed, uint64_t step,
                        float *out_aos);

@@ -324,6 +324,20 @@ class SofiaGPU:
         self._get_pos()
         self._get_dr()
 
+    def berendsen(self, mu: float):
+        """Source/mod_ensemble.f90:167-205 with the scale factor mu computed by the host from
+        the pressure: positions and drx/dry/drz on the device, then box / ibox (real*4)."""
+        self._put_pos()
+        self._put_dr()
+        m = np.array([mu], f64)
+        b = np.zeros(1, f32)
+        check(lib().sofia_berendsen(self.ctx, ptr(m), ptr(b)))
+        self._get_pos()
+        self._get_dr()
+        self.d["box"] = f32(b[0])
+        self.d["ibox"] = f32(1.0) / f32(b[0])
+        return float(b[0])
+
     def pbc(self, step: int = 0):
         """Source/mod_pbc.f90:6-39."""
         self._put_pos()

@@ -268,6 +268,16 @@ def pbc(pos, types, box):
     return a[:, :3].copy()
 
 
+def berendsen(pos, types, drx, dry, drz, mu, box):
+    """berendsen, Source/mod_ensemble.f90:167-205: (positions, drx, dry, drz, box, ibox)."""
+    a = ainfo(pos, types)
+    dx, dy, dz = (np.ascontiguousarray(x, _f).copy() for x in (drx, dry, drz))
+    b, ib = C.c_float(box), C.c_float(0.0)
+    lib().orc_berendsen(C.c_int32(len(pos)), P(a), P(dx), P(dy), P(dz), C.c_double(mu),
+                        C.byref(b), C.byref(ib))
+    return a[:, :3].copy(), dx, dy, dz, np.float32(b.value), np.float32(ib.value)
+
+
 def langevin(pos, types, v, mass, fric, gconst, r1, r2, r3):
     n = len(pos)
     a = ainfo(pos, types)

@@ -583,6 +583,24 @@ void orc_pbc(int32_t np, orc_ainfo *pos, float box) {
     }
 }
 
+/* berendsen, Source/mod_ensemble.f90:167-205, with the scale factor mu given: each
+ * coordinate becomes real*4(dble(x)*mu), drx/dry/drz add (new - dble(old)) in real*8 and
+ * store real*4; box = real*4(box*mu), ibox = 1.0/box in real*4. */
+void orc_berendsen(int32_t np, orc_ainfo *pos, float *drx, float *dry, float *drz, double mu,
+                   float *box, float *ibox) {
+    for (int i = 0; i < np; ++i) {
+        float *c[3] = {&pos[i].x, &pos[i].y, &pos[i].z};
+        float *dr[3] = {&drx[i], &dry[i], &drz[i]};
+        for (int d = 0; d < 3; ++d) {
+            const double cur = (double)*c[d];
+            *c[d] = (float)(cur * mu);
+            *dr[d] = (float)((double)*dr[d] + ((double)*c[d] - cur));
+        }
+    }
+    *box = (float)((double)*box * mu);
+    *ibox = 1.0f / *box;
+}
+
 /* langevin_debug_intel, Source/mod_langevin.f90:15-43, with the three uniform draws per
  * atom supplied by the caller: ff_lang = -fric*m*v + gconst*sqrt(m)*U(-1/2, 1/2). */
 void orc_langevin(int32_t np, const orc_ainfo *pos, const orc_coords *v, const double *mass,

@@ -91,6 +91,8 @@ void orc_nve_v(int32_t np, float dt_partial, const orc_ainfo *pos, const double 
 void orc_nve_x(int32_t np, float dt, const orc_coords *v, orc_ainfo *pos, float *drx,
                float *dry, float *drz);
 void orc_pbc(int32_t np, orc_ainfo *pos, float box);
+void orc_berendsen(int32_t np, orc_ainfo *pos, float *drx, float *dry, float *drz, double mu,
+                   float *box, float *ibox);
 void orc_langevin(int32_t np, const orc_ainfo *pos, const orc_coords *v, const double *mass,
                   float fric, float gconst, const float *r1, const float *r2, const float *r3,
                   orc_coords *ff_lang);

@@ -268,6 +268,43 @@ def test_integrator_pbc_shake_langevin(case):
         g.close()
 
 
+def test_berendsen_then_rebuild(case):
+    """berendsen on the device equals the oracle bit for bit (positions, drx/dry/drz, box), and
+    the next neighbor_wrapper sorts and lists exactly as the reference would on the new box."""
+    name, s, d, ref = case
+    g = _gpu()(s)
+    try:
+        g.neighbor_wrapper(0)
+        say(f"[gpu {name}] berendsen")
+        g.drx[:] = np.float32(0.013); g.dry[:] = np.float32(-0.02); g.drz[:] = np.float32(0.0)
+        pos0, types, d0 = g.position.copy(), g.type.copy(), (g.drx.copy(), g.dry.copy(), g.drz.copy())
+        box0 = np.float32(g.d["box"])
+        mu = 1.0 + 2.5e-4
+        box1 = g.berendsen(mu)
+        p, x, y, z, b, ib = O.berendsen(pos0, types, *d0, mu, box0)
+        assert np.array_equal(g.position, p)
+        assert np.array_equal(g.drx, x) and np.array_equal(g.dry, y) and np.array_equal(g.drz, z)
+        assert np.float32(box1) == b
+        say(f"[gpu {name}] rebuild on the new box")
+        num3, spec0 = g.num3bond.copy(), g.specbond.copy()
+        tt = g.neighbor_wrapper(0)
+        rn, nT, nD = O.cell_grid(b, d["rcut"], d["cut_coul"])
+        ott, perm, start, endp, atombin = O.sort_cells(p, types, b, rn, nT, nD)
+        assert np.array_equal(tt, ott)
+        inv = perm - 1
+        sp = spec0[inv]
+        spec = np.where(sp > 0, ott[np.maximum(sp, 1) - 1], 0).astype(np.int32)
+        onl, onn, ona = O.build_neighbor(p[inv], types[inv], atombin, start, endp, O.neighcell(nT, nD),
+                                         num3[inv], spec, b, ib, d["rv2"], S.NEIGH_ALLOC0)
+        nl, nn, na = g.get_nlist()
+        assert na == ona and np.array_equal(nn, onn)
+        a = nl.reshape(s.np, na); o = onl.reshape(s.np, ona)
+        for i in range(s.np):
+            assert np.array_equal(a[i, :nn[i]], o[i, :nn[i]])
+    finally:
+        g.close()
+
+
 def test_check_neighbor_closed_form():
     SofiaGPU = _gpu()
     s = S.make_system(10, 150, seed=3, params=S.Params(rcut=5.5, cut_coul=5.5, pad=1.0))

@@ -337,6 +337,30 @@ def test_constant_force_kinematics():
     np.testing.assert_allclose(drx, 2.0 * v1[:, 0], rtol=1e-6)
 
 
+def test_berendsen_scaling_closed_form():
+    """Box scaling by mu: identity at mu = 1; coordinates are mu * x to float rounding;
+    the displacement accumulators gain exactly the coordinate change; fractional
+    coordinates are preserved; box and ibox follow in real*4."""
+    rng = np.random.default_rng(7)
+    n = 400
+    L = np.float32(40.0)
+    pos = rng.uniform(0.01, float(L), (n, 3)).astype(np.float32)
+    types = rng.integers(1, 13, n).astype(np.int32)
+    d0 = [rng.normal(0, 0.1, n).astype(np.float32) for _ in range(3)]
+    p1, x1, y1, z1, b1, ib1 = O.berendsen(pos, types, *d0, 1.0, L)
+    assert np.array_equal(p1, pos) and b1 == L
+    assert all(np.array_equal(a, b) for a, b in zip((x1, y1, z1), d0))
+    mu = 1.0 + 3.7e-4
+    p2, x2, y2, z2, b2, ib2 = O.berendsen(pos, types, *d0, mu, L)
+    np.testing.assert_allclose(p2, pos.astype(np.float64) * mu, rtol=7e-8)
+    for k, dr in enumerate((x2, y2, z2)):
+        expect = (d0[k].astype(np.float64) + (p2[:, k].astype(np.float64) - pos[:, k].astype(np.float64)))
+        np.testing.assert_allclose(dr, expect, rtol=0, atol=4e-8 + 1e-7 * np.abs(expect).max())
+        assert np.all(np.sign(dr - d0[k]) >= 0)                    # expansion moves every x >= 0 up
+    np.testing.assert_allclose(p2 / b2, pos / L, rtol=3e-7)
+    assert b2 == np.float32(float(L) * mu) and ib2 == np.float32(1.0) / b2
+
+
 def test_langevin_friction_only():
     rng = np.random.default_rng(1)
     n = 50
