@@ -287,6 +287,7 @@ def test_berendsen_then_rebuild(case):
         assert np.float32(box1) == b
         say(f"[gpu {name}] rebuild on the new box")
         num3, spec0 = g.num3bond.copy(), g.specbond.copy()
+        na_before = g.get_nlist()[2]          # neigh_alloc persists across builds
         tt = g.neighbor_wrapper(0)
         rn, nT, nD = O.cell_grid(b, d["rcut"], d["cut_coul"])
         ott, perm, start, endp, atombin = O.sort_cells(p, types, b, rn, nT, nD)
@@ -295,7 +296,7 @@ def test_berendsen_then_rebuild(case):
         sp = spec0[inv]
         spec = np.where(sp > 0, ott[np.maximum(sp, 1) - 1], 0).astype(np.int32)
         onl, onn, ona = O.build_neighbor(p[inv], types[inv], atombin, start, endp, O.neighcell(nT, nD),
-                                         num3[inv], spec, b, ib, d["rv2"], S.NEIGH_ALLOC0)
+                                         num3[inv], spec, b, ib, d["rv2"], na_before)
         nl, nn, na = g.get_nlist()
         assert na == ona and np.array_equal(nn, onn)
         a = nl.reshape(s.np, na); o = onl.reshape(s.np, ona)
