@@ -275,6 +275,7 @@ def test_berendsen_then_rebuild(case):
     g = _gpu()(s)
     try:
         g.neighbor_wrapper(0)
+        na_before = g.get_nlist()[2]          # neigh_alloc persists across builds
         say(f"[gpu {name}] berendsen")
         g.drx[:] = np.float32(0.013); g.dry[:] = np.float32(-0.02); g.drz[:] = np.float32(0.0)
         pos0, types, d0 = g.position.copy(), g.type.copy(), (g.drx.copy(), g.dry.copy(), g.drz.copy())
@@ -287,7 +288,6 @@ def test_berendsen_then_rebuild(case):
         assert np.float32(box1) == b
         say(f"[gpu {name}] rebuild on the new box")
         num3, spec0 = g.num3bond.copy(), g.specbond.copy()
-        na_before = g.get_nlist()[2]          # neigh_alloc persists across builds
         tt = g.neighbor_wrapper(0)
         rn, nT, nD = O.cell_grid(b, d["rcut"], d["cut_coul"])
         ott, perm, start, endp, atombin = O.sort_cells(p, types, b, rn, nT, nD)
