@@ -243,8 +243,12 @@ contains
 
   ! neighbor_wrapper(step): the device sorts and builds the list; the returned tt re-indexes
   ! the host arrays the device does not hold (mol, integrate_flag), as sort_posit_charge does.
+  ! neighbor_wrapper, Source/mod_neighbor.f90:8-86.  The device sorts and lists; the host
+  ! arrays the device does not return are re-indexed through tt here, as sort_posit_charge
+  ! and the sort_* routines do (Source/mod_neighbor.f90:946-1168): q, mol, integrate_flag,
+  ! num1/2/3bond + specbond, and the bond, angle, dihedral, improper and SHAKE lists.
   subroutine neighbor_wrapper_hip(step)
-    integer :: step, i
+    integer :: step, i, j
     nn_builds = nn_builds + 1
     call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
     call sofia_check(sofia_neighbor_wrapper(ctx, int(step, c_int32_t), tt_buf), 'neighbor_wrapper')
@@ -253,10 +257,48 @@ contains
        qs(tt_buf(i)) = q(i)
        molsort(:, tt_buf(i)) = mol(:, i)
        integrate_flag_ss(tt_buf(i)) = integrate_flag(i)
+       num1bondss(tt_buf(i)) = num1bond(i)
+       num2bondss(tt_buf(i)) = num2bond(i)
+       num3bondss(tt_buf(i)) = num3bond(i)
+       do j = 1, num3bond(i)
+          specbond_ss(j, tt_buf(i)) = tt_buf(specbond(j, i))
+       end do
     end do
     q(1:np) = qs(1:np)
     mol(:,:) = molsort(:,:)
     integrate_flag(:) = integrate_flag_ss(:)
+    specbond(:,:) = specbond_ss(:,:)
+    num1bond(:) = num1bondss(:)
+    num2bond(:) = num2bondss(:)
+    num3bond(:) = num3bondss(:)
+    do i = 1, numbond
+       bondlist(1:2, i) = tt_buf(bondlist(1:2, i))
+    end do
+    do i = 1, numangle
+       anglelist(1:3, i) = tt_buf(anglelist(1:3, i))
+    end do
+    do i = 1, numdihed
+       dihedlist(1:4, i) = tt_buf(dihedlist(1:4, i))
+    end do
+    do i = 1, numimpro
+       improlist(1:4, i) = tt_buf(improlist(1:4, i))
+    end do
+    do i = 1, nshake
+       select case (shakeatom(i)%num)
+       case (1)
+          shakeatom(i)%atom1 = tt_buf(shakeatom(i)%atom1)
+          shakeatom(i)%atom2 = tt_buf(shakeatom(i)%atom2)
+       case (3, -3)
+          shakeatom(i)%atom1 = tt_buf(shakeatom(i)%atom1)
+          shakeatom(i)%atom2 = tt_buf(shakeatom(i)%atom2)
+          shakeatom(i)%atom3 = tt_buf(shakeatom(i)%atom3)
+       case (4)
+          shakeatom(i)%atom1 = tt_buf(shakeatom(i)%atom1)
+          shakeatom(i)%atom2 = tt_buf(shakeatom(i)%atom2)
+          shakeatom(i)%atom3 = tt_buf(shakeatom(i)%atom3)
+          shakeatom(i)%atom4 = tt_buf(shakeatom(i)%atom4)
+       end select
+    end do
     drx(:) = 0.0; dry(:) = 0.0; drz(:) = 0.0
   end subroutine neighbor_wrapper_hip
 
