@@ -107,6 +107,14 @@ int sofia_set_bonded_coeffs(sofia_ctx *ctx, int32_t nbt, const double *bond_k,
                             const double *angle_th0, int32_t ntt, const double *tors_k,
                             const double *tors_n, const double *tors_shift,
                             const int32_t *torsions, int32_t ntor, const int32_t *ntor_rep);
+/* Improper dihedrals for the engine: improper_harmonic_lammps14
+ * (Source/mod_force_dihed.f90:1546-1724; force wrapper call Source/mod_force.f90:542-544)
+ * E = K (chi - chi0)^2, chi the angle between the planes (i1, i2, i3) and (i2, i3, i4).
+ * k, chi0 (rad) per improper type (improcoeff(1:2, :), Source/global.f90:455);
+ * improlist(5, ni) = i1, i2, i3 (central), i4 (1-based per replica), type
+ * (Source/global.f90:450); ni_rep per-replica counts (NULL for one replica). */
+int sofia_set_impropers(sofia_ctx *ctx, int32_t nit, const double *k, const double *chi0,
+                        const int32_t *improlist, int32_t ni, const int32_t *ni_rep);
 /* SHAKE clusters: num (1, 3, 4, -3), atoms(4, ns) (0 = unused), bond lengths(3, ns)
  * (Source/global.f90:351-355 shake_info). */
 int sofia_set_shake(sofia_ctx *ctx, int32_t ns, const int32_t *ns_rep, const int32_t *num,
@@ -232,8 +240,9 @@ int sofia_md_resume_forces(sofia_ctx *ctx, const double *f, const int32_t *have)
  * moves along its force by min(gamma*|F|, max_disp) A per step; ends with a full force
  * evaluation so sofia_md_run can continue.  For relaxing generated starting structures. */
 int sofia_minimize(sofia_ctx *ctx, int32_t nsteps, double gamma, double max_disp);
-/* Energies per replica after the last step, 8 doubles each: potential (LJ), e_coul,
- * pot_14, e_coul_14, e_bond, e_angle, e_dihedral, ke. */
+/* Energies per replica after the last step, SOFIA_NENERGY doubles each: potential (LJ),
+ * e_coul, pot_14, e_coul_14, e_bond, e_angle, e_dihedral, ke, e_improper. */
+#define SOFIA_NENERGY 9
 int sofia_md_energies(sofia_ctx *ctx, double *out);
 /* Number of list builds per replica since md_init (nrep ints). */
 int sofia_md_rebuilds(sofia_ctx *ctx, int32_t *out);
@@ -253,7 +262,7 @@ int sofia_profile_copy(sofia_ctx *ctx, int64_t bytes, int32_t launches, double *
 /* Runs nsteps MD steps eagerly with HIP events between the phases of a step and returns
  * the average ms per step of each phase in phase_ms[SOFIA_NPHASE]: half-kick, drift,
  * displacement check, cell sort + gather + topology remap, list build, rebuild
- * bookkeeping, force zeroing, pair kernel, bonded (1-4, bond, angle, torsion), Langevin,
+ * bookkeeping, force zeroing, pair kernel, bonded (1-4, bond, angle, torsion, improper), Langevin,
  * SHAKE, second half-kick.  full_rebuild != 0 makes every replica rebuild on every step
  * (the cost of one full rebuild).  Energies are not reduced on these steps. */
 enum { SOFIA_NPHASE = 12 };

@@ -17,6 +17,7 @@ SOFIA_OK = 0
 ERRORS = {-1: "E_NODEV", -2: "E_OOM", -3: "E_ARG", -4: "E_OVERFLOW", -5: "E_HIP",
           -6: "E_SHAKE", -7: "E_STATE"}
 PREC_FP64, PREC_MIXED = 0, 1
+NENERGY = 9     # SOFIA_NENERGY: energies per replica returned by sofia_md_energies
 COUL_DSF, COUL_EWALD = 1, 2
 
 
@@ -63,6 +64,7 @@ SIGNATURES = {
     "sofia_set_bonded": [_vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp],
     "sofia_set_bonded_coeffs": [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp,
                                 _i32, _vp],
+    "sofia_set_impropers": [_vp, _i32, _vp, _vp, _vp, _i32, _vp],
     "sofia_set_shake": [_vp, _i32, _vp, _vp, _vp, _vp],
     "sofia_put_state": [_vp] * 5,
     "sofia_get_state": [_vp] * 4,

@@ -1,5 +1,6 @@
 // Bonded-pair kernels: Ewald exclusion corrections, the scaled 1-4 DSF pair, and the
-// harmonic bond / harmonic angle / cosine torsion terms of the full MD step.
+// harmonic bond / harmonic angle / cosine torsion / harmonic improper terms of the full MD
+// step.
 //
 // One thread per list item; forces go to a per-atom int64 fixed-point accumulator with
 // atomics (2^-32 kcal/mol/A quanta), so totals are exact and independent of thread
@@ -258,6 +259,65 @@ __global__ void k_torsion(int32_t items, const int32_t *__restrict__ tl, const d
     wave_add(to_efix(e), live, r, eacc, E_DIHED);
 }
 
+// improper_harmonic_lammps14, Source/mod_force_dihed.f90:1546-1724: E = K (chi - chi0)^2
+// with c = cos(chi) from the bond-vector cosines c0, c1, c2 and the LAMMPS improper_harmonic
+// gradient coefficients a11..a33; SMALL is the reference's real*4 literal 0.001.
+__global__ void k_improper(int32_t items, const int32_t *__restrict__ il, const double *__restrict__ xd,
+                           const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
+                           const double *__restrict__ kk, const double *__restrict__ chi0,
+                           int32_t n, long long *__restrict__ fout, long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < items;
+    double e = 0.0;
+    int r = 0;
+    if (live) {
+        constexpr double SMALL = (double)0.001f;
+        const int i1 = il[(size_t)m * 5], i2 = il[(size_t)m * 5 + 1], i3 = il[(size_t)m * 5 + 2];
+        const int i4 = il[(size_t)m * 5 + 3], t = il[(size_t)m * 5 + 4] - 1;
+        r = rep[i1];
+        const double bx = reps[r].box, ib = reps[r].ibox;
+        double v1[3], v2[3], v3[3];
+        dvec64(xd, n, i1, i2, bx, ib, v1);
+        dvec64(xd, n, i3, i2, bx, ib, v2);
+        dvec64(xd, n, i4, i3, bx, ib, v3);
+        const double ss1 = 1.0 / (v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+        const double ss2 = 1.0 / (v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+        const double ss3 = 1.0 / (v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+        const double r1 = sqrt(ss1), r2 = sqrt(ss2), r3 = sqrt(ss3);
+        const double c0 = (v1[0] * v3[0] + v1[1] * v3[1] + v1[2] * v3[2]) * r1 * r3;
+        const double c1 = (v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]) * r1 * r2;
+        const double c2 = -(v3[0] * v2[0] + v3[1] * v2[1] + v3[2] * v2[2]) * r3 * r2;
+        const double s1 = 1.0 / fmax(1.0 - c1 * c1, SMALL);
+        const double s2 = 1.0 / fmax(1.0 - c2 * c2, SMALL);
+        double s12 = sqrt(s1 * s2);
+        double c = fmin(fmax((c1 * c2 + c0) * s12, -1.0), 1.0);
+        const double s = fmax(sqrt(1.0 - c * c), SMALL);
+        const double domega = acos(c) - chi0[t];
+        double a = kk[t] * domega;
+        e = a * domega;
+        a = -a * 2.0 / s;
+        c *= a;
+        s12 *= a;
+        const double a11 = c * ss1 * s1;
+        const double a22 = -ss2 * (2.0 * c0 * s12 - c * (s1 + s2));
+        const double a33 = c * ss3 * s2;
+        const double a12 = -r1 * r2 * (c1 * c * s1 + c2 * s12);
+        const double a13 = -r1 * r3 * s12;
+        const double a23 = r2 * r3 * (c2 * c * s2 + c1 * s12);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double sx2 = a22 * v2[k] + a23 * v3[k] + a12 * v1[k];
+            const double f1 = a12 * v2[k] + a13 * v3[k] + a11 * v1[k];
+            const double f4 = a23 * v2[k] + a33 * v3[k] + a13 * v1[k];
+            atomic_fix(&fout[(size_t)k * n + i1], f1);
+            atomic_fix(&fout[(size_t)k * n + i2], -sx2 - f1);
+            atomic_fix(&fout[(size_t)k * n + i3], sx2 - f4);
+            atomic_fix(&fout[(size_t)k * n + i4], f4);
+        }
+    }
+    wave_add(to_efix(e), live, r, eacc, E_IMPRO);
+}
+
 }  // namespace
 
 void launch_correct(sofia_ctx *c, int which, long long *fout) {
@@ -299,6 +359,10 @@ void launch_bonded_engine(sofia_ctx *c) {
         hipLaunchKernelGGL(k_torsion, dim3(nblocks(c->ntor, 256)), dim3(256), 0, c->stream, c->ntor,
                            c->torsions, c->a.xd, c->rep, c->reps_d, c->tors_k, c->tors_n,
                            c->tors_shift, c->tors_cs, n, c->fix, c->eacc);
+    if (c->nimp)
+        hipLaunchKernelGGL(k_improper, dim3(nblocks(c->nimp, 256)), dim3(256), 0, c->stream, c->nimp,
+                           c->improlist, c->a.xd, c->rep, c->reps_d, c->impro_k, c->impro_chi0, n,
+                           c->fix, c->eacc);
 }
 
 }  // namespace sofia

@@ -390,6 +390,7 @@ void launch_remap_topology(sofia_ctx *c) {
     if (c->na) hipLaunchKernelGGL(k_remap, dim3(nblk(c->na, 256)), dim3(256), 0, c->stream, c->na, 4, 3, c->anglelist, c->tt);
     if (c->nd) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nd, 256)), dim3(256), 0, c->stream, c->nd, 5, 4, c->dihedlist, c->tt);
     if (c->ntor) hipLaunchKernelGGL(k_remap, dim3(nblk(c->ntor, 256)), dim3(256), 0, c->stream, c->ntor, 5, 4, c->torsions, c->tt);
+    if (c->nimp) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nimp, 256)), dim3(256), 0, c->stream, c->nimp, 5, 4, c->improlist, c->tt);
     if (c->ns) hipLaunchKernelGGL(k_remap, dim3(nblk(c->ns, 256)), dim3(256), 0, c->stream, c->ns, 4, 4, c->shake_atoms, c->tt);
     if (c->nbe) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nbe, 256)), dim3(256), 0, c->stream, c->nbe, 3, 2, c->ebond, c->tt);
     if (c->nae) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nae, 256)), dim3(256), 0, c->stream, c->nae, 4, 3, c->eangle, c->tt);

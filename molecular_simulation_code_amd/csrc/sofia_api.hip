@@ -229,6 +229,7 @@ int split_water(sofia_ctx *c) {
     };
     TRY(touch(c->dihedlist, c->nd, 5, 4));
     TRY(touch(c->torsions, c->ntor, 5, 4));
+    TRY(touch(c->improlist, c->nimp, 5, 4));
     for (int m = 0; m < c->ns; ++m)
         for (int k = 0; k < 4; ++k) {
             const int i = at[4 * (size_t)m + k];
@@ -360,7 +361,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     c->N = atom0;
     c->ncell_total = cell0;
     c->ncell_cap = cell0;
-    c->last_e.assign((size_t)nrep * 8, 0.0);
+    c->last_e.assign((size_t)nrep * SOFIA_NENERGY, 0.0);
     const int n = c->N;
     TRY(alloc_atoms(c->a, n));
     TRY(alloc_atoms(c->b, n));
@@ -509,7 +510,8 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms);
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
-    dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
+    dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->improlist); dfree(c->impro_k);
+    dfree(c->impro_chi0); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -635,6 +637,32 @@ int sofia_set_bonded_coeffs(sofia_ctx *c, int32_t nbt, const double *bk, const d
         c->ntor = ntor;
     }
     c->nbt = nbt; c->nat = nat; c->ntt = ntt;
+    return SOFIA_OK;
+}
+
+int sofia_set_impropers(sofia_ctx *c, int32_t nit, const double *k, const double *chi0,
+                        const int32_t *improlist, int32_t ni, const int32_t *ni_rep) {
+    if (!c || nit < 0 || ni < 0) return fail(SOFIA_E_ARG, "bad argument");
+    if (ni > 0 && (nit == 0 || !k || !chi0 || !improlist))
+        return fail(SOFIA_E_ARG, "impropers need their type coefficients");
+    TRY(unsplit_water(c));
+    dfree(c->impro_k); dfree(c->impro_chi0); dfree(c->improlist);
+    c->nimp = 0;
+    if (!ni) return SOFIA_OK;
+    for (int m = 0; m < ni; ++m) {
+        const int t = improlist[5 * (size_t)m + 4];
+        if (t < 1 || t > nit) return fail(SOFIA_E_ARG, "improper type out of range");
+        for (int w = 0; w < 4; ++w)
+            if (improlist[5 * (size_t)m + w] == 0) return fail(SOFIA_E_ARG, "improper with a missing atom");
+    }
+    TRY(dalloc(&c->impro_k, nit)); TRY(dalloc(&c->impro_chi0, nit));
+    TRY(upload(c->impro_k, k, sizeof(double) * nit));
+    TRY(upload(c->impro_chi0, chi0, sizeof(double) * nit));
+    std::vector<int32_t> g;
+    TRY(to_global_list(c, improlist, ni, 5, 4, ni_rep, g));
+    TRY(dalloc(&c->improlist, g.size()));
+    TRY(upload(c->improlist, g.data(), sizeof(int32_t) * g.size()));
+    c->nimp = ni;
     return SOFIA_OK;
 }
 
@@ -1090,10 +1118,10 @@ int collect_energies(sofia_ctx *c) {
     std::vector<long long> e;
     TRY(read_energies(c, e));
     for (int r = 0; r < c->nrep; ++r) {
-        double *o = c->last_e.data() + (size_t)r * 8;
+        double *o = c->last_e.data() + (size_t)r * SOFIA_NENERGY;
         o[0] = E(e, r, E_LJ); o[1] = E(e, r, E_COUL); o[2] = E(e, r, E_POT14);
         o[3] = E(e, r, E_COUL14); o[4] = E(e, r, E_BOND); o[5] = E(e, r, E_ANGLE);
-        o[6] = E(e, r, E_DIHED); o[7] = E(e, r, E_KE);
+        o[6] = E(e, r, E_DIHED); o[7] = E(e, r, E_KE); o[8] = E(e, r, E_IMPRO);
     }
     return SOFIA_OK;
 }

@@ -22,7 +22,7 @@ constexpr int PAIR_SHIFT = 8;                // 2^32 / 2^24
 // energy / virial accumulator slots (int64 fixed point, per replica)
 enum {
     E_LJ = 0, E_COUL, E_POT14, E_COUL14, E_BOND, E_ANGLE, E_DIHED, E_KE,
-    V_X, V_Y, V_Z, E_CORR, NE
+    V_X, V_Y, V_Z, E_CORR, E_IMPRO, NE
 };
 
 // Per-replica geometry and thermostat (device table).
@@ -120,6 +120,7 @@ struct sofia_ctx {
     int32_t *anglelist = nullptr, na = 0;
     int32_t *dihedlist = nullptr, nd = 0;
     int32_t *torsions = nullptr, ntor = 0;
+    int32_t *improlist = nullptr, nimp = 0;   // (i1, i2, i3 central, i4, type) per improper
     int32_t *shake_num = nullptr, *shake_atoms = nullptr, ns = 0;
     double *shake_bond = nullptr;
     int4 *water = nullptr;       // per SHAKE cluster: rigid-water bond/angle types + fused flag
@@ -134,6 +135,7 @@ struct sofia_ctx {
     double *bond_k = nullptr, *bond_r0 = nullptr, *angle_k = nullptr, *angle_th0 = nullptr;
     double *tors_k = nullptr, *tors_n = nullptr, *tors_shift = nullptr;
     double *tors_cs = nullptr;  // per torsion type: cos(shift), sin(shift)
+    double *impro_k = nullptr, *impro_chi0 = nullptr;   // per improper type
     float4 *dsf_tab = nullptr;  // cubic pieces of the DSF Coulomb force / energy (mixed kernel)
     int32_t nbt = 0, nat = 0, ntt = 0;
     bool have_ff = false;

@@ -20,8 +20,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import system as S
-from ._lib import (COUL_DSF, COUL_EWALD, PREC_FP64, PREC_MIXED, SofiaParams, SofiaThermostat,
-                   check, lib, ptr)
+from ._lib import (COUL_DSF, COUL_EWALD, NENERGY, PREC_FP64, PREC_MIXED, SofiaParams,
+                   SofiaThermostat, check, lib, ptr)
 
 f32, f64, i32 = np.float32, np.float64, np.int32
 
@@ -80,6 +80,16 @@ def _bonded_coeffs(ctx, tors: np.ndarray, ntor_rep=None):
                                         len(tk), ptr(tk), ptr(tn), ptr(ts), ptr(tors), len(tors),
                                         ptr(nr)))
     return keep
+
+
+def _impropers(ctx, lists):
+    """improper_harmonic_lammps14 items (i1, i2, i3 central, i4, type) per replica."""
+    ik = _c([c[0] for c in S.IMPRO_K_CHI0], f64)
+    ic = _c([c[1] for c in S.IMPRO_K_CHI0], f64)
+    il = _c(np.concatenate(lists) if lists else np.zeros((0, 5), i32), i32)
+    nr = _c([len(x) for x in lists], i32)
+    check(lib().sofia_set_impropers(ctx, len(ik), ptr(ik), ptr(ic), ptr(il), len(il), ptr(nr)))
+    return [ik, ic, il, nr]
 
 
 class SofiaGPU:
@@ -420,6 +430,7 @@ class Engine:
             tl = [torsion_list(r.system) for r in reps]
             self._keep += _bonded_coeffs(h, np.concatenate(tl) if tl else np.zeros((0, 5), i32),
                                          [len(t) for t in tl])
+            self._keep += _impropers(h, [r.system.improlist for r in reps])
         sn = cat(lambda s: s.shake_num, i32)
         sa = cat(lambda s: s.shake_atoms, i32)
         sb = cat(lambda s: s.shake_bond, f64)
@@ -528,10 +539,11 @@ class Engine:
         check(lib().sofia_synchronize(self.ctx))
 
     def energies(self) -> np.ndarray:
-        """(nrep, 8): potential, e_coul, pot_14, e_coul_14, e_bond, e_angle, e_dihedral, ke."""
-        out = np.empty(len(self.reps) * 8, f64)
+        """(nrep, 9): potential, e_coul, pot_14, e_coul_14, e_bond, e_angle, e_dihedral, ke,
+        e_improper."""
+        out = np.empty(len(self.reps) * NENERGY, f64)
         check(lib().sofia_md_energies(self.ctx, ptr(out)))
-        return out.reshape(-1, 8)
+        return out.reshape(-1, NENERGY)
 
     def rebuilds(self) -> np.ndarray:
         out = np.empty(len(self.reps), i32)

@@ -75,6 +75,11 @@ ANGLE_COEFF = [(68.03, 123.11), (78.17, 130.38), (65.07, 114.21), (47.23, 109.54
                (75.00, 109.47)]
 DIHED_COEFF = [(0.0000, 2, 180), (0.1556, 3, 0)]
 IMPRO_COEFF = [(1.1000, -1, 2)]
+# improper types as the engine takes them: (K kcal/mol/rad^2, chi0 rad).  The reference
+# reads K from the line above but sets improcoeff(2, :) from `theta0`, a local that the
+# read never assigns (Source/mod_readfile.f90:230, 427-433); chi0 = 0 is the value it
+# holds in zero-initialised storage and keeps the improper a planarity restraint.
+IMPRO_K_CHI0 = [(1.1000, 0.0)]
 
 
 @dataclass

@@ -342,6 +342,20 @@ def torsion_cos(torsions, pos, types, k, mult, shift_deg, box, ibox):
     return f, e.value
 
 
+def improper_harmonic(improlist, pos, types, k, chi0, box, ibox):
+    """improper_harmonic_lammps14: (forces (n, 3), e_improper, virial (3,))."""
+    n = len(pos)
+    a = ainfo(pos, types)
+    f = np.zeros((n, 3), _d)
+    e = C.c_double(0.0)
+    vir = np.zeros(3, _d)
+    il = np.ascontiguousarray(improlist, _i)
+    lib().orc_improper_harmonic(C.c_int32(len(il)), P(il), P(a), P(np.ascontiguousarray(k, _d)),
+                                P(np.ascontiguousarray(chi0, _d)), C.c_float(box),
+                                C.c_float(ibox), P(f), C.byref(e), P(vir))
+    return f, e.value, vir
+
+
 # ---------------------------------------------------------------- reference library front end
 class Ref:
     """The reference's Fortran routines (oracle/_ref/libsofia_ref.so)."""
@@ -423,6 +437,16 @@ class Ref:
         th0 = np.ascontiguousarray(th0, _d)
         self.L.ref_angle(C.c_int32(len(lst)), P(lst), C.c_int32(len(k)), P(k), P(th0), P(f), C.byref(e))
         return f, e.value
+
+    def improper(self, improlist, k, chi0):
+        lst = np.ascontiguousarray(improlist, _i)
+        k = np.ascontiguousarray(k, _d); chi0 = np.ascontiguousarray(chi0, _d)
+        f = np.zeros((self.n, 3), _f)
+        e = C.c_double(0.0)
+        vir = np.zeros(3, _d)
+        self.L.ref_improper(C.c_int32(len(lst)), P(lst), C.c_int32(len(k)), P(k), P(chi0), P(f),
+                            C.byref(e), P(vir))
+        return f, e.value, vir
 
     def pbc(self, pos):
         xyz = np.ascontiguousarray(pos, _f).copy()

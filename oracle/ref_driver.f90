@@ -225,6 +225,31 @@ contains
     ea = e_angle
   end subroutine ref_angle
 
+  ! improper_harmonic_lammps14 (Source/mod_force_dihed.f90:1546-1724).
+  subroutine ref_improper(n, lst, nit, k, chi0, fout, ei, vir) bind(c, name="ref_improper")
+    integer(c_int), value :: n, nit
+    integer(c_int), intent(in) :: lst(5, n)
+    real(c_double), intent(in) :: k(nit), chi0(nit)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: ei, vir(3)
+    integer :: i
+    if (allocated(improlist)) deallocate(improlist)
+    allocate(improlist(5, n)); improlist = lst; numimpro = n
+    if (allocated(improcoeff)) deallocate(improcoeff)
+    allocate(improcoeff(2, nit)); improcoeff(1, :) = k; improcoeff(2, :) = chi0
+    if (allocated(ff_impro)) deallocate(ff_impro)
+    allocate(ff_impro(np))
+    do i = 1, np
+       ff_impro(i) = coords(0.0, 0.0, 0.0)
+    end do
+    e_improper = 0.0d0
+    v_improx = 0.0d0; v_improy = 0.0d0; v_improz = 0.0d0
+    call improper_harmonic_lammps14(0)
+    call get_forces(ff_impro, fout)
+    ei = e_improper
+    vir(1) = v_improx; vir(2) = v_improy; vir(3) = v_improz
+  end subroutine ref_improper
+
   ! pbc (Source/mod_pbc.f90:6-39) on xyz in place.
   subroutine ref_pbc(xyz) bind(c, name="ref_pbc")
     real(c_float), intent(inout) :: xyz(3, np)

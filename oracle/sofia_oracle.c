@@ -943,3 +943,67 @@ void orc_torsion_cos(int32_t nd, const int32_t *torsions, const orc_ainfo *pos,
     }
     *e_dihed = e;
 }
+
+/* improper_harmonic_lammps14, Source/mod_force_dihed.f90:1546-1724: E = K (chi - chi0)^2
+ * with chi the angle between the planes (1,2,3) and (2,3,4) of improlist(1:4) (third atom
+ * central), the LAMMPS improper_harmonic gradient, real*4 coordinate differences wrapped
+ * by minimg (Source/mod_minimg.f90:5-15), and SMALL = 0.001 as a real*4 literal stored in
+ * a double.  virial[3] gets v1.f1 + v2.f3 + (v3 + v2).f4 per component
+ * (Source/mod_force_dihed.f90:1712-1717). */
+void orc_improper_harmonic(int32_t ni, const int32_t *improlist, const orc_ainfo *pos,
+                           const double *k, const double *chi0, float box, float ibox,
+                           double *f, double *e_improper, double *virial) {
+    const double small = (double)0.001f;
+    double e = *e_improper;
+    for (int m = 0; m < ni; ++m) {
+        const int i1 = improlist[5 * m] - 1, i2 = improlist[5 * m + 1] - 1;
+        const int i3 = improlist[5 * m + 2] - 1, i4 = improlist[5 * m + 3] - 1;
+        const int t = improlist[5 * m + 4] - 1;
+        double v1[3], v2[3], v3[3];
+        dvec(pos, i1, i2, box, ibox, v1);
+        dvec(pos, i3, i2, box, ibox, v2);
+        dvec(pos, i4, i3, box, ibox, v3);
+        const double ss1 = 1.0 / (v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+        const double ss2 = 1.0 / (v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+        const double ss3 = 1.0 / (v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+        const double r1 = sqrt(ss1), r2 = sqrt(ss2), r3 = sqrt(ss3);
+        const double c0 = (v1[0] * v3[0] + v1[1] * v3[1] + v1[2] * v3[2]) * r1 * r3;
+        const double c1 = (v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]) * r1 * r2;
+        const double c2 = -(v3[0] * v2[0] + v3[1] * v2[1] + v3[2] * v2[2]) * r3 * r2;
+        double s1 = 1.0 - c1 * c1;
+        if (s1 < small) s1 = small;
+        s1 = 1.0 / s1;
+        double s2 = 1.0 - c2 * c2;
+        if (s2 < small) s2 = small;
+        s2 = 1.0 / s2;
+        double s12 = sqrt(s1 * s2);
+        double c = (c1 * c2 + c0) * s12;
+        if (c > 1.0) c = 1.0;
+        if (c < -1.0) c = -1.0;
+        double s = sqrt(1.0 - c * c);
+        if (s < small) s = small;
+        const double domega = acos(c) - chi0[t];
+        double a = k[t] * domega;
+        e += a * domega;
+        a = -a * 2.0 / s;
+        c = c * a;
+        s12 = s12 * a;
+        const double a11 = c * ss1 * s1;
+        const double a22 = -ss2 * (2.0 * c0 * s12 - c * (s1 + s2));
+        const double a33 = c * ss3 * s2;
+        const double a12 = -r1 * r2 * (c1 * c * s1 + c2 * s12);
+        const double a13 = -r1 * r3 * s12;
+        const double a23 = r2 * r3 * (c2 * c * s2 + c1 * s12);
+        for (int q = 0; q < 3; ++q) {
+            const double sx2 = a22 * v2[q] + a23 * v3[q] + a12 * v1[q];
+            const double f1 = a12 * v2[q] + a13 * v3[q] + a11 * v1[q];
+            const double f2 = -sx2 - f1;
+            const double f4 = a23 * v2[q] + a33 * v3[q] + a13 * v1[q];
+            const double f3 = sx2 - f4;
+            f[3 * i1 + q] += f1; f[3 * i2 + q] += f2;
+            f[3 * i3 + q] += f3; f[3 * i4 + q] += f4;
+            if (virial) virial[q] = virial[q] + v1[q] * f1 + v2[q] * f3 + (v3[q] + v2[q]) * f4;
+        }
+    }
+    *e_improper = e;
+}

@@ -103,7 +103,7 @@ int32_t orc_shake(int32_t iflag, int32_t np, const orc_ainfo *pos, const orc_coo
                   int32_t *max_iter_seen);
 
 /* --- bonded terms (Source/mod_force_bond.f90, mod_force_angle.f90,
- *     mod_force_dihed_amber.f90 cosine series) --- */
+ *     mod_force_dihed_amber.f90 cosine series, mod_force_dihed.f90 improper) --- */
 void orc_bond_harmonic(int32_t nb, const int32_t *bondlist, const orc_ainfo *pos,
                        const double *kb, const double *r0, float box, float ibox,
                        double *f, double *e_bond);
@@ -113,6 +113,11 @@ void orc_angle_harmonic(int32_t na, const int32_t *anglelist, const orc_ainfo *p
 void orc_torsion_cos(int32_t nd, const int32_t *torsions, const orc_ainfo *pos,
                      const double *k, const double *mult, const double *shift_deg, float box,
                      float ibox, double *f, double *e_dihed);
+/* improper_harmonic_lammps14 (Source/mod_force_dihed.f90:1546-1724); improlist(5, ni) =
+ * i1, i2, i3 (central), i4, type; k, chi0 (rad) per type; virial may be NULL. */
+void orc_improper_harmonic(int32_t ni, const int32_t *improlist, const orc_ainfo *pos,
+                           const double *k, const double *chi0, float box, float ibox,
+                           double *f, double *e_improper, double *virial);
 
 #ifdef __cplusplus
 }

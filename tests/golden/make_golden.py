@@ -11,6 +11,7 @@ the outputs of the reference Fortran routines on it:
 * dihedral_opls_DSF (zero torsion k)   1-4 pair forces, e_coul_14, pot_14
 * lj_cut_coul_dsf_N2                   forces, potential, e_coul
 * bond_harmonic, angle_harmonic        forces, energies
+* improper_harmonic_lammps14           forces, e_improper, virial (two improper types)
 * pbc                                  wrapped coordinates
 
 The Verlet list these routines consume is the oracle's (mod_neighbor.f90 needs Intel's
@@ -41,6 +42,10 @@ CASES = {
     "gly100_w600_rc10": (100, 600, 12, 10.0, 10.0, 2.0, 0.2, 0.3),
 }
 
+# improper types: (K kcal/mol/rad^2, chi0 rad); the second has a non-zero equilibrium angle
+IMPRO_K = np.array([1.1, 0.6])
+IMPRO_CHI0 = np.array([0.0, 0.35])
+
 
 def pair_digest(nlist, numneigh, neigh_alloc, ids=None):
     """sha256 of the sorted (i, j) pair set of a Verlet list (1-based, optionally mapped
@@ -69,7 +74,9 @@ def sorted_system(case):
     bl = s.bondlist.copy(); bl[:, :2] = tt[bl[:, :2] - 1]
     al = s.anglelist.copy(); al[:, :3] = tt[al[:, :3] - 1]
     dl = s.dihedlist.copy(); dl[:, :4] = tt[dl[:, :4] - 1]
-    out.update(bondlist=bl, anglelist=al, dihedlist=dl, start=start, endposit=endp,
+    il = s.improlist.copy(); il[:, :4] = tt[il[:, :4] - 1]
+    il[1::2, 4] = 2
+    out.update(bondlist=bl, anglelist=al, dihedlist=dl, improlist=il, start=start, endposit=endp,
                atombin=atombin, tt=tt, perm=perm,
                box=np.float32(d["box"]), rcut=np.float32(rcut), cut_coul=np.float32(cc),
                pad=np.float32(pad), alpha=np.float64(alpha), gewald=np.float64(gewald))
@@ -102,6 +109,8 @@ def main():
         th = np.array([c[1] for c in S.ANGLE_COEFF]) * (2.0 * np.arccos(-1.0) / 360.0)
         g["bond_f"], g["bond_e"] = R.bond(g["bondlist"], kb, r0)
         g["angle_f"], g["angle_e"] = R.angle(g["anglelist"], ka, th)
+        g["impro_k"], g["impro_chi0"] = IMPRO_K, IMPRO_CHI0
+        g["impro_f"], g["impro_e"], g["impro_vir"] = R.improper(g["improlist"], IMPRO_K, IMPRO_CHI0)
         rng = np.random.default_rng(3)
         x = g["pos"].copy()
         k = len(x) // 8

@@ -101,7 +101,7 @@ def test_engine_list_holds_every_pair_inside_the_cutoff():
 
 
 def test_engine_bonded_energies_match_oracle():
-    """Bond, angle and torsion energies of the engine (rigid waters evaluated inside the SHAKE
+    """Bond, angle, torsion and improper energies of the engine (rigid waters evaluated inside the SHAKE
     kernel, the rest in the bonded kernels; torsions by the multiple-angle recurrence) against
     the oracle on the same positions, in a batch where the replicas' lists are concatenated."""
     host = _engine()
@@ -120,15 +120,48 @@ def test_engine_bonded_energies_match_oracle():
             tk = np.array([c[0] for c in S.DIHED_COEFF]); tm = np.array([c[1] for c in S.DIHED_COEFF], float)
             ts = np.array([c[2] for c in S.DIHED_COEFF], float)
             _, et = O.torsion_cos(host.torsion_list(s), s.pos, s.types, tk, tm, ts, d["box"], d["ibox"])
+            ik = np.array([c[0] for c in S.IMPRO_K_CHI0]); ic = np.array([c[1] for c in S.IMPRO_K_CHI0])
+            _, ei, _ = O.improper_harmonic(s.improlist, s.pos, s.types, ik, ic, d["box"], d["ibox"])
             say(f"[engine bonded] replica {k}: bond {E[k][4]:.6f} vs {eb:.6f}, angle {E[k][5]:.6f} vs {ea:.6f}, "
-                f"torsion {E[k][6]:.6f} vs {et:.6f}")
+                f"torsion {E[k][6]:.6f} vs {et:.6f}, improper {E[k][8]:.6f} vs {ei:.6f}")
             np.testing.assert_allclose(E[k][6], et, rtol=2e-5, atol=1e-4)
+            np.testing.assert_allclose(E[k][8], ei, rtol=2e-5, atol=1e-4)
             # the oracle follows the reference's real*4 coordinate differences; the engine
             # differences FP64 positions, so the sums agree to float32 rounding
             np.testing.assert_allclose(E[k][4], eb, rtol=2e-5, atol=1e-4)
             np.testing.assert_allclose(E[k][5], ea, rtol=2e-5, atol=1e-4)
     finally:
         e.close()
+
+
+def test_engine_improper_forces_match_oracle():
+    """The engine's improper forces, isolated as the difference of the initial forces of two
+    engines on the same positions with and without the improper list, against the oracle's
+    improper_harmonic_lammps14 on those positions."""
+    import dataclasses
+    host = _engine()
+    from oracle import oracle as O
+    s = S.make_system(12, 900, seed=43)
+    bare = dataclasses.replace(s, improlist=np.zeros((0, 5), np.int32))
+    f = []
+    for sys_ in (s, bare):
+        e = host.Engine([sys_], precision=host.PREC_FP64)
+        try:
+            e.init()
+            f.append(e.forces()[0])
+        finally:
+            e.close()
+    fi = f[0] - f[1]
+    d = s.derived()
+    ik = np.array([c[0] for c in S.IMPRO_K_CHI0]); ic = np.array([c[1] for c in S.IMPRO_K_CHI0])
+    ref, _, _ = O.improper_harmonic(s.improlist, s.pos, s.types, ik, ic, d["box"], d["ibox"])
+    rel = np.sqrt(((fi - ref) ** 2).sum() / (ref ** 2).sum())
+    say(f"[engine improper] {len(s.improlist)} impropers: force rel RMS {rel:.2e}")
+    assert (ref ** 2).sum() > 0
+    assert rel < 1e-5
+    # atoms in no improper get exactly nothing from it
+    idle = np.setdiff1d(np.arange(s.np), s.improlist[:, :4].ravel() - 1)
+    assert np.abs(fi[idle]).max() == 0.0
 
 
 def test_skin_changes_speed_not_trajectory():

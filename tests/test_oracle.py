@@ -416,7 +416,7 @@ def test_shake2_closed_form():
     assert st == 0
 
 
-@pytest.mark.parametrize("term", ["bond", "angle", "torsion"])
+@pytest.mark.parametrize("term", ["bond", "angle", "torsion", "improper"])
 def test_bonded_forces_are_minus_gradient(term):
     s = S.make_system(3, 0, seed=9, box=30.0)
     rng = np.random.default_rng(4)
@@ -430,6 +430,10 @@ def test_bonded_forces_are_minus_gradient(term):
         k = np.array([c[0] for c in S.ANGLE_COEFF])
         th = np.radians([c[1] for c in S.ANGLE_COEFF])
         fn = lambda x: O.angle_harmonic(s.anglelist, x, s.types, k, th, L, ib)
+    elif term == "improper":
+        il = s.improlist.copy(); il[1::2, 4] = 2
+        k = np.array([1.1, 0.6]); chi0 = np.array([0.2, 0.35])
+        fn = lambda x: O.improper_harmonic(il, x, s.types, k, chi0, L, ib)[:2]
     else:
         tors = np.array([(a, b, c, dd, 2) for (a, b, c, dd, _) in S.GLYCINE.diheds], np.int32)
         k = np.array([0.3, 0.7]); mult = np.array([2.0, 3.0]); sh = np.array([30.0, 75.0])
@@ -443,3 +447,64 @@ def test_bonded_forces_are_minus_gradient(term):
             dx = float(xp[atom, c]) - float(xm[atom, c])
             g = (fn(xp)[1] - fn(xm)[1]) / dx
             assert f[atom, c] == pytest.approx(-g, rel=2e-3, abs=2e-3)
+
+
+# ------------------------------------------------------------------ improper dihedrals
+def test_golden_improper(golden):
+    d = _derived(golden)
+    f, e, vir = O.improper_harmonic(golden["improlist"], golden["pos"], golden["types"],
+                                    golden["impro_k"], golden["impro_chi0"], d["box"], d["ibox"])
+    ref = golden["impro_f"].astype(np.float64)
+    assert np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()) < 1e-6
+    assert e == pytest.approx(float(golden["impro_e"]), rel=1e-12)
+    np.testing.assert_allclose(vir, golden["impro_vir"], rtol=1e-12, atol=1e-12)
+
+
+def _plane_angle(x, i1, i2, i3, i4):
+    """Angle between the planes (x1, x2, x3) and (x2, x3, x4) from cross products."""
+    b1, b2, b3 = x[i2] - x[i1], x[i3] - x[i2], x[i4] - x[i3]
+    n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+    return math.acos(np.clip(n1 @ n2 / (np.linalg.norm(n1) * np.linalg.norm(n2)), -1.0, 1.0))
+
+
+def test_improper_closed_form_and_invariants():
+    """E = K (chi - chi0)^2 with chi from plane normals; total force and torque vanish;
+    the reference virial equals sum_i r_i f_i of the unwrapped quadruplet."""
+    rng = np.random.default_rng(21)
+    L = np.float32(40.0)
+    ib = np.float32(1.0) / L
+    k = np.array([1.1, 0.45]); chi0 = np.array([0.0, 0.6])
+    for trial in range(20):
+        x = (np.float32(20.0) + rng.normal(0, 1.2, (4, 3))).astype(np.float32)
+        if trial % 3 == 0:                          # straddle the periodic boundary
+            x += np.float32(19.5)
+            x -= L * (x >= L).astype(np.float32)
+        t = 1 + trial % 2
+        il = np.array([[1, 2, 3, 4, t]], np.int32)
+        f, e, vir = O.improper_harmonic(il, x, np.ones(4, np.int32), k, chi0, L, ib)
+        # unwrapped geometry relative to atom 2
+        xu = x.astype(np.float64)
+        for i in (0, 2, 3):
+            dd = xu[i] - xu[1]
+            xu[i] = xu[1] + dd - float(L) * np.rint(dd / float(L))
+        chi = _plane_angle(xu, 0, 1, 2, 3)
+        assert e == pytest.approx(k[t - 1] * (chi - chi0[t - 1]) ** 2, rel=1e-9, abs=1e-12)
+        np.testing.assert_allclose(f.sum(axis=0), 0.0, atol=1e-9)
+        np.testing.assert_allclose(np.cross(xu, f).sum(axis=0), 0.0, atol=1e-8)
+        np.testing.assert_allclose(vir, (xu * f).sum(axis=0), rtol=1e-9, atol=1e-9)
+
+
+def test_improper_planar_cis_and_trans():
+    """Planar quadruplets: cis (chi = 0) is the minimum at chi0 = 0, with zero force; trans
+    (chi = pi) has E = K pi^2 (the energy uses acos(c), whose range is [0, pi])."""
+    il = np.array([[1, 2, 3, 4, 1]], np.int32)
+    L = np.float32(40.0)
+    cis = np.array([[10, 11, 10], [10, 10, 10], [11, 10, 10], [11, 11.2, 10]], np.float32)
+    trans = np.array([[10, 10, 10], [11, 10, 10], [11.5, 11, 10], [12.5, 11.2, 10]], np.float32)
+    f, e, _ = O.improper_harmonic(il, cis, np.ones(4, np.int32), np.array([2.0]), np.array([0.0]),
+                                  L, np.float32(1.0) / L)
+    assert e == pytest.approx(0.0, abs=1e-12)
+    np.testing.assert_allclose(f, 0.0, atol=1e-5)
+    _, e, _ = O.improper_harmonic(il, trans, np.ones(4, np.int32), np.array([2.0]), np.array([0.0]),
+                                  L, np.float32(1.0) / L)
+    assert e == pytest.approx(2.0 * math.pi ** 2, rel=1e-7)   # float32 input geometry
