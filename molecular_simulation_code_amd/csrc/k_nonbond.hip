@@ -30,6 +30,19 @@
 namespace sofia {
 namespace {
 
+// FP32 pair arithmetic shared by the mixed kernels.  Every multiply-add is an explicit
+// fmaf or kept apart, so nothing is left for the compiler to fuse: a pair rounds the same
+// way in every kernel that evaluates it, and their quantized force sums agree bit for bit.
+__device__ __forceinline__ float wrap_f(float d, float box, float ibox) {
+    return __builtin_fmaf(-box, __builtin_rintf(d * ibox), d);
+}
+__device__ __forceinline__ float norm2_f(float dx, float dy, float dz) {
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+__device__ __forceinline__ float cubic_f(const float4 &c, float t) {
+    return __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, c.w, c.z), c.y), c.x);
+}
+
 template <int L, int MODE, bool ENERGY>
 __global__ void __launch_bounds__(256)
 k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ q,
@@ -146,43 +159,37 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
             pn = pos4[jnn];
             qn = q[jnn];
             jnn = ulst[min(k + 2 * L, last)];
-            float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
-            dx -= box * __builtin_rintf(dx * ibox);
-            dy -= box * __builtin_rintf(dy * ibox);
-            dz -= box * __builtin_rintf(dz * ibox);
-            const float dr2 = dx * dx + dy * dy + dz * dz;
+            const float dx = wrap_f(xi.x - pj.x, box, ibox);
+            const float dy = wrap_f(xi.y - pj.y, box, ibox);
+            const float dz = wrap_f(xi.z - pj.z, box, ibox);
+            const float dr2 = norm2_f(dx, dy, dz);
             const float dri = __builtin_amdgcn_rsqf(dr2);
             const float r = dr2 * dri;
             const float dr2i = dri * dri;
             float dr6i = dr2i * dr2i * dr2i;
             if (dr2 > rcut2) dr6i = 0.0f;
             const float4 lj = ljrow[__float_as_int(pj.w)];
-            const float flj = dr6i * (lj.x * dr6i - lj.y);
-            float fc, ec;
+            const float flj = dr6i * __builtin_fmaf(lj.x, dr6i, -lj.y);
+            float force, ec;
             if (MODE == 1) {
-                // DSF Coulomb from the cubic table in r: force/r = C qi qj g(r)
+                // DSF Coulomb from the cubic table in r: force/r = C qi qj g(r) + F_LJ/r
                 float qq = qc * qj;
                 if (dr2 > cut_coulsq) qq = 0.0f;
                 const float u = (r - tr0) * tinv;
                 const int kb = min(max((int)u, 0), tn - 1);
                 const float t = u - (float)kb;
-                const float4 cg = s_tab[kb];
-                fc = qq * (cg.x + t * (cg.y + t * (cg.z + t * cg.w)));
-                if (ENERGY) {
-                    const float4 ce = s_tab[tn + kb];
-                    ec = qq * (ce.x + t * (ce.y + t * (ce.z + t * ce.w)));
-                }
+                force = __builtin_fmaf(qq, cubic_f(s_tab[kb], t), flj * dr2i);
+                if (ENERGY) ec = qq * cubic_f(s_tab[tn + kb], t);
             } else {
                 float pref = qc * qj * dri;
                 if (dr2 > cut_coulsq) pref = 0.0f;
                 const float erfcd = __builtin_amdgcn_exp2f(nal2e * dr2);
                 const float t = __builtin_amdgcn_rcpf(1.0f + pa * r);
                 const float erfcc = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * erfcd;
-                fc = pref * (erfcc + c2 * (alpha * r) * erfcd);
+                const float fc = pref * (erfcc + c2 * (alpha * r) * erfcd);   // F_C * r
                 ec = pref * erfcc;
+                force = (fc + flj) * dr2i;
             }
-            // mode 1: fc is already force/r; mode 2: fc is force*r
-            const float force = (MODE == 1) ? fc + flj * dr2i : (fc + flj) * dr2i;
             if (MODE == 1) {
                 // truncated quanta are integers; FP64 sums of integers below 2^53 are exact
                 const float fs = force * PAIR_QUANT;
@@ -195,7 +202,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
                 fzd += (double)(dz * force);
             }
             if (ENERGY) {
-                pot += (double)(dr6i * (dr6i * lj.z - lj.w));
+                pot += (double)(dr6i * __builtin_fmaf(dr6i, lj.z, -lj.w));
                 ecl += (double)ec;
                 vx += (double)(dx * dx * force);
                 vy += (double)(dy * dy * force);
@@ -298,29 +305,27 @@ k_nonbond_compact(int32_t n, const float4 *__restrict__ pos4, const float *__res
             const float4 e = ring[slot];
             const int tj = rtype[slot];
             const float dx = e.x, dy = e.y, dz = e.z;
-            const float dr2 = dx * dx + dy * dy + dz * dz;
+            const float dr2 = norm2_f(dx, dy, dz);
             const float dri = __builtin_amdgcn_rsqf(dr2);
             const float r = dr2 * dri;
             const float dr2i = dri * dri;
             float dr6i = dr2i * dr2i * dr2i;
             if (dr2 > rcut2) dr6i = 0.0f;
             const float4 lj = ljrow[tj];
-            const float flj = dr6i * (lj.x * dr6i - lj.y);
+            const float flj = dr6i * __builtin_fmaf(lj.x, dr6i, -lj.y);
             float qq = qc * e.w;
             if (dr2 > cut_coulsq) qq = 0.0f;
             const float u = (r - tr0) * tinv;
             const int kb = min(max((int)u, 0), tn - 1);
             const float t = u - (float)kb;
-            const float4 cg = s_tab[kb];
-            const float force = qq * (cg.x + t * (cg.y + t * (cg.z + t * cg.w))) + flj * dr2i;
+            const float force = __builtin_fmaf(qq, cubic_f(s_tab[kb], t), flj * dr2i);
             const float fs = force * PAIR_QUANT;
             fxq += (double)__builtin_truncf(dx * fs);
             fyq += (double)__builtin_truncf(dy * fs);
             fzq += (double)__builtin_truncf(dz * fs);
             if (ENERGY) {
-                const float4 ce = s_tab[tn + kb];
-                pot += (double)(dr6i * (dr6i * lj.z - lj.w));
-                ecl += (double)(qq * (ce.x + t * (ce.y + t * (ce.z + t * ce.w))));
+                pot += (double)(dr6i * __builtin_fmaf(dr6i, lj.z, -lj.w));
+                ecl += (double)(qq * cubic_f(s_tab[tn + kb], t));
                 vx += (double)(dx * dx * force);
                 vy += (double)(dy * dy * force);
                 vz += (double)(dz * dz * force);
@@ -347,11 +352,10 @@ k_nonbond_compact(int32_t n, const float4 *__restrict__ pos4, const float *__res
             p1 = pos4[j2];
             q1 = q[j2];
             j2 = j3;
-            float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
-            dx -= box * __builtin_rintf(dx * ibox);
-            dy -= box * __builtin_rintf(dy * ibox);
-            dz -= box * __builtin_rintf(dz * ibox);
-            const float dr2 = dx * dx + dy * dy + dz * dz;
+            const float dx = wrap_f(xi.x - pj.x, box, ibox);
+            const float dy = wrap_f(xi.y - pj.y, box, ibox);
+            const float dz = wrap_f(xi.z - pj.z, box, ibox);
+            const float dr2 = norm2_f(dx, dy, dz);
             const bool in = k < nn && dr2 <= cut2;
             const uint64_t m = __ballot(in);
             if (in) {
