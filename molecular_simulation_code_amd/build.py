@@ -23,8 +23,8 @@ COMMON = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-Wall",
 # The list build must evaluate the real*4 distance without FMA contraction to match the
 # reference bit for bit; the force kernels may contract.
 PER_FILE = {"k_neighbor.hip": ["-ffp-contract=off"]}
-SOURCES = ["sofia_api.hip", "k_neighbor.hip", "k_list_engine.hip", "k_nonbond.hip", "k_bonded.hip",
-           "k_integrate.hip", "k_cluster.hip"]
+SOURCES = ["sofia_api.hip", "k_neighbor.hip", "k_list_engine.hip", "k_nonbond.hip",
+           "k_dsf_mixed.hip", "k_bonded.hip", "k_integrate.hip", "k_cluster.hip"]
 
 
 def hipcc() -> str:

@@ -2,7 +2,7 @@
 //
 //   mode 0  DSF, FP64 pair arithmetic        lj_cut_coul_dsf_nonewton_NOMIC/_MIC
 //                                            (Source/mod_force_nonbond.f90:105-196 / 8-101)
-//   mode 1  DSF, FP32 pair arithmetic with per-pair int64 fixed-point force sums
+//   mode 1  DSF, FP32 pair arithmetic with exact per-atom force sums (k_dsf_mixed.hip)
 //   mode 2  Ewald real space, FP32 pair math lj_cut_coul_long_nonewton_NOMIC (:300-400)
 //
 // Pair expressions (Source/mod_force_nonbond.f90:153-183):
@@ -17,31 +17,14 @@
 // walks list entries s, s+L, ... so each group reads a contiguous run of the atom's list,
 // and the L partial sums are combined with cross-lane shuffles.  The loop is software
 // pipelined (the next entry's position and charge are in flight during this entry's math),
-// and blocks are dealt to the XCDs in runs so that an XCD's L2 holds one region of atoms.  Positions are the
-// cell-sorted float4 (x, y, z, type) array, so the j gathers of neighbouring atoms hit
-// the same cache lines.  No atomics on forces: every atom's sum is owned by one group.
-//
-// Mode 1 turns each pair force into an integer number of 2^-24 kcal/mol/A (truncation of
-// the float product) and sums those integers in FP64, exactly while a sum stays below 2^53
-// quanta (5e8 kcal/mol/A), so per-atom sums are independent of L, of list order and of how
-// replicas are batched.
+// and blocks are dealt to the XCDs in runs so that an XCD's L2 holds one region of atoms.
+// Positions are the cell-sorted float4 (x, y, z, type) array, so the j gathers of
+// neighbouring atoms hit the same cache lines.  No atomics on forces: every atom's sum is
+// owned by one group.
 #include "sofia_device.h"
 
 namespace sofia {
 namespace {
-
-// FP32 pair arithmetic shared by the mixed kernels.  Every multiply-add is an explicit
-// fmaf or kept apart, so nothing is left for the compiler to fuse: a pair rounds the same
-// way in every kernel that evaluates it, and their quantized force sums agree bit for bit.
-__device__ __forceinline__ float wrap_f(float d, float box, float ibox) {
-    return __builtin_fmaf(-box, __builtin_rintf(d * ibox), d);
-}
-__device__ __forceinline__ float norm2_f(float dx, float dy, float dz) {
-    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-}
-__device__ __forceinline__ float cubic_f(const float4 &c, float t) {
-    return __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, c.w, c.z), c.y), c.x);
-}
 
 template <int L, int MODE, bool ENERGY>
 __global__ void __launch_bounds__(256)
@@ -50,17 +33,12 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
           const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh, int32_t na,
           const float4 *__restrict__ ljtab, const float4 *__restrict__ tab, DevParams P,
           double *__restrict__ fnb, float *__restrict__ ff32, long long *__restrict__ eacc) {
-    // LDS: the LJ table, then (mode 1) the DSF Coulomb force table and, with energies, the
-    // energy table -- sized at launch
+    // LDS: the LJ table, sized at launch
+    static_assert(MODE == 0 || MODE == 2, "mode 1 is k_dsf_mixed");
     extern __shared__ float4 s_dyn[];
     float4 *s_lj = s_dyn;
     const int nt2 = P.ntypes * P.ntypes;
-    float4 *s_tab = s_dyn + nt2;
     for (int t = threadIdx.x; t < nt2; t += blockDim.x) s_lj[t] = ljtab[t];
-    if (MODE == 1) {
-        const int ntab = ENERGY ? 2 * P.tab_n : P.tab_n;
-        for (int t = threadIdx.x; t < ntab; t += blockDim.x) s_tab[t] = tab[t];
-    }
     __syncthreads();
 
     const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -77,8 +55,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
     const int32_t *lst = nlist + (size_t)ii * na;
     const float4 *ljrow = s_lj + ti * P.ntypes - 1;   // index by jtype (1-based)
 
-    double fxd = 0.0, fyd = 0.0, fzd = 0.0;           // modes 0, 2
-    double fxq = 0.0, fyq = 0.0, fzq = 0.0;           // mode 1: sums of integer quanta
+    double fxd = 0.0, fyd = 0.0, fzd = 0.0;
     double pot = 0.0, ecl = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
 
     if (MODE == 0) {
@@ -136,14 +113,12 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
     } else {
         const float rcut2 = (float)P.rcut2, cut_coulsq = (float)P.cut_coulsq;
         const float qc = P.coulpre * q[ii];
-        const float alpha = (float)(MODE == 1 ? P.alpha : P.gewald);
+        const float alpha = (float)P.gewald;
         const float nal2e = -alpha * alpha * 1.4426950408889634f;   // exp(-a^2 r^2) = 2^(..)
         const float pa = (float)P.ewald_p * alpha;
-        const float c2 = (float)(MODE == 1 ? (2.0 * P.alpha) * P.my_pis_inv : P.ewald_f);
+        const float c2 = (float)P.ewald_f;
         const float A1 = (float)P.a1, A2 = (float)P.a2, A3 = (float)P.a3, A4 = (float)P.a4,
                     A5 = (float)P.a5;
-        const float tr0 = P.tab_r0, tinv = P.tab_inv_h;
-        const int tn = P.tab_n;
         // software pipeline: an entry's data is loaded one iteration ahead; loads past the
         // end of the row re-read its last entry (clamped index), so no branch is needed
         const int last = max(nn - 1, 0);
@@ -159,51 +134,31 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
             pn = pos4[jnn];
             qn = q[jnn];
             jnn = ulst[min(k + 2 * L, last)];
-            const float dx = wrap_f(xi.x - pj.x, box, ibox);
-            const float dy = wrap_f(xi.y - pj.y, box, ibox);
-            const float dz = wrap_f(xi.z - pj.z, box, ibox);
-            const float dr2 = norm2_f(dx, dy, dz);
+            float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
+            dx -= box * __builtin_rintf(dx * ibox);
+            dy -= box * __builtin_rintf(dy * ibox);
+            dz -= box * __builtin_rintf(dz * ibox);
+            const float dr2 = dx * dx + dy * dy + dz * dz;
             const float dri = __builtin_amdgcn_rsqf(dr2);
             const float r = dr2 * dri;
             const float dr2i = dri * dri;
             float dr6i = dr2i * dr2i * dr2i;
             if (dr2 > rcut2) dr6i = 0.0f;
             const float4 lj = ljrow[__float_as_int(pj.w)];
-            const float flj = dr6i * __builtin_fmaf(lj.x, dr6i, -lj.y);
-            float force, ec;
-            if (MODE == 1) {
-                // DSF Coulomb from the cubic table in r: force/r = C qi qj g(r) + F_LJ/r
-                float qq = qc * qj;
-                if (dr2 > cut_coulsq) qq = 0.0f;
-                const float u = (r - tr0) * tinv;
-                const int kb = min(max((int)u, 0), tn - 1);
-                const float t = u - (float)kb;
-                force = __builtin_fmaf(qq, cubic_f(s_tab[kb], t), flj * dr2i);
-                if (ENERGY) ec = qq * cubic_f(s_tab[tn + kb], t);
-            } else {
-                float pref = qc * qj * dri;
-                if (dr2 > cut_coulsq) pref = 0.0f;
-                const float erfcd = __builtin_amdgcn_exp2f(nal2e * dr2);
-                const float t = __builtin_amdgcn_rcpf(1.0f + pa * r);
-                const float erfcc = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * erfcd;
-                const float fc = pref * (erfcc + c2 * (alpha * r) * erfcd);   // F_C * r
-                ec = pref * erfcc;
-                force = (fc + flj) * dr2i;
-            }
-            if (MODE == 1) {
-                // truncated quanta are integers; FP64 sums of integers below 2^53 are exact
-                const float fs = force * PAIR_QUANT;
-                fxq += (double)__builtin_truncf(dx * fs);
-                fyq += (double)__builtin_truncf(dy * fs);
-                fzq += (double)__builtin_truncf(dz * fs);
-            } else {
-                fxd += (double)(dx * force);
-                fyd += (double)(dy * force);
-                fzd += (double)(dz * force);
-            }
+            const float flj = dr6i * (lj.x * dr6i - lj.y);
+            float pref = qc * qj * dri;
+            if (dr2 > cut_coulsq) pref = 0.0f;
+            const float erfcd = __builtin_amdgcn_exp2f(nal2e * dr2);
+            const float t = __builtin_amdgcn_rcpf(1.0f + pa * r);
+            const float erfcc = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * erfcd;
+            const float fc = pref * (erfcc + c2 * (alpha * r) * erfcd);   // F_C * r
+            const float force = (fc + flj) * dr2i;
+            fxd += (double)(dx * force);
+            fyd += (double)(dy * force);
+            fzd += (double)(dz * force);
             if (ENERGY) {
-                pot += (double)(dr6i * __builtin_fmaf(dr6i, lj.z, -lj.w));
-                ecl += (double)ec;
+                pot += (double)(dr6i * (dr6i * lj.z - lj.w));
+                ecl += (double)(pref * erfcc);
                 vx += (double)(dx * dx * force);
                 vy += (double)(dy * dy * force);
                 vz += (double)(dz * dz * force);
@@ -211,17 +166,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
         }
     }
 
-    double Fx, Fy, Fz;
-    if (MODE == 1) {
-        const double qinv = 1.0 / (double)PAIR_QUANT;
-        Fx = group_sum<L>(fxq) * qinv;
-        Fy = group_sum<L>(fyq) * qinv;
-        Fz = group_sum<L>(fzq) * qinv;
-    } else {
-        Fx = group_sum<L>(fxd);
-        Fy = group_sum<L>(fyd);
-        Fz = group_sum<L>(fzd);
-    }
+    const double Fx = group_sum<L>(fxd), Fy = group_sum<L>(fyd), Fz = group_sum<L>(fzd);
     const bool owner = live && sub == 0;
     if (owner) {
         fnb[i] = Fx;
@@ -247,181 +192,13 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
     }
 }
 
-// Mixed DSF kernel with in-cutoff compaction, one atom per wavefront.
-//
-// Phase A walks the atom's list 64 entries at a time (loads software pipelined one chunk
-// ahead), forms the minimum-image separation and appends the entries inside the cutoff to
-// a per-wavefront ring in LDS (dx, dy, dz, q_j and type_j).  Phase B takes 64 queued
-// entries at a time and runs the force math with every lane busy, so no lane computes a
-// pair beyond the cutoff (about 42 % of a list with a 2 A skin).  Each pair is quantised
-// as in k_nonbond<L, 1>, so per-atom sums are exact and independent of the queue order.
-constexpr int QRING = 128;
-
-template <bool ENERGY>
-__global__ void __launch_bounds__(256)
-k_nonbond_compact(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ q,
-                  const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
-                  const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh,
-                  int32_t na, const float4 *__restrict__ ljtab, const float4 *__restrict__ tab,
-                  DevParams P, double *__restrict__ fnb, float *__restrict__ ff32,
-                  long long *__restrict__ eacc) {
-    extern __shared__ float4 s_dyn[];
-    const int nt2 = P.ntypes * P.ntypes;
-    const int tn = P.tab_n;
-    float4 *s_lj = s_dyn;
-    float4 *s_tab = s_dyn + nt2;
-    float4 *s_ring = s_tab + (ENERGY ? 2 : 1) * tn;
-    int *s_rtype = (int *)(s_ring + 4 * QRING);
-    for (int t = threadIdx.x; t < nt2; t += blockDim.x) s_lj[t] = ljtab[t];
-    for (int t = threadIdx.x; t < (ENERGY ? 2 : 1) * tn; t += blockDim.x) s_tab[t] = tab[t];
-    __syncthreads();
-
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
-    if (i >= n) return;   // whole wavefront; no barrier follows
-    float4 *ring = s_ring + wave * QRING;
-    int *rtype = s_rtype + wave * QRING;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-
-    const float4 xi = pos4[i];
-    const int ti = __float_as_int(xi.w) - 1;
-    const RepInfo &R = reps[rep[i]];
-    const float box = R.box, ibox = R.ibox;
-    const int nn = min(numneigh[i], na);
-    const int32_t *lst = nlist + (size_t)i * na;
-    const float4 *ljrow = s_lj + ti * P.ntypes - 1;
-    const float rcut2 = (float)P.rcut2, cut_coulsq = (float)P.cut_coulsq;
-    const float cut2 = fmaxf(rcut2, cut_coulsq);
-    const float qc = P.coulpre * q[i];
-    const float tr0 = P.tab_r0, tinv = P.tab_inv_h;
-
-    double fxq = 0.0, fyq = 0.0, fzq = 0.0;   // exact sums of integer quanta
-    double pot = 0.0, ecl = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
-    int head = 0, tail = 0;
-
-    auto consume = [&](int cnt) {
-        if (lane < cnt) {
-            const int slot = (head + lane) & (QRING - 1);
-            const float4 e = ring[slot];
-            const int tj = rtype[slot];
-            const float dx = e.x, dy = e.y, dz = e.z;
-            const float dr2 = norm2_f(dx, dy, dz);
-            const float dri = __builtin_amdgcn_rsqf(dr2);
-            const float r = dr2 * dri;
-            const float dr2i = dri * dri;
-            float dr6i = dr2i * dr2i * dr2i;
-            if (dr2 > rcut2) dr6i = 0.0f;
-            const float4 lj = ljrow[tj];
-            const float flj = dr6i * __builtin_fmaf(lj.x, dr6i, -lj.y);
-            float qq = qc * e.w;
-            if (dr2 > cut_coulsq) qq = 0.0f;
-            const float u = (r - tr0) * tinv;
-            const int kb = min(max((int)u, 0), tn - 1);
-            const float t = u - (float)kb;
-            const float force = __builtin_fmaf(qq, cubic_f(s_tab[kb], t), flj * dr2i);
-            const float fs = force * PAIR_QUANT;
-            fxq += (double)__builtin_truncf(dx * fs);
-            fyq += (double)__builtin_truncf(dy * fs);
-            fzq += (double)__builtin_truncf(dz * fs);
-            if (ENERGY) {
-                pot += (double)(dr6i * __builtin_fmaf(dr6i, lj.z, -lj.w));
-                ecl += (double)(qq * cubic_f(s_tab[tn + kb], t));
-                vx += (double)(dx * dx * force);
-                vy += (double)(dy * dy * force);
-                vz += (double)(dz * dz * force);
-            }
-        }
-        head += cnt;
-    };
-
-    if (nn > 0) {
-        // two-deep pipeline: chunk c uses positions gathered two chunks earlier; the index
-        // load is issued before the gathers so waiting for it leaves the gathers in flight
-        const int last = nn - 1;
-        const int j0 = lst[min(lane, last)], j1 = lst[min(64 + lane, last)];
-        int j2 = lst[min(128 + lane, last)];
-        float4 p0 = pos4[j0], p1 = pos4[j1];
-        float q0 = q[j0], q1 = q[j1];
-        for (int base = 0; base < nn; base += 64) {
-            const int k = base + lane;
-            const float4 pj = p0;
-            const float qj = q0;
-            p0 = p1;
-            q0 = q1;
-            const int j3 = lst[min(base + 192 + lane, last)];
-            p1 = pos4[j2];
-            q1 = q[j2];
-            j2 = j3;
-            const float dx = wrap_f(xi.x - pj.x, box, ibox);
-            const float dy = wrap_f(xi.y - pj.y, box, ibox);
-            const float dz = wrap_f(xi.z - pj.z, box, ibox);
-            const float dr2 = norm2_f(dx, dy, dz);
-            const bool in = k < nn && dr2 <= cut2;
-            const uint64_t m = __ballot(in);
-            if (in) {
-                const int slot = (tail + __popcll(m & lt_mask)) & (QRING - 1);
-                ring[slot] = make_float4(dx, dy, dz, qj);
-                rtype[slot] = __float_as_int(pj.w);
-            }
-            tail += __popcll(m);
-            __builtin_amdgcn_wave_barrier();
-            if (tail - head >= 64) consume(64);
-            __builtin_amdgcn_wave_barrier();
-        }
-        if (tail > head) consume(tail - head);
-    }
-
-    const double qinv = 1.0 / (double)PAIR_QUANT;
-    const double Fx = group_sum<64>(fxq) * qinv, Fy = group_sum<64>(fyq) * qinv,
-                 Fz = group_sum<64>(fzq) * qinv;
-    const bool owner = lane == 0;
-    if (owner) {
-        fnb[i] = Fx;
-        fnb[(size_t)n + i] = Fy;
-        fnb[2 * (size_t)n + i] = Fz;
-        if (ff32) {
-            ff32[3 * (size_t)i + 0] = (float)Fx;
-            ff32[3 * (size_t)i + 1] = (float)Fy;
-            ff32[3 * (size_t)i + 2] = (float)Fz;
-        }
-    }
-    if (ENERGY) {
-        const int r = rep[i];
-        pot = group_sum<64>(pot);
-        ecl = group_sum<64>(ecl);
-        vx = group_sum<64>(vx);
-        vy = group_sum<64>(vy);
-        vz = group_sum<64>(vz);
-        wave_add(to_efix(0.5 * pot), owner, r, eacc, E_LJ);
-        wave_add(to_efix(0.5 * ecl), owner, r, eacc, E_COUL);
-        wave_add(to_efix(0.5 * vx), owner, r, eacc, V_X);
-        wave_add(to_efix(0.5 * vy), owner, r, eacc, V_Y);
-        wave_add(to_efix(0.5 * vz), owner, r, eacc, V_Z);
-    }
-}
-
-void launch_compact(sofia_ctx *c, bool energies, float *ff32) {
-    const int grid = (c->N + 3) / 4;
-    const int nt2 = c->dp.ntypes * c->dp.ntypes;
-    const size_t lds = sizeof(float4) * (nt2 + (energies ? 2 : 1) * c->dp.tab_n + 4 * QRING) +
-                       sizeof(int) * 4 * QRING;
-    if (energies)
-        hipLaunchKernelGGL(k_nonbond_compact<true>, dim3(grid), dim3(256), lds, c->stream, c->N,
-                           c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
-                           c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
-    else
-        hipLaunchKernelGGL(k_nonbond_compact<false>, dim3(grid), dim3(256), lds, c->stream, c->N,
-                           c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
-                           c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
-}
-
 template <int L, int MODE>
 void launch_t(sofia_ctx *c, bool energies, float *ff32) {
     const int block = c->block;
     const long long threads = (long long)c->N * L;
     const int grid = (int)((threads + block - 1) / block);
     const int nt2 = c->dp.ntypes * c->dp.ntypes;
-    const size_t lds = sizeof(float4) * (nt2 + (MODE == 1 ? (energies ? 2 : 1) * c->dp.tab_n : 0));
+    const size_t lds = sizeof(float4) * nt2;
     if (energies)
         hipLaunchKernelGGL((k_nonbond<L, MODE, true>), dim3(grid), dim3(block), lds, c->stream, c->N,
                            c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
@@ -434,10 +211,6 @@ void launch_t(sofia_ctx *c, bool energies, float *ff32) {
 
 template <int MODE>
 void launch_mode(sofia_ctx *c, bool energies, float *ff32) {
-    if (MODE == 1 && c->lanes == 0) {
-        launch_compact(c, energies, ff32);
-        return;
-    }
     switch (c->lanes) {
         case 4: launch_t<4, MODE>(c, energies, ff32); break;
         case 8: launch_t<8, MODE>(c, energies, ff32); break;
@@ -449,14 +222,17 @@ void launch_mode(sofia_ctx *c, bool energies, float *ff32) {
 
 }  // namespace
 
+// Drop-in launches (positions from the float copy the caller handed over).
 void launch_nonbond_out(sofia_ctx *c, int mode, bool energies, float *ff32) {
     if (mode == 0) launch_mode<0>(c, energies, ff32);
-    else if (mode == 1) launch_mode<1>(c, energies, ff32);
+    else if (mode == 1) launch_dsf_mixed(c, energies, ff32, false);
     else launch_mode<2>(c, energies, ff32);
 }
 
+// Engine launches: the mixed kernel takes its fixed-point positions from the FP64 state.
 void launch_nonbond(sofia_ctx *c, int mode, bool energies) {
-    launch_nonbond_out(c, mode, energies, nullptr);
+    if (mode == 1) launch_dsf_mixed(c, energies, nullptr, c->engine_ready);
+    else launch_nonbond_out(c, mode, energies, nullptr);
 }
 
 // Number of list entries inside the force cutoff (the algorithmic pair count of a launch).

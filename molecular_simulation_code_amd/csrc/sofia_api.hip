@@ -312,14 +312,16 @@ int build_dsf_table(sofia_ctx *c) {
     };
     auto u = [&](double r) { return erfc_as(r) / r - p.e_shift - r * p.f_shift; };
     auto slope = [&](auto f, double r) { const double d = 1e-5 * r; return (f(r + d) - f(r - d)) / (2.0 * d); };
-    std::vector<float4> tab((size_t)2 * nb);
+    // force pieces [0, nb], energy pieces [nb + 1, 2 nb + 1]; piece nb of each is all zero
+    // and takes r >= cut_coul
+    std::vector<float4> tab((size_t)2 * (nb + 1), make_float4(0.f, 0.f, 0.f, 0.f));
     for (int which = 0; which < 2; ++which)
         for (int k = 0; k < nb; ++k) {
             const double ra = r0 + k * h, rb = ra + h;
             double fa, fb, da, db;
             if (which == 0) { fa = g(ra); fb = g(rb); da = slope(g, ra) * h; db = slope(g, rb) * h; }
             else { fa = u(ra); fb = u(rb); da = slope(u, ra) * h; db = slope(u, rb) * h; }
-            tab[(size_t)which * nb + k] = make_float4((float)fa, (float)da, (float)(3.0 * (fb - fa) - 2.0 * da - db),
+            tab[(size_t)which * (nb + 1) + k] = make_float4((float)fa, (float)da, (float)(3.0 * (fb - fa) - 2.0 * da - db),
                                                       (float)(2.0 * (fa - fb) + da + db));
         }
     TRY(dalloc(&c->dsf_tab, tab.size()));
@@ -369,6 +371,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     TRY(dalloc(&c->dr32, (size_t)3 * n));
     TRY(dalloc(&c->drd, (size_t)3 * n));
     TRY(dalloc(&c->fnb, (size_t)3 * n));
+    TRY(dalloc(&c->pfix, (size_t)n));
     TRY(dalloc(&c->fix, (size_t)3 * n));
     TRY(dalloc(&c->fscratch, (size_t)3 * n));
     TRY(dalloc(&c->stage32, (size_t)3 * n));
@@ -506,7 +509,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep); dfree(c->acell);
     dfree(c->aslot); dfree(c->perm); dfree(c->tt); dfree(c->atombin); dfree(c->sorted_ids);
     dfree(c->nlist); dfree(c->numneigh); dfree(c->maxcnt); dfree(c->rebuild); dfree(c->nbuilds);
-    dfree(c->shake_err); dfree(c->shake_iters); dfree(c->water); dfree(c->dsf_tab); dfree(c->ebond); dfree(c->eangle); dfree(c->bondlist); dfree(c->anglelist);
+    dfree(c->shake_err); dfree(c->shake_iters); dfree(c->water); dfree(c->dsf_tab); dfree(c->pfix); dfree(c->ebond); dfree(c->eangle); dfree(c->bondlist); dfree(c->anglelist);
     dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms);
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);

@@ -52,7 +52,7 @@ struct DevParams {
     int32_t natom;
     int32_t neigh_alloc;
     float tab_r0, tab_inv_h;     // DSF Coulomb table of the mixed kernel: r0 + k / inv_h
-    int32_t tab_n;               // bins (force table, then energy table)
+    int32_t tab_n;               // bins per table; each table has one more, all-zero bin
 };
 
 constexpr int DSF_TAB_BINS = 1024;
@@ -137,6 +137,7 @@ struct sofia_ctx {
     double *tors_cs = nullptr;  // per torsion type: cos(shift), sin(shift)
     double *impro_k = nullptr, *impro_chi0 = nullptr;   // per improper type
     float4 *dsf_tab = nullptr;  // cubic pieces of the DSF Coulomb force / energy (mixed kernel)
+    int4 *pfix = nullptr;       // mixed kernel positions: x 2^32 / box mod 2^32, type (per launch)
     int32_t nbt = 0, nat = 0, ntt = 0;
     bool have_ff = false;
 
@@ -186,6 +187,7 @@ void launch_count_pairs(sofia_ctx *c, unsigned long long *out);
 void launch_correct(sofia_ctx *c, int which, long long *fout);
 void launch_pair14(sofia_ctx *c, long long *fout);
 void launch_bonded_engine(sofia_ctx *c);
+void launch_dsf_mixed(sofia_ctx *c, bool energies, float *ff32, bool from_xd);
 
 // integration (k_integrate.hip)
 void launch_zero_forces(sofia_ctx *c, long long *f, bool energies);
