@@ -2,12 +2,15 @@
 // lj_cut_coul_dsf_nonewton_NOMIC (Source/mod_force_nonbond.f90:105-196) with FP32 pair
 // arithmetic and exact per-atom force sums.
 //
-// Positions.  Before a launch, k_fixpos writes each atom as three unsigned 32-bit fractions
-// of its replica's box, X = round(x 2^32 / box) mod 2^32.  The difference of two such words
-// taken in 32-bit integers is already the minimum image (the box maps to 2^32 exactly), so
-// the pair loop has no rint/fma wrap; the integer difference is converted to float and
-// scaled by box 2^-32.  Fixed point also removes the cancellation of FP32 coordinates near
-// the box edge: separations carry full FP32 relative precision.
+// Positions.  Before a launch, k_fixpos writes each atom as one 16-byte record: three
+// unsigned 32-bit fractions of its replica's box, X = round(x 2^32 / box) mod 2^32, with the
+// atom type in the low 6 bits of X (the x resolution becomes 2^-26 box, 3e-6 A at 217 A,
+// finer than a float32 coordinate there), and the charge.  The difference of two such
+// words taken in 32-bit integers is already the minimum image (the box maps to 2^32
+// exactly), so the pair loop has no rint/fma wrap; the integer difference is converted to
+// float and scaled by box 2^-32.  Fixed point also removes the cancellation of FP32
+// coordinates near the box edge: separations carry full FP32 relative precision.  One
+// gather per list entry brings position, type and charge.
 //
 // Pair math (force over r):
 //   dr6i = (1/r^2)^3, 0 beyond rcut2;  F_LJ/r = dr6i (lj1 dr6i - lj2) / r^2
@@ -88,11 +91,13 @@ __device__ __forceinline__ PairOut pair_dsf(float dr2, float qq, const float4 &l
     return o;
 }
 
-// x -> round(x 2^32 / box) mod 2^32 per replica; w keeps the atom type.  From the FP64
-// engine state (xd != nullptr) or from the float positions.
+constexpr uint32_t TYPE_BITS = 0x3fu;   // atom type in the low bits of X (types 1..63)
+
+// x -> round(x 2^32 / box) mod 2^32 per replica, type in the low bits of X, charge in w.
+// From the FP64 engine state (xd != nullptr) or from the float positions.
 __global__ void k_fixpos(int32_t n, const double *__restrict__ xd, const float4 *__restrict__ pos4,
-                         const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
-                         int4 *__restrict__ pf) {
+                         const float *__restrict__ q, const int32_t *__restrict__ rep,
+                         const RepInfo *__restrict__ reps, int4 *__restrict__ pf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = pos4[i];
@@ -103,16 +108,17 @@ __global__ void k_fixpos(int32_t n, const double *__restrict__ xd, const float4 
     } else {
         x = p.x; y = p.y; z = p.z;
     }
-    pf[i] = make_int4((int)(uint32_t)(unsigned long long)__double2ll_rn(x * s),
+    const uint32_t fx = (uint32_t)(unsigned long long)__double2ll_rn(x * s);
+    pf[i] = make_int4((int)((fx & ~TYPE_BITS) | ((uint32_t)__float_as_int(p.w) & TYPE_BITS)),
                       (int)(uint32_t)(unsigned long long)__double2ll_rn(y * s),
-                      (int)(uint32_t)(unsigned long long)__double2ll_rn(z * s), __float_as_int(p.w));
+                      (int)(uint32_t)(unsigned long long)__double2ll_rn(z * s), __float_as_int(q[i]));
 }
 
 // L lanes per atom; lane s of a group walks list entries s, s+L, ... with the loads of the
 // next entry in flight during this entry's math.  Blocks are dealt to the XCDs in runs.
 template <int L, bool ENERGY>
 __global__ void __launch_bounds__(256)
-k_dsf_mixed(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ q,
+k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
             const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh, int32_t na,
             const float4 *__restrict__ ljtab, const float4 *__restrict__ tab, DevParams P,
@@ -136,17 +142,17 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ q,
     const int iw = __builtin_amdgcn_readfirstlane(ii);
     const auto lrs = rsrc(nlist + (size_t)iw * na, clamp_bytes((size_t)(n - iw) * na * 4));
     const auto prs = rsrc(pf, clamp_bytes((size_t)n * 16));
-    const auto qrs = rsrc(q, clamp_bytes((size_t)n * 4));
 
     const int4 pi = pf[ii];
-    const float4 *ljrow = s_lj + (pi.w - 1) * P.ntypes - 1;   // index by jtype (1-based)
+    const float4 *ljrow = s_lj + ((int)((uint32_t)pi.x & TYPE_BITS) - 1) * P.ntypes - 1;   // by jtype
+    const int xi = (int)((uint32_t)pi.x & ~TYPE_BITS);
     const int r = rep[ii];
     const float sbox = reps[r].box * TWO_M32;                 // box 2^-32 (exact)
     const float s2 = sbox * sbox;
     const float fq = sbox * (float)PAIR_QUANT;                // quanta per (raw dx * F/r)
     const int nn = live ? min(numneigh[ii], na) : 0;
     const float rcut2 = (float)P.rcut2;
-    const float qc = P.coulpre * q[ii];
+    const float qc = P.coulpre * __int_as_float(pi.w);
     const float tinv = P.tab_inv_h, u0 = -P.tab_r0 * P.tab_inv_h;
     const float4 *s_etab = s_tab + tn + 1;
 
@@ -163,7 +169,6 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ q,
     const uint32_t step = 4u * L;
     uint32_t off = (uint32_t)(ii - iw) * (uint32_t)na * 4u + (uint32_t)sub * 4u;
     int4 PJ[4];
-    float QJ[4];
     uint32_t JJ[4];
     // prologue in the loop's own load order, so the wait counts at the loop head are the
     // steady-state ones
@@ -173,7 +178,6 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ q,
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         PJ[u] = ld128(prs, JJ[u] * 16u);
-        QJ[u] = __uint_as_float(ld32(qrs, JJ[u] * 4u));
         JJ[u] = ld32(lrs, off + (u + 4) * step);
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -183,11 +187,12 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ q,
         for (int u = 0; u < 4; ++u) {
             const bool valid = k + u * L < nn;
             const int4 pj = PJ[u];
-            const float qj = QJ[u];
-            const float dx = sep_f(pi.x, pj.x), dy = sep_f(pi.y, pj.y), dz = sep_f(pi.z, pj.z);
+            const uint32_t tj = (uint32_t)pj.x & TYPE_BITS;
+            const float dx = sep_f(xi, (int)((uint32_t)pj.x & ~TYPE_BITS));
+            const float dy = sep_f(pi.y, pj.y), dz = sep_f(pi.z, pj.z);
             const float dr2 = norm2_f(dx, dy, dz) * s2;
-            const float4 lj = ljrow[valid ? pj.w : 1];
-            const float qq = qc * qj;
+            const float4 lj = ljrow[valid ? tj : 1u];
+            const float qq = qc * __int_as_float(pj.w);
             const PairOut o = pair_dsf(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2);
             const float fs = valid ? o.force * fq : 0.0f;
             fxq += (double)__builtin_truncf(dx * fs);
@@ -204,7 +209,6 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ q,
                 }
             }
             PJ[u] = ld128(prs, JJ[u] * 16u);
-            QJ[u] = __uint_as_float(ld32(qrs, JJ[u] * 4u));
             JJ[u] = ld32(lrs, off + u * step);
             // VALU, SALU and LDS work may cross; memory loads stay in slot order
             __builtin_amdgcn_sched_barrier(0x0086);
@@ -253,7 +257,7 @@ constexpr int QRING = 128;
 
 template <bool ENERGY>
 __global__ void __launch_bounds__(256)
-k_dsf_compact(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ q,
+k_dsf_compact(int32_t n, const int4 *__restrict__ pf,
               const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
               const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh, int32_t na,
               const float4 *__restrict__ ljtab, const float4 *__restrict__ tab, DevParams P,
@@ -280,9 +284,9 @@ k_dsf_compact(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ 
 
     const auto lrs = rsrc(nlist + (size_t)i * na, clamp_bytes((size_t)(n - i) * na * 4));
     const auto prs = rsrc(pf, clamp_bytes((size_t)n * 16));
-    const auto qrs = rsrc(q, clamp_bytes((size_t)n * 4));
     const int4 pi = pf[i];
-    const float4 *ljrow = s_lj + (pi.w - 1) * P.ntypes - 1;
+    const float4 *ljrow = s_lj + ((int)((uint32_t)pi.x & TYPE_BITS) - 1) * P.ntypes - 1;
+    const int xi = (int)((uint32_t)pi.x & ~TYPE_BITS);
     const int r = rep[i];
     const float sbox = reps[r].box * TWO_M32;
     const float s2 = sbox * sbox;
@@ -290,7 +294,7 @@ k_dsf_compact(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ 
     const int nn = min(numneigh[i], na);
     const float rcut2 = (float)P.rcut2;
     const float keep2 = fmaxf(rcut2, (float)P.cut_coulsq) * 1.0001f;
-    const float qc = P.coulpre * q[i];
+    const float qc = P.coulpre * __int_as_float(pi.w);
     const float tinv = P.tab_inv_h, u0 = -P.tab_r0 * P.tab_inv_h;
     const float4 *s_etab = s_tab + tn + 1;
 
@@ -330,24 +334,21 @@ k_dsf_compact(int32_t n, const int4 *__restrict__ pf, const float *__restrict__ 
         const uint32_t j0 = ld32(lrs, l4), j1 = ld32(lrs, 256u + l4);
         uint32_t j2 = ld32(lrs, 512u + l4);
         int4 p0 = ld128(prs, j0 * 16u), p1 = ld128(prs, j1 * 16u);
-        float q0 = __uint_as_float(ld32(qrs, j0 * 4u)), q1 = __uint_as_float(ld32(qrs, j1 * 4u));
         for (int base = 0; base < nn; base += 64) {
             const int k = base + lane;
             const int4 pj = p0;
-            const float qj = q0;
             p0 = p1;
-            q0 = q1;
             const uint32_t j3 = ld32(lrs, (uint32_t)(base + 192) * 4u + l4);
             p1 = ld128(prs, j2 * 16u);
-            q1 = __uint_as_float(ld32(qrs, j2 * 4u));
             j2 = j3;
-            const float dx = sep_f(pi.x, pj.x), dy = sep_f(pi.y, pj.y), dz = sep_f(pi.z, pj.z);
+            const float dx = sep_f(xi, (int)((uint32_t)pj.x & ~TYPE_BITS));
+            const float dy = sep_f(pi.y, pj.y), dz = sep_f(pi.z, pj.z);
             const bool in = k < nn && norm2_f(dx, dy, dz) * s2 <= keep2;
             const uint64_t m = __ballot(in);
             if (in) {
                 const int slot = (tail + __popcll(m & lt_mask)) & (QRING - 1);
-                ring[slot] = make_float4(dx, dy, dz, qj);
-                rtype[slot] = pj.w;
+                ring[slot] = make_float4(dx, dy, dz, __int_as_float(pj.w));
+                rtype[slot] = (int)((uint32_t)pj.x & TYPE_BITS);
             }
             tail += __popcll(m);
             __builtin_amdgcn_wave_barrier();
@@ -394,11 +395,11 @@ void launch_lanes(sofia_ctx *c, bool energies, float *ff32) {
     const size_t lds = sizeof(float4) * (nt2 + (energies ? 2 : 1) * (c->dp.tab_n + 1));
     if (energies)
         hipLaunchKernelGGL((k_dsf_mixed<L, true>), dim3(grid), dim3(block), lds, c->stream, c->N,
-                           c->pfix, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
+                           c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh,
                            c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
     else
         hipLaunchKernelGGL((k_dsf_mixed<L, false>), dim3(grid), dim3(block), lds, c->stream, c->N,
-                           c->pfix, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
+                           c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh,
                            c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
 }
 
@@ -409,11 +410,11 @@ void launch_compact(sofia_ctx *c, bool energies, float *ff32) {
                        sizeof(int) * 4 * QRING;
     if (energies)
         hipLaunchKernelGGL(k_dsf_compact<true>, dim3(grid), dim3(256), lds, c->stream, c->N,
-                           c->pfix, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
+                           c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh,
                            c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
     else
         hipLaunchKernelGGL(k_dsf_compact<false>, dim3(grid), dim3(256), lds, c->stream, c->N,
-                           c->pfix, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
+                           c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh,
                            c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
 }
 
@@ -421,7 +422,7 @@ void launch_compact(sofia_ctx *c, bool energies, float *ff32) {
 
 void launch_dsf_mixed(sofia_ctx *c, bool energies, float *ff32, bool from_xd) {
     hipLaunchKernelGGL(k_fixpos, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N,
-                       from_xd ? c->a.xd : nullptr, c->a.pos4, c->rep, c->reps_d, c->pfix);
+                       from_xd ? c->a.xd : nullptr, c->a.pos4, c->a.q, c->rep, c->reps_d, c->pfix);
     switch (c->lanes) {
         case 0: launch_compact(c, energies, ff32); break;
         case 4: launch_lanes<4>(c, energies, ff32); break;
