@@ -495,7 +495,7 @@ def main():
         "hbm_copy_GBps": round(2.0 * (1 << 30) / (ms_copy * 1e-3) / 1e9, 1),
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic_bytes(),
-                     "kernel": "k_nonbond<mixed> pair kernel", "ms_per_launch": ms_b,
+                     "kernel": "k_dsf_mixed pair kernel", "ms_per_launch": ms_b,
                      "pairs_in_cutoff_per_launch": pairs_b, "flop_per_pair": FLOP_PER_PAIR},
         "cpu_baseline": cpu,
     }

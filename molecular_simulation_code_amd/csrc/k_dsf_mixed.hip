@@ -164,8 +164,7 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     // consumed, so its next gather (4 entries ahead) and list index (8 entries ahead) are
     // in flight for the other three slots and no register is moved while a load is
     // outstanding; scheduling barriers keep the compiler from hoisting the reloads.
-    // Entries past the end of the row are computed on whatever the prefetch read and then
-    // dropped.
+    // Entries past the end of the row are computed on stale slot data and then dropped.
     const uint32_t step = 4u * L;
     uint32_t off = (uint32_t)(ii - iw) * (uint32_t)na * 4u + (uint32_t)sub * 4u;
     int4 PJ[4];
@@ -208,7 +207,10 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
                     vz += (double)(dz * dz * fv);
                 }
             }
-            PJ[u] = ld128(prs, JJ[u] * 16u);
+            // entries past the end of the row (their index is whatever follows in the
+            // list buffer) gather the atom's own record instead, a cache hit
+            const uint32_t jg = (k + (u + 4) * L < nn) ? JJ[u] : (uint32_t)ii;
+            PJ[u] = ld128(prs, jg * 16u);
             JJ[u] = ld32(lrs, off + u * step);
             // VALU, SALU and LDS work may cross; memory loads stay in slot order
             __builtin_amdgcn_sched_barrier(0x0086);
