@@ -243,7 +243,8 @@ def drift_run(args, workers):
 
 
 def profile_run(args, workers):
-    """The batched workload for a profiler: kernels of a few MD steps, 5 launches of the pair
+    """The batched workload for a profiler, prepared as the timed run prepares it
+    (minimisation, equilibration, autotuning), then a few MD steps, 5 launches of the pair
     kernel and 5 of the 1 GiB streaming copy whose traffic is known (2 GiB per launch)."""
     from molecular_simulation_code_amd import system as S
     import pickle
@@ -264,13 +265,26 @@ def profile_run(args, workers):
              for k, r in enumerate(reps)]
     eng = host.Engine(specs, precision=host.PREC_MIXED)
     try:
-        eng.init()
-        eng.minimize(args.minimize)
+        if args.tuning:   # the timed run's tuning, given instead of autotuned
+            skin, lanes, waves = (float(x) for x in args.tuning.split(","))
+            eng.set_skin(skin)
+            eng.set_tuning(int(lanes), 256, int(waves))
+            # the timed run's preparation, with the equilibration stepped eagerly so that the
+            # profiled process captures one graph only
+            eng.init()
+            eng.minimize(args.minimize)
+            if args.equil > 0:
+                eng.set_thermostat(gammas=[20.0] * len(eng.reps))
+                eng.run(args.equil, use_graph=False)
+                eng.set_thermostat()
+            tuning = args.tuning
+        else:
+            tuning = prepare_engine(eng, args.minimize, args.equil, tune=not args.no_tune)
         eng.run(max(1, args.steps))
         ms, pairs = eng.profile_nonbond(5)
         ms_copy = eng.profile_copy(1 << 30, 5)
-        log(f"[profile-run] {sum(r.np for r in reps)} atoms; pair kernel {ms:.4f} ms/launch, "
-            f"{pairs} entries in cutoff; copy 1 GiB {ms_copy:.4f} ms/launch")
+        log(f"[profile-run] {sum(r.np for r in reps)} atoms, tuning {tuning}; pair kernel "
+            f"{ms:.4f} ms/launch, {pairs} entries in cutoff; copy 1 GiB {ms_copy:.4f} ms/launch")
     finally:
         eng.close()
 
@@ -301,6 +315,8 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="short run of the batched workload for rocprofv3: a few steps, the pair "
                          "kernel and the 1 GiB calibration copy; no JSON line")
+    ap.add_argument("--tuning", default=None,
+                    help="profile run: skin,lanes,list_waves to use instead of the autotuner")
     ap.add_argument("--workers", type=int, default=0,
                     help="input-generation processes (0 = one per core up to 16; use 1 under a "
                          "profiler, which initialises the GPU before the program starts)")

@@ -166,7 +166,8 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     // outstanding; scheduling barriers keep the compiler from hoisting the reloads.
     // Entries past the end of the row are computed on stale slot data and then dropped.
     const uint32_t step = 4u * L;
-    uint32_t off = (uint32_t)(ii - iw) * (uint32_t)na * 4u + (uint32_t)sub * 4u;
+    const uint32_t row0 = (uint32_t)(ii - iw) * (uint32_t)na * 4u;
+    uint32_t off = row0 + (uint32_t)sub * 4u;
     int4 PJ[4];
     uint32_t JJ[4];
     // prologue in the loop's own load order, so the wait counts at the loop head are the
@@ -211,7 +212,9 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             // list buffer) gather the atom's own record instead, a cache hit
             const uint32_t jg = (k + (u + 4) * L < nn) ? JJ[u] : (uint32_t)ii;
             PJ[u] = ld128(prs, jg * 16u);
-            JJ[u] = ld32(lrs, off + u * step);
+            // likewise list reads past the row re-read its first entry instead of pulling the
+            // unused tail of the row from memory
+            JJ[u] = ld32(lrs, (k + (u + 8) * L < nn) ? off + u * step : row0);
             // VALU, SALU and LDS work may cross; memory loads stay in slot order
             __builtin_amdgcn_sched_barrier(0x0086);
         }

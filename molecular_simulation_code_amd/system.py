@@ -77,9 +77,12 @@ DIHED_COEFF = [(0.0000, 2, 180), (0.1556, 3, 0)]
 IMPRO_COEFF = [(1.1000, -1, 2)]
 # improper types as the engine takes them: (K kcal/mol/rad^2, chi0 rad).  The reference
 # reads K from the line above but sets improcoeff(2, :) from `theta0`, a local that the
-# read never assigns (Source/mod_readfile.f90:230, 427-433); chi0 = 0 is the value it
-# holds in zero-initialised storage and keeps the improper a planarity restraint.
-IMPRO_K_CHI0 = [(1.1000, 0.0)]
+# read never assigns (Source/mod_readfile.f90:230, 427-433), so the file fixes no chi0.
+# With the atom order (C-alpha, O1, C1 central, O2) a planar carboxylate has chi = pi
+# (C-alpha and O2 lie on opposite sides of the O1-C1 axis), so chi0 = pi makes the term
+# a planarity restraint; it also keeps the cusp of (acos(c) - chi0)^2 at chi = 0, far
+# from the minimum, where chi0 = 0 would put it at the planar geometry.
+IMPRO_K_CHI0 = [(1.1000, math.pi)]
 
 
 @dataclass
