@@ -284,6 +284,28 @@ int sofia_set_molecules(sofia_ctx *ctx, const int32_t *mol);
 int sofia_cluster_largest(sofia_ctx *ctx, float rcontact, int32_t n_o, const int32_t *o_types,
                           int32_t n_n, const int32_t *n_types, int32_t *largest);
 
+/* ---------------------------------------------------------------- files (host only) */
+/* Restart file as dump_global_velocity_id_mpi writes it (Source/mod_datadump.f90:113-151):
+ * box, temp, num, timeclock, then one "id x y z vx vy vz" record per atom ((i5,6f12.6);
+ * ids of 100000 and more widen the id field instead of printing *****).  pos, v: AoS 3*np
+ * in the caller's order, global_id the 1-based ids. */
+int sofia_write_restart(const char *path, float box, float temp, int32_t num, float timeclock,
+                        int32_t np, const int32_t *global_id, const float *pos, const float *v);
+/* read_dump_global_id_v (Source/mod_datadump.f90:212-252): records go to the slot of their
+ * global id (pos, v AoS 3*np in global-id order); an id out of 1..np, a repeated id or a
+ * short file is an error. */
+int sofia_read_restart(const char *path, int32_t np, float *box, float *temp, int32_t *num,
+                       float *timeclock, float *pos, float *v);
+/* read_in_Grofile / read_in_Grofile_box (Source/mod_GroFile.f90:89-159), parsed by column:
+ * atom count into *np, coordinates (A, AoS 3*np, file order) into pos when pos != NULL (an
+ * error if the count exceeds np_max), cubic box (A) into *box.  pos == NULL only sizes. */
+int sofia_read_gro(const char *path, int32_t np_max, int32_t *np, float *pos, float *box);
+/* GRO writer ((i5,2a5,i5,3f8.3) atom lines as in template_generator,
+ * Source/mod_GroFile.f90:7-68; numbers wrap at 100000): resid per atom, resname and
+ * atomname 5 chars per atom, pos in A, cubic box in A. */
+int sofia_write_gro(const char *path, const char *title, int32_t np, const int32_t *resid,
+                    const char *resname, const char *atomname, const float *pos, float box);
+
 #ifdef __cplusplus
 }
 #endif

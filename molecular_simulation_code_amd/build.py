@@ -24,7 +24,7 @@ COMMON = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-Wall",
 # reference bit for bit; the force kernels may contract.
 PER_FILE = {"k_neighbor.hip": ["-ffp-contract=off"]}
 SOURCES = ["sofia_api.hip", "k_neighbor.hip", "k_list_engine.hip", "k_nonbond.hip",
-           "k_dsf_mixed.hip", "k_bonded.hip", "k_integrate.hip", "k_cluster.hip"]
+           "k_dsf_mixed.hip", "k_bonded.hip", "k_integrate.hip", "k_cluster.hip", "sofia_io.cpp"]
 
 
 def hipcc() -> str:
@@ -40,7 +40,7 @@ def build(verbose: bool = False) -> str:
     procs = []
     objs = []
     for src in SOURCES:
-        obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmd = [cc, *COMMON, *PER_FILE.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:

@@ -21,6 +21,8 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+int io_fail(int code, const std::string &msg) { return fail(code, msg); }
+
 int check_hip(hipError_t e, const char *what) {
     g_last_error = std::string(what) + ": " + hipGetErrorString(e);
     return e == hipErrorOutOfMemory ? SOFIA_E_OOM : SOFIA_E_HIP;

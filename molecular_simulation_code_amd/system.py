@@ -81,7 +81,8 @@ IMPRO_COEFF = [(1.1000, -1, 2)]
 # With the atom order (C-alpha, O1, C1 central, O2) a planar carboxylate has chi = pi
 # (C-alpha and O2 lie on opposite sides of the O1-C1 axis), so chi0 = pi makes the term
 # a planarity restraint; it also keeps the cusp of (acos(c) - chi0)^2 at chi = 0, far
-# from the minimum, where chi0 = 0 would put it at the planar geometry.
+# from the minimum, where chi0 = 0 would put it at the planar geometry.  The reference's
+# minimised input input_files/glycine_1500_solv_min.gro has chi = 166-180 deg (mean 175).
 IMPRO_K_CHI0 = [(1.1000, math.pi)]
 
 
@@ -456,6 +457,22 @@ def make_system(n_gly: int, n_wat: int, seed: int = 20261202, params: Params | N
     pos = np.mod(xyz, L).astype(f32)
     pos = np.where(pos <= 0, pos + box32, pos).astype(f32)
     pos = np.where(pos > box32, pos - box32, pos).astype(f32)
+    return assemble(pos, box32, n_gly, n_wat, seed, params, rng)
+
+
+def assemble(pos: np.ndarray, box32, n_gly: int, n_wat: int, seed: int, params: Params,
+             rng: np.random.Generator) -> SofiaSystem:
+    """Topology, charges, SHAKE clusters and Maxwell-Boltzmann velocities for positions
+    (float32, n_gly glycines of 10 atoms in template order, then n_wat waters O, H, H)."""
+    nmol = n_gly + n_wat
+    natom_mol = np.array([10] * n_gly + [3] * n_wat)
+    offsets = np.concatenate([[0], np.cumsum(natom_mol)])
+    N = int(offsets[-1])
+    mol_of_atom = np.repeat(np.arange(nmol), natom_mol)
+    types = np.concatenate([np.array(GLYCINE.types * n_gly, np.int32),
+                            np.array(WATER.types * n_wat, np.int32)]) if nmol else np.zeros(0, np.int32)
+    if len(pos) != N:
+        raise ValueError(f"{len(pos)} positions for {n_gly} glycines and {n_wat} waters ({N} atoms)")
     templates = [GLYCINE] * n_gly + [WATER] * n_wat
 
     # ---- topology -------------------------------------------------------------------
