@@ -115,6 +115,14 @@ int sofia_set_bonded_coeffs(sofia_ctx *ctx, int32_t nbt, const double *bond_k,
  * (Source/global.f90:450); ni_rep per-replica counts (NULL for one replica). */
 int sofia_set_impropers(sofia_ctx *ctx, int32_t nit, const double *k, const double *chi0,
                         const int32_t *improlist, int32_t ni, const int32_t *ni_rep);
+/* Improper dihedrals in the cvff form of impro_amber_force, which the PME force wrappers
+ * call (Source/mod_force_impro_amber.f90:8-230; Source/mod_force.f90:141-143):
+ * E = K [1 + d cos(n phi)] with k_impro, sign_impro (d = +-1) and multiplicity_impro
+ * (n = 0..6) per type (Source/global.f90:158-159, read at Source/mod_readfile.f90:454-456).
+ * Replaces a harmonic set; improlist as for sofia_set_impropers. */
+int sofia_set_impropers_cvff(sofia_ctx *ctx, int32_t nit, const double *k, const int32_t *sign,
+                             const int32_t *mult, const int32_t *improlist, int32_t ni,
+                             const int32_t *ni_rep);
 /* SHAKE clusters: num (1, 3, 4, -3), atoms(4, ns) (0 = unused), bond lengths(3, ns)
  * (Source/global.f90:351-355 shake_info). */
 int sofia_set_shake(sofia_ctx *ctx, int32_t ns, const int32_t *ns_rep, const int32_t *num,

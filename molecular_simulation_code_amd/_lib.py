@@ -65,6 +65,7 @@ SIGNATURES = {
     "sofia_set_bonded_coeffs": [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp,
                                 _i32, _vp],
     "sofia_set_impropers": [_vp, _i32, _vp, _vp, _vp, _i32, _vp],
+    "sofia_set_impropers_cvff": [_vp, _i32, _vp, _vp, _vp, _vp, _i32, _vp],
     "sofia_set_shake": [_vp, _i32, _vp, _vp, _vp, _vp],
     "sofia_put_state": [_vp] * 5,
     "sofia_get_state": [_vp] * 4,

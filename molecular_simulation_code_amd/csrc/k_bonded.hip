@@ -318,6 +318,83 @@ __global__ void k_improper(int32_t items, const int32_t *__restrict__ il, const 
     wave_add(to_efix(e), live, r, eacc, E_IMPRO);
 }
 
+// impro_amber_force, Source/mod_force_impro_amber.f90:8-230 (cvff form):
+// E = K [1 + d cos(n phi)], cos(n phi) the Chebyshev polynomial T_n(c) of c = cos(phi),
+// dE/dc = K d T_n'(c); the gradient of c uses the same a11..a33 coefficients as
+// k_improper, with the bond-angle sines clamped from below by SMALL.  The energy factor is
+// kept in FP64 here (the reference stores it in a real*4 variable).
+__global__ void k_improper_cvff(int32_t items, const int32_t *__restrict__ il,
+                                const double *__restrict__ xd, const int32_t *__restrict__ rep,
+                                const RepInfo *__restrict__ reps, const double *__restrict__ kk,
+                                const int32_t *__restrict__ dn, int32_t n,
+                                long long *__restrict__ fout, long long *__restrict__ eacc) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = m < items;
+    double e = 0.0;
+    int r = 0;
+    if (live) {
+        constexpr double SMALL = (double)0.001f;
+        const int i1 = il[(size_t)m * 5], i2 = il[(size_t)m * 5 + 1], i3 = il[(size_t)m * 5 + 2];
+        const int i4 = il[(size_t)m * 5 + 3], t = il[(size_t)m * 5 + 4] - 1;
+        r = rep[i1];
+        const double bx = reps[r].box, ib = reps[r].ibox;
+        double b1[3], b2[3], b3[3];
+        dvec64(xd, n, i1, i2, bx, ib, b1);
+        dvec64(xd, n, i3, i2, bx, ib, b2);
+        dvec64(xd, n, i4, i3, bx, ib, b3);
+        const double b1sq = b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2];
+        const double b2sq = b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2];
+        const double b3sq = b3[0] * b3[0] + b3[1] * b3[1] + b3[2] * b3[2];
+        const double sb1 = 1.0 / b1sq, sb2 = 1.0 / b2sq, sb3 = 1.0 / b3sq;
+        const double rb1 = sqrt(sb1), rb3 = sqrt(sb3);
+        const double c0 = (b1[0] * b3[0] + b1[1] * b3[1] + b1[2] * b3[2]) * rb1 * rb3;
+        const double r12c1 = 1.0 / (sqrt(b1sq) * sqrt(b2sq));
+        const double c1 = (b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2]) * r12c1;
+        const double r12c2 = 1.0 / (sqrt(b2sq) * sqrt(b3sq));
+        const double c2 = -(b2[0] * b3[0] + b2[1] * b3[1] + b2[2] * b3[2]) * r12c2;
+        const double sc1 = 1.0 / fmax(sqrt(1.0 - c1 * c1), SMALL);
+        const double sc2 = 1.0 / fmax(sqrt(1.0 - c2 * c2), SMALL);
+        const double s1 = sc1 * sc1, s2 = sc2 * sc2;
+        double s12 = sc1 * sc2;
+        double c = fmin(fmax((c0 + c1 * c2) * s12, -1.0), 1.0);
+        const int d = dn[2 * t], nm = dn[2 * t + 1];
+        // p = 1 + T_n(c), dp/dc = T_n'(c)
+        const double rc2 = c * c;
+        double p, dp;
+        switch (nm) {
+            case 0: p = 2.0; dp = 0.0; break;
+            case 1: p = c + 1.0; dp = 1.0; break;
+            case 2: p = 2.0 * rc2; dp = 4.0 * c; break;
+            case 3: p = (4.0 * rc2 - 3.0) * c + 1.0; dp = 12.0 * rc2 - 3.0; break;
+            case 4: p = 8.0 * (rc2 - 1.0) * rc2 + 2.0; dp = (32.0 * rc2 - 16.0) * c; break;
+            case 5: p = ((16.0 * rc2 - 20.0) * rc2 + 5.0) * c + 1.0; dp = (80.0 * rc2 - 60.0) * rc2 + 5.0; break;
+            default: p = ((32.0 * rc2 - 48.0) * rc2 + 18.0) * rc2; dp = (192.0 * (rc2 - 1.0) * rc2 + 36.0) * c; break;
+        }
+        if (d == -1) { p = 2.0 - p; dp = -dp; }
+        e = kk[t] * p;
+        const double a = kk[t] * dp;
+        c *= a;
+        s12 *= a;
+        const double a11 = c * sb1 * s1;
+        const double a22 = -sb2 * (2.0 * c0 * s12 - c * (s1 + s2));
+        const double a33 = c * sb3 * s2;
+        const double a12 = -r12c1 * (c1 * c * s1 + c2 * s12);
+        const double a13 = -rb1 * rb3 * s12;
+        const double a23 = r12c2 * (c2 * c * s2 + c1 * s12);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double sx2 = a12 * b1[k] + a22 * b2[k] + a23 * b3[k];
+            const double f1 = a11 * b1[k] + a12 * b2[k] + a13 * b3[k];
+            const double f4 = a13 * b1[k] + a23 * b2[k] + a33 * b3[k];
+            atomic_fix(&fout[(size_t)k * n + i1], f1);
+            atomic_fix(&fout[(size_t)k * n + i2], -sx2 - f1);
+            atomic_fix(&fout[(size_t)k * n + i3], sx2 - f4);
+            atomic_fix(&fout[(size_t)k * n + i4], f4);
+        }
+    }
+    wave_add(to_efix(e), live, r, eacc, E_IMPRO);
+}
+
 }  // namespace
 
 void launch_correct(sofia_ctx *c, int which, long long *fout) {
@@ -359,10 +436,14 @@ void launch_bonded_engine(sofia_ctx *c) {
         hipLaunchKernelGGL(k_torsion, dim3(nblocks(c->ntor, 256)), dim3(256), 0, c->stream, c->ntor,
                            c->torsions, c->a.xd, c->rep, c->reps_d, c->tors_k, c->tors_n,
                            c->tors_shift, c->tors_cs, n, c->fix, c->eacc);
-    if (c->nimp)
+    if (c->nimp && c->impro_style == 0)
         hipLaunchKernelGGL(k_improper, dim3(nblocks(c->nimp, 256)), dim3(256), 0, c->stream, c->nimp,
                            c->improlist, c->a.xd, c->rep, c->reps_d, c->impro_k, c->impro_chi0, n,
                            c->fix, c->eacc);
+    else if (c->nimp)
+        hipLaunchKernelGGL(k_improper_cvff, dim3(nblocks(c->nimp, 256)), dim3(256), 0, c->stream,
+                           c->nimp, c->improlist, c->a.xd, c->rep, c->reps_d, c->impro_k,
+                           c->impro_dn, n, c->fix, c->eacc);
 }
 
 }  // namespace sofia

@@ -516,7 +516,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
     dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->improlist); dfree(c->impro_k);
-    dfree(c->impro_chi0); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
+    dfree(c->impro_chi0); dfree(c->impro_dn); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -645,14 +645,23 @@ int sofia_set_bonded_coeffs(sofia_ctx *c, int32_t nbt, const double *bk, const d
     return SOFIA_OK;
 }
 
-int sofia_set_impropers(sofia_ctx *c, int32_t nit, const double *k, const double *chi0,
-                        const int32_t *improlist, int32_t ni, const int32_t *ni_rep) {
+namespace {
+int set_impropers(sofia_ctx *c, int style, int32_t nit, const double *k, const double *chi0,
+                  const int32_t *sign, const int32_t *mult, const int32_t *improlist, int32_t ni,
+                  const int32_t *ni_rep) {
     if (!c || nit < 0 || ni < 0) return fail(SOFIA_E_ARG, "bad argument");
-    if (ni > 0 && (nit == 0 || !k || !chi0 || !improlist))
+    if (ni > 0 && (nit == 0 || !k || !improlist || (style == 0 && !chi0) ||
+                   (style == 1 && (!sign || !mult))))
         return fail(SOFIA_E_ARG, "impropers need their type coefficients");
+    if (style == 1)
+        for (int t = 0; t < nit; ++t) {
+            if (sign[t] != 1 && sign[t] != -1) return fail(SOFIA_E_ARG, "cvff improper sign must be +1 or -1");
+            if (mult[t] < 0 || mult[t] > 6) return fail(SOFIA_E_ARG, "cvff improper multiplicity must be 0..6");
+        }
     TRY(unsplit_water(c));
-    dfree(c->impro_k); dfree(c->impro_chi0); dfree(c->improlist);
+    dfree(c->impro_k); dfree(c->impro_chi0); dfree(c->impro_dn); dfree(c->improlist);
     c->nimp = 0;
+    c->impro_style = style;
     if (!ni) return SOFIA_OK;
     for (int m = 0; m < ni; ++m) {
         const int t = improlist[5 * (size_t)m + 4];
@@ -660,15 +669,35 @@ int sofia_set_impropers(sofia_ctx *c, int32_t nit, const double *k, const double
         for (int w = 0; w < 4; ++w)
             if (improlist[5 * (size_t)m + w] == 0) return fail(SOFIA_E_ARG, "improper with a missing atom");
     }
-    TRY(dalloc(&c->impro_k, nit)); TRY(dalloc(&c->impro_chi0, nit));
+    TRY(dalloc(&c->impro_k, nit));
     TRY(upload(c->impro_k, k, sizeof(double) * nit));
-    TRY(upload(c->impro_chi0, chi0, sizeof(double) * nit));
+    if (style == 0) {
+        TRY(dalloc(&c->impro_chi0, nit));
+        TRY(upload(c->impro_chi0, chi0, sizeof(double) * nit));
+    } else {
+        std::vector<int32_t> dn((size_t)2 * nit);
+        for (int t = 0; t < nit; ++t) { dn[2 * (size_t)t] = sign[t]; dn[2 * (size_t)t + 1] = mult[t]; }
+        TRY(dalloc(&c->impro_dn, dn.size()));
+        TRY(upload(c->impro_dn, dn.data(), sizeof(int32_t) * dn.size()));
+    }
     std::vector<int32_t> g;
     TRY(to_global_list(c, improlist, ni, 5, 4, ni_rep, g));
     TRY(dalloc(&c->improlist, g.size()));
     TRY(upload(c->improlist, g.data(), sizeof(int32_t) * g.size()));
     c->nimp = ni;
     return SOFIA_OK;
+}
+}  // namespace
+
+int sofia_set_impropers(sofia_ctx *c, int32_t nit, const double *k, const double *chi0,
+                        const int32_t *improlist, int32_t ni, const int32_t *ni_rep) {
+    return set_impropers(c, 0, nit, k, chi0, nullptr, nullptr, improlist, ni, ni_rep);
+}
+
+int sofia_set_impropers_cvff(sofia_ctx *c, int32_t nit, const double *k, const int32_t *sign,
+                             const int32_t *mult, const int32_t *improlist, int32_t ni,
+                             const int32_t *ni_rep) {
+    return set_impropers(c, 1, nit, k, nullptr, sign, mult, improlist, ni, ni_rep);
 }
 
 int sofia_set_shake(sofia_ctx *c, int32_t ns, const int32_t *ns_rep, const int32_t *num,

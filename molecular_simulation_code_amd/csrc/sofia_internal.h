@@ -136,6 +136,8 @@ struct sofia_ctx {
     double *tors_k = nullptr, *tors_n = nullptr, *tors_shift = nullptr;
     double *tors_cs = nullptr;  // per torsion type: cos(shift), sin(shift)
     double *impro_k = nullptr, *impro_chi0 = nullptr;   // per improper type
+    int32_t *impro_dn = nullptr;   // cvff style: (d, n) per improper type
+    int32_t impro_style = 0;       // 0 harmonic (lammps14), 1 cvff (amber wrapper)
     float4 *dsf_tab = nullptr;  // cubic pieces of the DSF Coulomb force / energy (mixed kernel)
     int4 *pfix = nullptr;       // mixed kernel positions: x 2^32 / box mod 2^32, type (per launch)
     int32_t nbt = 0, nat = 0, ntt = 0;

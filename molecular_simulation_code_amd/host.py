@@ -82,12 +82,22 @@ def _bonded_coeffs(ctx, tors: np.ndarray, ntor_rep=None):
     return keep
 
 
-def _impropers(ctx, lists):
-    """improper_harmonic_lammps14 items (i1, i2, i3 central, i4, type) per replica."""
-    ik = _c([c[0] for c in S.IMPRO_K_CHI0], f64)
-    ic = _c([c[1] for c in S.IMPRO_K_CHI0], f64)
+def _impropers(ctx, lists, coul_flag=COUL_DSF):
+    """Improper items (i1, i2, i3 central, i4, type) per replica, in the form the reference's
+    force wrapper of that electrostatics calls: improper_harmonic_lammps14 with DSF
+    (Source/mod_force.f90:542-544), the cvff impro_amber_force with Ewald
+    (Source/mod_force.f90:141-143) and the topology's (K, d, n)."""
     il = _c(np.concatenate(lists) if lists else np.zeros((0, 5), i32), i32)
     nr = _c([len(x) for x in lists], i32)
+    if coul_flag == COUL_EWALD:
+        ik = _c([c[0] for c in S.IMPRO_COEFF], f64)
+        isg = _c([c[1] for c in S.IMPRO_COEFF], i32)
+        imt = _c([c[2] for c in S.IMPRO_COEFF], i32)
+        check(lib().sofia_set_impropers_cvff(ctx, len(ik), ptr(ik), ptr(isg), ptr(imt), ptr(il),
+                                             len(il), ptr(nr)))
+        return [ik, isg, imt, il, nr]
+    ik = _c([c[0] for c in S.IMPRO_K_CHI0], f64)
+    ic = _c([c[1] for c in S.IMPRO_K_CHI0], f64)
     check(lib().sofia_set_impropers(ctx, len(ik), ptr(ik), ptr(ic), ptr(il), len(il), ptr(nr)))
     return [ik, ic, il, nr]
 
@@ -430,7 +440,7 @@ class Engine:
             tl = [torsion_list(r.system) for r in reps]
             self._keep += _bonded_coeffs(h, np.concatenate(tl) if tl else np.zeros((0, 5), i32),
                                          [len(t) for t in tl])
-            self._keep += _impropers(h, [r.system.improlist for r in reps])
+            self._keep += _impropers(h, [r.system.improlist for r in reps], coul_flag)
         sn = cat(lambda s: s.shake_num, i32)
         sa = cat(lambda s: s.shake_atoms, i32)
         sb = cat(lambda s: s.shake_bond, f64)
@@ -571,6 +581,15 @@ class Engine:
         v = np.empty((self.N, 3), f64)
         check(lib().sofia_get_state_f64(self.ctx, ptr(x), ptr(v)))
         return self._by_gid((x, v))
+
+    def write_restart(self, k: int, path, num: int = 0, timeclock: float = 0.0):
+        """Replica k's positions and velocities as a restart file in the layout of
+        dump_global_velocity_id_mpi (Source/mod_datadump.f90:113-151), in global-id order."""
+        from . import io as sio
+        x, v = self.state()[k]
+        s = self.reps[k].system
+        sio.write_restart(path, float(s.box), float(self.reps[k].temp), num, timeclock,
+                          np.arange(1, s.np + 1, dtype=i32), x, v)
 
     def forces(self):
         """Total force per replica after the last step (global_id order): the force the

@@ -356,6 +356,20 @@ def improper_harmonic(improlist, pos, types, k, chi0, box, ibox):
     return f, e.value, vir
 
 
+def improper_cvff(improlist, pos, types, k, sign, mult, box, ibox):
+    """impro_amber_force: (forces (n, 3), e_improper, virial (3,))."""
+    n = len(pos)
+    a = ainfo(pos, types)
+    f = np.zeros((n, 3), _d)
+    e = C.c_double(0.0)
+    vir = np.zeros(3, _d)
+    il = np.ascontiguousarray(improlist, _i)
+    lib().orc_improper_cvff(C.c_int32(len(il)), P(il), P(a), P(np.ascontiguousarray(k, _d)),
+                            P(np.ascontiguousarray(sign, _i)), P(np.ascontiguousarray(mult, _i)),
+                            C.c_float(box), C.c_float(ibox), P(f), C.byref(e), P(vir))
+    return f, e.value, vir
+
+
 # ---------------------------------------------------------------- reference library front end
 class Ref:
     """The reference's Fortran routines (oracle/_ref/libsofia_ref.so)."""
@@ -446,6 +460,17 @@ class Ref:
         vir = np.zeros(3, _d)
         self.L.ref_improper(C.c_int32(len(lst)), P(lst), C.c_int32(len(k)), P(k), P(chi0), P(f),
                             C.byref(e), P(vir))
+        return f, e.value, vir
+
+    def improper_cvff(self, improlist, k, sign, mult):
+        lst = np.ascontiguousarray(improlist, _i)
+        k = np.ascontiguousarray(k, _d)
+        sign = np.ascontiguousarray(sign, _i); mult = np.ascontiguousarray(mult, _i)
+        f = np.zeros((self.n, 3), _f)
+        e = C.c_double(0.0)
+        vir = np.zeros(3, _d)
+        self.L.ref_improper_cvff(C.c_int32(len(lst)), P(lst), C.c_int32(len(k)), P(k), P(sign),
+                                 P(mult), P(f), C.byref(e), P(vir))
         return f, e.value, vir
 
     def pbc(self, pos):

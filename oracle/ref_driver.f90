@@ -2,7 +2,8 @@
 !
 ! oracle/build_ref.sh compiles this file together with the reference modules where they
 ! lie (/root/reference/Source: global, mod_minimg, mod_force_nonbond,
-! mod_force_correction, mod_force_dihed, mod_force_bond, mod_force_angle, mod_pbc) into
+! mod_force_correction, mod_force_dihed, mod_force_bond, mod_force_angle, mod_pbc,
+! mod_force_impro_amber) into
 ! oracle/_ref/libsofia_ref.so.  The entry points below only copy caller arrays into
 ! `module global`, call one reference subroutine unchanged, and copy its outputs back,
 ! so the golden-vector script in tests/golden/ can compare the C oracle with the
@@ -16,6 +17,7 @@ module sofia_ref_driver
   use mod_force_bond
   use mod_force_angle
   use mod_pbc
+  use mod_force_impro_amber
   implicit none
 contains
 
@@ -249,6 +251,33 @@ contains
     ei = e_improper
     vir(1) = v_improx; vir(2) = v_improy; vir(3) = v_improz
   end subroutine ref_improper
+
+  ! impro_amber_force (Source/mod_force_impro_amber.f90:8-230).
+  subroutine ref_improper_cvff(n, lst, nit, k, sgn, mlt, fout, ei, vir) &
+       bind(c, name="ref_improper_cvff")
+    integer(c_int), value :: n, nit
+    integer(c_int), intent(in) :: lst(5, n), sgn(nit), mlt(nit)
+    real(c_double), intent(in) :: k(nit)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: ei, vir(3)
+    integer :: i
+    if (allocated(improlist)) deallocate(improlist)
+    allocate(improlist(5, n)); improlist = lst; numimpro = n
+    if (allocated(k_impro)) deallocate(k_impro, sign_impro, multiplicity_impro)
+    allocate(k_impro(nit), sign_impro(nit), multiplicity_impro(nit))
+    k_impro = k; sign_impro = sgn; multiplicity_impro = mlt
+    if (allocated(ff_impro)) deallocate(ff_impro)
+    allocate(ff_impro(np))
+    do i = 1, np
+       ff_impro(i) = coords(0.0, 0.0, 0.0)
+    end do
+    e_improper = 0.0d0
+    v_improx = 0.0d0; v_improy = 0.0d0; v_improz = 0.0d0
+    call impro_amber_force()
+    call get_forces(ff_impro, fout)
+    ei = e_improper
+    vir(1) = v_improx; vir(2) = v_improy; vir(3) = v_improz
+  end subroutine ref_improper_cvff
 
   ! pbc (Source/mod_pbc.f90:6-39) on xyz in place.
   subroutine ref_pbc(xyz) bind(c, name="ref_pbc")

@@ -1007,3 +1007,94 @@ void orc_improper_harmonic(int32_t ni, const int32_t *improlist, const orc_ainfo
     }
     *e_improper = e;
 }
+
+/* impro_amber_force, Source/mod_force_impro_amber.f90:8-230 (the cvff form):
+ * E = K [1 + d cos(n phi)] with cos(n phi) the Chebyshev polynomial of c = cos(phi),
+ * d = sign_impro = +-1 and n = multiplicity_impro in 0..6.  The routine keeps the energy
+ * factor p in the module's real*4 pressure variable P (Source/global.f90:137), so p is
+ * rounded to float before E += K p; pd stays double.  SMALL is the real*4 literal 0.001
+ * and sqrt(1 - c^2) of the two bond angles is clamped from below by it. */
+void orc_improper_cvff(int32_t ni, const int32_t *improlist, const orc_ainfo *pos,
+                       const double *k, const int32_t *sign, const int32_t *mult, float box,
+                       float ibox, double *f, double *e_improper, double *virial) {
+    const double small = (double)0.001f;
+    double e = *e_improper;
+    for (int m = 0; m < ni; ++m) {
+        const int i1 = improlist[5 * m] - 1, i2 = improlist[5 * m + 1] - 1;
+        const int i3 = improlist[5 * m + 2] - 1, i4 = improlist[5 * m + 3] - 1;
+        const int t = improlist[5 * m + 4] - 1;
+        double b1[3], b2[3], b2m[3], b3[3];
+        dvec(pos, i1, i2, box, ibox, b1);
+        dvec(pos, i3, i2, box, ibox, b2);
+        for (int q = 0; q < 3; ++q) b2m[q] = minimg_nint(-b2[q], box, ibox);
+        dvec(pos, i4, i3, box, ibox, b3);
+        const double sb1 = 1.0 / (b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2]);
+        const double sb2 = 1.0 / (b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
+        const double sb3 = 1.0 / (b3[0] * b3[0] + b3[1] * b3[1] + b3[2] * b3[2]);
+        const double rb1 = sqrt(sb1), rb3 = sqrt(sb3);
+        const double c0 = (b1[0] * b3[0] + b1[1] * b3[1] + b1[2] * b3[2]) * rb1 * rb3;
+        const double b1mag = sqrt(b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2]);
+        const double b2mag = sqrt(b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
+        const double b3mag = sqrt(b3[0] * b3[0] + b3[1] * b3[1] + b3[2] * b3[2]);
+        const double r12c1 = 1.0 / (b1mag * b2mag);
+        const double c1mag = (b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2]) * r12c1;
+        const double r12c2 = 1.0 / (b2mag * b3mag);
+        const double c2mag = (b2m[0] * b3[0] + b2m[1] * b3[1] + b2m[2] * b3[2]) * r12c2;
+        double sc1 = sqrt(1.0 - c1mag * c1mag);
+        if (sc1 < small) sc1 = small;
+        sc1 = 1.0 / sc1;
+        double sc2 = sqrt(1.0 - c2mag * c2mag);
+        if (sc2 < small) sc2 = small;
+        sc2 = 1.0 / sc2;
+        const double s1 = sc1 * sc1, s2 = sc2 * sc2;
+        double s12 = sc1 * sc2;
+        double c = (c0 + c1mag * c2mag) * s12;
+        if (c > 1.0) c = 1.0;
+        if (c < -1.0) c = -1.0;
+        float p = 0.0f;
+        double pd = 0.0, rc2;
+        switch (mult[t]) {
+            case 2: p = (float)(2.0 * c * c); pd = 2.0 * c; break;
+            case 3: rc2 = c * c; p = (float)((4.0 * rc2 - 3.0) * c + 1.0); pd = 6.0 * rc2 - 1.5; break;
+            case 4: rc2 = c * c; p = (float)(8.0 * (rc2 - 1) * rc2 + 2.0); pd = (16.0 * rc2 - 8.0) * c; break;
+            case 6:
+                rc2 = c * c;
+                p = (float)(((32.0 * rc2 - 48.0) * rc2 + 18.0) * rc2);
+                pd = (96.0 * (rc2 - 1.0) * rc2 + 18.0) * c;
+                break;
+            case 1: p = (float)(c + 1.0); pd = 0.5; break;
+            case 5:
+                rc2 = c * c;
+                p = (float)(((16.0 * rc2 - 20.0) * rc2 + 5.0) * c + 1.0);
+                pd = (40.0 * rc2 - 30.0) * rc2 + 2.5;
+                break;
+            case 0: p = 2.0f; pd = 0.0; break;
+            default: break;   /* the reference leaves p, pd as they were */
+        }
+        if (sign[t] == -1) {
+            p = 2.0f - p;
+            pd = -pd;
+        }
+        e = e + k[t] * (double)p;
+        const double a = 2.0 * k[t] * pd;
+        c = c * a;
+        s12 = s12 * a;
+        const double a11 = c * sb1 * s1;
+        const double a22 = -sb2 * (2.0 * c0 * s12 - c * (s1 + s2));
+        const double a33 = c * sb3 * s2;
+        const double a12 = -r12c1 * (c1mag * c * s1 + c2mag * s12);
+        const double a13 = -rb1 * rb3 * s12;
+        const double a23 = r12c2 * (c2mag * c * s2 + c1mag * s12);
+        for (int q = 0; q < 3; ++q) {
+            const double sx2 = a12 * b1[q] + a22 * b2[q] + a23 * b3[q];
+            const double f1 = a11 * b1[q] + a12 * b2[q] + a13 * b3[q];
+            const double f2 = -sx2 - f1;
+            const double f4 = a13 * b1[q] + a23 * b2[q] + a33 * b3[q];
+            const double f3 = sx2 - f4;
+            f[3 * i1 + q] += f1; f[3 * i2 + q] += f2;
+            f[3 * i3 + q] += f3; f[3 * i4 + q] += f4;
+            if (virial) virial[q] = virial[q] + b1[q] * f1 + b2[q] * f3 + (b3[q] + b2[q]) * f4;
+        }
+    }
+    *e_improper = e;
+}

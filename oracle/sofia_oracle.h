@@ -118,6 +118,11 @@ void orc_torsion_cos(int32_t nd, const int32_t *torsions, const orc_ainfo *pos,
 void orc_improper_harmonic(int32_t ni, const int32_t *improlist, const orc_ainfo *pos,
                            const double *k, const double *chi0, float box, float ibox,
                            double *f, double *e_improper, double *virial);
+/* impro_amber_force (Source/mod_force_impro_amber.f90:8-230): E = K [1 + d cos(n phi)];
+ * k, sign (d = +-1) and mult (n = 0..6) per type; virial may be NULL. */
+void orc_improper_cvff(int32_t ni, const int32_t *improlist, const orc_ainfo *pos,
+                       const double *k, const int32_t *sign, const int32_t *mult, float box,
+                       float ibox, double *f, double *e_improper, double *virial);
 
 #ifdef __cplusplus
 }

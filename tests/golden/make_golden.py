@@ -12,6 +12,7 @@ the outputs of the reference Fortran routines on it:
 * lj_cut_coul_dsf_N2                   forces, potential, e_coul
 * bond_harmonic, angle_harmonic        forces, energies
 * improper_harmonic_lammps14           forces, e_improper, virial (two improper types)
+* impro_amber_force (cvff form)        forces, e_improper, virial (two improper types)
 * pbc                                  wrapped coordinates
 
 The Verlet list these routines consume is the oracle's (mod_neighbor.f90 needs Intel's
@@ -45,6 +46,10 @@ CASES = {
 # improper types: (K kcal/mol/rad^2, chi0 rad); the second has a non-zero equilibrium angle
 IMPRO_K = np.array([1.1, 0.6])
 IMPRO_CHI0 = np.array([0.0, 0.35])
+# cvff improper types (K, d, n): the topology's line 1.1 -1 2, and a second with d = +1, n = 3
+CVFF_K = np.array([1.1, 0.7])
+CVFF_SIGN = np.array([-1, 1], np.int32)
+CVFF_MULT = np.array([2, 3], np.int32)
 
 
 def pair_digest(nlist, numneigh, neigh_alloc, ids=None):
@@ -111,6 +116,9 @@ def main():
         g["angle_f"], g["angle_e"] = R.angle(g["anglelist"], ka, th)
         g["impro_k"], g["impro_chi0"] = IMPRO_K, IMPRO_CHI0
         g["impro_f"], g["impro_e"], g["impro_vir"] = R.improper(g["improlist"], IMPRO_K, IMPRO_CHI0)
+        g["cvff_k"], g["cvff_sign"], g["cvff_mult"] = CVFF_K, CVFF_SIGN, CVFF_MULT
+        g["cvff_f"], g["cvff_e"], g["cvff_vir"] = R.improper_cvff(g["improlist"], CVFF_K, CVFF_SIGN,
+                                                                 CVFF_MULT)
         rng = np.random.default_rng(3)
         x = g["pos"].copy()
         k = len(x) // 8

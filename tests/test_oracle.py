@@ -508,3 +508,42 @@ def test_improper_planar_cis_and_trans():
     _, e, _ = O.improper_harmonic(il, trans, np.ones(4, np.int32), np.array([2.0]), np.array([0.0]),
                                   L, np.float32(1.0) / L)
     assert e == pytest.approx(2.0 * math.pi ** 2, rel=1e-7)   # float32 input geometry
+
+
+def test_golden_improper_cvff(golden):
+    d = _derived(golden)
+    f, e, vir = O.improper_cvff(golden["improlist"], golden["pos"], golden["types"],
+                                golden["cvff_k"], golden["cvff_sign"], golden["cvff_mult"],
+                                d["box"], d["ibox"])
+    ref = golden["cvff_f"].astype(np.float64)
+    assert np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()) < 1e-6
+    assert e == pytest.approx(float(golden["cvff_e"]), rel=1e-12)
+    np.testing.assert_allclose(vir, golden["cvff_vir"], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 4, 5, 6])
+def test_improper_cvff_closed_form(n):
+    """E = K [1 + d cos(n phi)] with phi the dihedral from plane normals, for every
+    multiplicity the reference codes, both signs; zero net force and torque; -dE/dx."""
+    rng = np.random.default_rng(30 + n)
+    L = np.float32(40.0)
+    ib = np.float32(1.0) / L
+    for trial in range(8):
+        x = (np.float32(20.0) + rng.normal(0, 1.2, (4, 3))).astype(np.float32)
+        d = 1 if trial % 2 else -1
+        il = np.array([[1, 2, 3, 4, 1]], np.int32)
+        args = (np.array([0.8]), np.array([d], np.int32), np.array([n], np.int32), L, ib)
+        f, e, _ = O.improper_cvff(il, x, np.ones(4, np.int32), *args)
+        phi = _plane_angle(x.astype(np.float64), 0, 1, 2, 3)
+        assert e == pytest.approx(0.8 * (1 + d * math.cos(n * phi)), rel=2e-6, abs=1e-6)
+        np.testing.assert_allclose(f.sum(axis=0), 0.0, atol=1e-9)
+        np.testing.assert_allclose(np.cross(x.astype(np.float64), f).sum(axis=0), 0.0, atol=1e-7)
+        h = 1e-3
+        for atom in range(4):
+            for c in range(3):
+                xp = x.copy(); xp[atom, c] += h
+                xm = x.copy(); xm[atom, c] -= h
+                dx = float(xp[atom, c]) - float(xm[atom, c])
+                g = (O.improper_cvff(il, xp, np.ones(4, np.int32), *args)[1]
+                     - O.improper_cvff(il, xm, np.ones(4, np.int32), *args)[1]) / dx
+                assert f[atom, c] == pytest.approx(-g, rel=5e-3, abs=5e-3)
