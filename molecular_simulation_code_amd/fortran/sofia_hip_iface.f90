@@ -13,6 +13,8 @@
 !   nve_v(dt_partial), nve_x                Source/mod_ensemble.f90:11-87
 !   pbc(step)                               Source/mod_pbc.f90:6-39
 !   shake(iflag)                            Source/mod_shake.f90:8-85
+!   dump_global_velocity_id_mpi(step, num)  Source/mod_datadump.f90:113-151
+!   read_dump_global_id_v(filename)         Source/mod_datadump.f90:212-252
 !
 ! A maintainer swaps the calls (or renames the wrappers) in force_driver / the time loops;
 ! sofia_hip_init is called once after the reference's initialisation sequence
@@ -164,6 +166,24 @@ module sofia_hip_iface
        type(*), intent(inout) :: ff(*)
        real(c_double), intent(inout) :: vir(3)
        integer(c_int32_t), intent(out) :: maxit
+     end function
+     integer(c_int) function sofia_write_restart(path, box, temp, num, timeclock, np, gid, pos, v) &
+          bind(c, name="sofia_write_restart")
+       import :: c_int, c_char, c_float, c_int32_t
+       character(kind=c_char), intent(in) :: path(*)
+       real(c_float), value :: box, temp, timeclock
+       integer(c_int32_t), value :: num, np
+       integer(c_int32_t), intent(in) :: gid(*)
+       real(c_float), intent(in) :: pos(*), v(*)
+     end function
+     integer(c_int) function sofia_read_restart(path, np, box, temp, num, timeclock, pos, v) &
+          bind(c, name="sofia_read_restart")
+       import :: c_int, c_char, c_float, c_int32_t
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: np
+       real(c_float), intent(out) :: box, temp, timeclock
+       integer(c_int32_t), intent(out) :: num
+       real(c_float), intent(out) :: pos(*), v(*)
      end function
      integer(c_int) function sofia_last_error(buf, n) bind(c, name="sofia_last_error")
        import :: c_int, c_char, c_size_t
@@ -388,5 +408,46 @@ contains
     call sofia_check(sofia_shake(ctx, int(iflag, c_int32_t), ff, vir, its), 'shake')
     virialx = vir(1); virialy = vir(2); virialz = vir(3)
   end subroutine shake_hip
+
+  ! restart_<step> under restart_files_path, as dump_global_velocity_id_mpi writes it (the
+  ! reference also flushes its observable files first; that stays with the caller).
+  subroutine dump_global_velocity_id_mpi_hip(step, num)
+    integer, intent(in) :: step, num
+    character(len=50) :: ints
+    character(len=256) :: filename
+    real(c_float), allocatable :: xyz(:, :), vel(:, :)
+    integer(c_int32_t), allocatable :: gid(:)
+    integer :: i
+    allocate(xyz(3, np), vel(3, np), gid(np))
+    do i = 1, np
+       xyz(1, i) = position(i)%x; xyz(2, i) = position(i)%y; xyz(3, i) = position(i)%z
+       vel(1, i) = v(i)%x; vel(2, i) = v(i)%y; vel(3, i) = v(i)%z
+       gid(i) = global_id(i)
+    end do
+    write(ints, '(i8)') step
+    filename = trim(restart_files_path)//'/'//'restart_'//trim(adjustl(ints))
+    call sofia_check(sofia_write_restart(trim(filename)//c_null_char, box, temp, int(num, c_int32_t), &
+                                         timeclock, int(np, c_int32_t), gid, xyz, vel), 'write_restart')
+  end subroutine dump_global_velocity_id_mpi_hip
+
+  ! read_dump_global_id_v: box, timeclock and the state placed by global id; hbox, ibox and
+  ! vol follow the box as in the reference.
+  subroutine read_dump_global_id_v_hip(filename)
+    character(len=*), intent(in) :: filename
+    real(c_float), allocatable :: xyz(:, :), vel(:, :)
+    real(c_float) :: junk
+    integer(c_int32_t) :: read_num
+    integer :: i
+    allocate(xyz(3, np), vel(3, np))
+    call sofia_check(sofia_read_restart(trim(filename)//c_null_char, int(np, c_int32_t), box, junk, &
+                                        read_num, timeclock, xyz, vel), 'read_restart')
+    hbox = box / 2.0
+    ibox = 1.0 / box
+    vol = box * box * box
+    do i = 1, np
+       position(i)%x = xyz(1, i); position(i)%y = xyz(2, i); position(i)%z = xyz(3, i)
+       v(i)%x = vel(1, i); v(i)%y = vel(2, i); v(i)%z = vel(3, i)
+    end do
+  end subroutine read_dump_global_id_v_hip
 
 end module sofia_hip_dropin
