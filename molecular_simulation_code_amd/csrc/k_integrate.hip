@@ -620,20 +620,147 @@ __device__ inline int shake_cluster(int m, const ShakeArgs &A, double vir[3], lo
     return rr;
 }
 
+// Rigid symmetric 3-site water (a -3 cluster with |O-H1| = |O-H2| and equal H masses) by
+// SETTLE (Miyamoto & Kollman, J. Comput. Chem. 13, 952, 1992): the closed-form solution of
+// the constraint equations that the iterative triangle SHAKE converges to.  Positions are
+// taken relative to the old O with the minimum image; the constraint force of an atom is
+// m (x_settled - x_unconstrained) / dtsq, the O force is minus the sum of the H forces in
+// fixed point (exact momentum), and the virial is sum_k lambda_k r_k^2, written through the
+// H forces and the old O-H vectors.  The fused water bond/angle terms are evaluated first,
+// as in shake_cluster<-3>.
+__device__ inline int settle_water(int m, const ShakeArgs &A, double vir[3], long long &eb,
+                                   long long &ea) {
+    int rr;
+    {
+        int idx[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) idx[k] = A.atoms[4 * (size_t)m + k];
+        const double mo = A.mass[__float_as_int(A.pos4[idx[0]].w) - 1];
+        const double mh = A.mass[__float_as_int(A.pos4[idx[1]].w) - 1];
+        const double doh = A.bond[3 * (size_t)m], dhh = A.bond[3 * (size_t)m + 2];
+        rr = A.rep[idx[0]];
+        const double box = A.reps[rr].box, ibox = A.reps[rr].ibox;
+        const int n = A.n;
+        long long own[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) own[a][c] = A.fix[(size_t)c * n + idx[a]];
+        const int4 w = A.water[m];
+        const bool fused = w.w != 0;
+        if (fused) water_bonded(w, A, idx, box, ibox, own, eb, ea);
+        // old (b0, c0) and unconstrained new (x1) positions relative to the old O
+        const double im[3] = {1.0 / mo, 1.0 / mh, 1.0 / mh};
+        double b0[3], c0[3], x1[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t oo = (size_t)c * n + idx[0];
+            const double xo = A.xd[oo];
+            double d = A.xd[(size_t)c * n + idx[1]] - xo;
+            b0[c] = d - box * rint(d * ibox);
+            d = A.xd[(size_t)c * n + idx[2]] - xo;
+            c0[c] = d - box * rint(d * ibox);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const size_t o = (size_t)c * n + idx[a];
+                const double f = A.fnb[o] + (double)own[a][c] * FIX_INV;
+                const double step = A.dt * A.vd[o] + A.dtsq * im[a] * f;
+                const double base = a == 0 ? 0.0 : (a == 1 ? b0[c] : c0[c]);
+                x1[a][c] = base + step;
+            }
+        }
+        // canonical triangle: O at ra above the centre of mass, the H-H midpoint rb below
+        const double wt = mo + 2.0 * mh, rc = 0.5 * dhh;
+        const double hgt = sqrt(doh * doh - rc * rc);
+        const double ra = 2.0 * mh * hgt / wt, rb = hgt - ra;
+        double com[3], a1[3], b1[3], c1[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            com[c] = (mo * x1[0][c] + mh * (x1[1][c] + x1[2][c])) / wt;
+            a1[c] = x1[0][c] - com[c];
+            b1[c] = x1[1][c] - com[c];
+            c1[c] = x1[2][c] - com[c];
+        }
+        auto cross = [](const double *u, const double *v, double *o) {
+            o[0] = u[1] * v[2] - u[2] * v[1];
+            o[1] = u[2] * v[0] - u[0] * v[2];
+            o[2] = u[0] * v[1] - u[1] * v[0];
+        };
+        double T[3][3];   // rows: the frame axes x', y', z'
+        cross(b0, c0, T[2]);
+        cross(a1, T[2], T[0]);
+        cross(T[2], T[0], T[1]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double inv = 1.0 / sqrt(T[k][0] * T[k][0] + T[k][1] * T[k][1] + T[k][2] * T[k][2]);
+            T[k][0] *= inv; T[k][1] *= inv; T[k][2] *= inv;
+        }
+        auto to = [&](const double *v, int k) { return T[k][0] * v[0] + T[k][1] * v[1] + T[k][2] * v[2]; };
+        const double xb0 = to(b0, 0), yb0 = to(b0, 1), xc0 = to(c0, 0), yc0 = to(c0, 1);
+        const double za1 = to(a1, 2);
+        const double xb1 = to(b1, 0), yb1 = to(b1, 1), zb1 = to(b1, 2);
+        const double xc1 = to(c1, 0), yc1 = to(c1, 1), zc1 = to(c1, 2);
+        const double sinphi = za1 / ra;
+        const double cosphi = sqrt(1.0 - sinphi * sinphi);
+        const double sinpsi = (zb1 - zc1) / (2.0 * rc * cosphi);
+        const double cospsi = sqrt(1.0 - sinpsi * sinpsi);
+        const double ya2 = ra * cosphi, xb2 = -rc * cospsi;
+        const double t1 = -rb * cosphi, t2 = rc * sinpsi * sinphi;
+        const double yb2 = t1 - t2, yc2 = t1 + t2;
+        const double alpha = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2;
+        const double beta = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
+        const double gamma = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
+        const double ab = alpha * alpha + beta * beta;
+        const double sint = (alpha * gamma - beta * sqrt(ab - gamma * gamma)) / ab;
+        const double cost = sqrt(1.0 - sint * sint);
+        const double p3[3][3] = {{-ya2 * sint, ya2 * cost, za1},
+                                 {xb2 * cost - yb2 * sint, xb2 * sint + yb2 * cost, zb1},
+                                 {-xb2 * cost - yc2 * sint, -xb2 * sint + yc2 * cost, zc1}};
+        const double ms[3] = {mo, mh, mh};
+        long long add[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            long long sum_h = 0;
+#pragma unroll
+            for (int a = 1; a < 3; ++a) {
+                const double xnew = T[0][c] * p3[a][0] + T[1][c] * p3[a][1] + T[2][c] * p3[a][2] + com[c];
+                const double f = ms[a] * (xnew - x1[a][c]) / A.dtsq;
+                add[a][c] = to_fix(f);
+                sum_h += add[a][c];
+                vir[c] += (a == 1 ? b0[c] : c0[c]) * f;
+            }
+            add[0][c] = -sum_h;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                long long *p = &A.fix[(size_t)c * n + idx[a]];
+                if (fused) *p = own[a][c] + add[a][c];
+                else atomicAdd((unsigned long long *)p, (unsigned long long)add[a][c]);
+            }
+    }
+    return rr;
+}
+
+// All engine constraints in one launch over kidx, the cluster indices sorted by solver
+// (kinds 1, 3, 4, -3, then the SETTLE waters; e1..e4 are the ends of the first four
+// groups), so that a wavefront almost always runs a single solver.
 __global__ void __launch_bounds__(128)
-k_shake64(int32_t ns, const int32_t *__restrict__ num, ShakeArgs A, long long *__restrict__ eacc) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = m < ns;
+k_constraints(int32_t count, const int32_t *__restrict__ kidx, int32_t e1, int32_t e2, int32_t e3,
+              int32_t e4, ShakeArgs A, long long *__restrict__ eacc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < count;
     double vir[3] = {0.0, 0.0, 0.0};
     long long eb = 0, ea = 0;
     int rr = 0;
     if (live) {
-        switch (num[m]) {   // kinds as in make_cluster: anything but 1, 3, 4 is the triangle
-            case 1: rr = shake_cluster<1>(m, A, vir, eb, ea); break;
-            case 3: rr = shake_cluster<3>(m, A, vir, eb, ea); break;
-            case 4: rr = shake_cluster<4>(m, A, vir, eb, ea); break;
-            default: rr = shake_cluster<-3>(m, A, vir, eb, ea); break;
-        }
+        const int m = kidx[t];
+        if (t >= e4) rr = settle_water(m, A, vir, eb, ea);
+        else if (t >= e3) rr = shake_cluster<-3>(m, A, vir, eb, ea);
+        else if (t >= e2) rr = shake_cluster<4>(m, A, vir, eb, ea);
+        else if (t >= e1) rr = shake_cluster<3>(m, A, vir, eb, ea);
+        else rr = shake_cluster<1>(m, A, vir, eb, ea);
     }
     wave_add(to_efix(vir[0]), live, rr, eacc, V_X);
     wave_add(to_efix(vir[1]), live, rr, eacc, V_Y);
@@ -803,8 +930,10 @@ ShakeArgs shake_args(sofia_ctx *c) {
 
 void launch_shake64(sofia_ctx *c) {
     if (!c->ns) return;
-    hipLaunchKernelGGL(k_shake64, dim3(nblocks(c->ns, 128)), dim3(128), 0, c->stream, c->ns,
-                       c->shake_num, shake_args(c), c->eacc);
+    const int32_t *n = c->shake_kcount;
+    const int e1 = n[0], e2 = e1 + n[1], e3 = e2 + n[2], e4 = e3 + n[3], total = e4 + n[4];
+    hipLaunchKernelGGL(k_constraints, dim3(nblocks(total, 128)), dim3(128), 0, c->stream, total,
+                       c->shake_kidx, e1, e2, e3, e4, shake_args(c), c->eacc);
 }
 
 void launch_copy16(sofia_ctx *c, size_t n16, const void *src, void *dst) {

@@ -512,7 +512,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->aslot); dfree(c->perm); dfree(c->tt); dfree(c->atombin); dfree(c->sorted_ids);
     dfree(c->nlist); dfree(c->numneigh); dfree(c->maxcnt); dfree(c->rebuild); dfree(c->nbuilds);
     dfree(c->shake_err); dfree(c->shake_iters); dfree(c->water); dfree(c->dsf_tab); dfree(c->pfix); dfree(c->ebond); dfree(c->eangle); dfree(c->bondlist); dfree(c->anglelist);
-    dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms);
+    dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_kidx);
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
     dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->improlist); dfree(c->impro_k);
@@ -705,6 +705,8 @@ int sofia_set_shake(sofia_ctx *c, int32_t ns, const int32_t *ns_rep, const int32
     if (!c || ns < 0) return fail(SOFIA_E_ARG, "bad argument");
     TRY(unsplit_water(c));
     dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_bond); dfree(c->water);
+    dfree(c->shake_kidx);
+    for (int q = 0; q < 5; ++q) c->shake_kcount[q] = 0;
     c->ns = 0;
     if (!ns) return SOFIA_OK;
     for (int k = 0; k < ns; ++k)
@@ -1284,9 +1286,48 @@ int engine_regrid(sofia_ctx *c) {
 
 // Build every replica's list with 50 % headroom (so an MD segment never has to regrow it)
 // and reset the engine's counters and flags.
+// Engine SHAKE launches: one per cluster kind (1, 3, 4, -3), each over its own index list so
+// that every kind gets the register budget of its own solver, and a fifth list of the -3
+// clusters that are symmetric rigid waters (|O-H1| = |O-H2|, equal H masses), which SETTLE
+// solves in closed form.  Needs the atom types, so it runs when the engine sets up.
+int classify_shake(sofia_ctx *c) {
+    dfree(c->shake_kidx);
+    for (int q = 0; q < 5; ++q) c->shake_kcount[q] = 0;
+    if (!c->ns) return SOFIA_OK;
+    TRY(sync_check(c));
+    const int ns = c->ns;
+    std::vector<int32_t> num(ns), at((size_t)4 * ns);
+    std::vector<double> bond((size_t)3 * ns), mass(c->p.ntypes);
+    std::vector<float4> pos(c->N);
+    TRY(download(num.data(), c->shake_num, sizeof(int32_t) * ns));
+    TRY(download(at.data(), c->shake_atoms, sizeof(int32_t) * at.size()));
+    TRY(download(bond.data(), c->shake_bond, sizeof(double) * bond.size()));
+    TRY(download(mass.data(), c->mass, sizeof(double) * mass.size()));
+    TRY(download(pos.data(), c->a.pos4, sizeof(float4) * pos.size()));
+    auto m_of = [&](int i) { return mass[__builtin_bit_cast(int, pos[i].w) - 1]; };
+    std::vector<int32_t> lists[5];
+    for (int k = 0; k < ns; ++k) {
+        int q = num[k] == 1 ? 0 : num[k] == 3 ? 1 : num[k] == 4 ? 2 : 3;
+        if (q == 3 && bond[3 * (size_t)k] == bond[3 * (size_t)k + 1] &&
+            m_of(at[4 * (size_t)k + 1]) == m_of(at[4 * (size_t)k + 2]) &&
+            bond[3 * (size_t)k + 2] < 2.0 * bond[3 * (size_t)k])
+            q = 4;
+        lists[q].push_back(k);
+    }
+    std::vector<int32_t> kidx;
+    for (int q = 0; q < 5; ++q) {
+        kidx.insert(kidx.end(), lists[q].begin(), lists[q].end());
+        c->shake_kcount[q] = (int32_t)lists[q].size();
+    }
+    TRY(dalloc(&c->shake_kidx, kidx.size()));
+    TRY(upload(c->shake_kidx, kidx.data(), sizeof(int32_t) * kidx.size()));
+    return SOFIA_OK;
+}
+
 int engine_lists(sofia_ctx *c) {
     TRY(engine_regrid(c));
     TRY(split_water(c));
+    TRY(classify_shake(c));
     launch_set_rebuild(c, 1);
     TRY(rebuild_now(c, true, false));
     std::vector<int32_t> nn(c->N);

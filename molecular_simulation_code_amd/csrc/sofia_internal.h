@@ -122,6 +122,10 @@ struct sofia_ctx {
     int32_t *torsions = nullptr, ntor = 0;
     int32_t *improlist = nullptr, nimp = 0;   // (i1, i2, i3 central, i4, type) per improper
     int32_t *shake_num = nullptr, *shake_atoms = nullptr, ns = 0;
+    // engine SHAKE launches: cluster indices grouped by kind 1, 3, 4, -3 (iterative) and
+    // symmetric rigid waters (SETTLE); built when the engine sets up its lists
+    int32_t *shake_kidx = nullptr;
+    int32_t shake_kcount[5] = {0, 0, 0, 0, 0};
     double *shake_bond = nullptr;
     int4 *water = nullptr;       // per SHAKE cluster: rigid-water bond/angle types + fused flag
     bool water_split = false;    // water bonds/angles moved into the SHAKE kernel

@@ -477,3 +477,49 @@ class Ref:
         xyz = np.ascontiguousarray(pos, _f).copy()
         self.L.ref_pbc(P(xyz))
         return xyz
+
+
+def settle(x0, x1, m_o, m_h, d_oh, d_hh):
+    """Rigid 3-site water by SETTLE (Miyamoto & Kollman, J. Comput. Chem. 13, 952, 1992), FP64.
+
+    x0: (n, 3, 3) constrained positions at t (O, H1, H2), x1: (n, 3, 3) unconstrained
+    positions at t + dt.  Returns the positions at t + dt that satisfy |O-H| = d_oh and
+    |H-H| = d_hh with displacements m_i (x - x1)_i along the old bonds: the solution that
+    SHAKE (Source/mod_shake.f90 shake3angle) iterates towards, in closed form.  Checker
+    for the engine's water constraint path; not a restatement of a reference routine.
+    """
+    x0 = np.asarray(x0, _d); x1 = np.asarray(x1, _d)
+    w = m_o + 2.0 * m_h
+    rc = 0.5 * d_hh
+    h = np.sqrt(d_oh * d_oh - rc * rc)
+    ra = 2.0 * m_h * h / w
+    rb = h - ra
+    com = (m_o * x1[:, 0] + m_h * (x1[:, 1] + x1[:, 2])) / w
+    a1, b1, c1 = x1[:, 0] - com, x1[:, 1] - com, x1[:, 2] - com
+    b0, c0 = x0[:, 1] - x0[:, 0], x0[:, 2] - x0[:, 0]
+    zk = np.cross(b0, c0)
+    xk = np.cross(a1, zk)
+    yk = np.cross(zk, xk)
+    T = np.stack([xk / np.linalg.norm(xk, axis=1)[:, None], yk / np.linalg.norm(yk, axis=1)[:, None],
+                  zk / np.linalg.norm(zk, axis=1)[:, None]], axis=1)        # rows: frame axes
+    p = lambda v: np.einsum("nij,nj->ni", T, v)
+    b0d, c0d, a1d, b1d, c1d = p(b0), p(c0), p(a1), p(b1), p(c1)
+    sinphi = a1d[:, 2] / ra
+    cosphi = np.sqrt(1.0 - sinphi ** 2)
+    sinpsi = (b1d[:, 2] - c1d[:, 2]) / (2.0 * rc * cosphi)
+    cospsi = np.sqrt(1.0 - sinpsi ** 2)
+    ya2 = ra * cosphi
+    xb2 = -rc * cospsi
+    t1, t2 = -rb * cosphi, rc * sinpsi * sinphi
+    yb2, yc2 = t1 - t2, t1 + t2
+    alpha = xb2 * (b0d[:, 0] - c0d[:, 0]) + b0d[:, 1] * yb2 + c0d[:, 1] * yc2
+    beta = xb2 * (c0d[:, 1] - b0d[:, 1]) + b0d[:, 0] * yb2 + c0d[:, 0] * yc2
+    gamma = b0d[:, 0] * b1d[:, 1] - b1d[:, 0] * b0d[:, 1] + c0d[:, 0] * c1d[:, 1] - c1d[:, 0] * c0d[:, 1]
+    ab = alpha ** 2 + beta ** 2
+    sint = (alpha * gamma - beta * np.sqrt(ab - gamma ** 2)) / ab
+    cost = np.sqrt(1.0 - sint ** 2)
+    a3 = np.stack([-ya2 * sint, ya2 * cost, a1d[:, 2]], axis=1)
+    b3 = np.stack([xb2 * cost - yb2 * sint, xb2 * sint + yb2 * cost, b1d[:, 2]], axis=1)
+    c3 = np.stack([-xb2 * cost - yc2 * sint, -xb2 * sint + yc2 * cost, c1d[:, 2]], axis=1)
+    back = lambda v: np.einsum("nji,nj->ni", T, v) + com
+    return np.stack([back(a3), back(b3), back(c3)], axis=1)

@@ -547,3 +547,53 @@ def test_improper_cvff_closed_form(n):
                 g = (O.improper_cvff(il, xp, np.ones(4, np.int32), *args)[1]
                      - O.improper_cvff(il, xm, np.ones(4, np.int32), *args)[1]) / dx
                 assert f[atom, c] == pytest.approx(-g, rel=5e-3, abs=5e-3)
+
+
+def test_settle_is_the_converged_shake_solution():
+    """O.settle (closed form) against the mathematics of the constraint problem and the
+    reference SHAKE (shake3angle) iterated to a tight tolerance on the same inputs."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(3)
+    n = 64
+    mo, mh, doh, dhh = 15.999, 1.008, 1.0, 1.633
+    th = np.arccos(1 - dhh ** 2 / (2 * doh ** 2))
+    geo = np.array([[0, 0, 0], [doh, 0, 0], [doh * np.cos(th), doh * np.sin(th), 0]])
+    R = Rotation.random(n, random_state=4).as_matrix()
+    x0 = np.einsum("nij,kj->nki", R, geo) + rng.uniform(5, 25, (n, 1, 3))
+    x1 = x0 + rng.normal(0, 0.02, x0.shape)
+    x = O.settle(x0, x1, mo, mh, doh, dhh)
+    d = lambda a, b: np.linalg.norm(x[:, a] - x[:, b], axis=1)
+    assert np.abs(d(0, 1) - doh).max() < 1e-12 and np.abs(d(0, 2) - doh).max() < 1e-12
+    assert np.abs(d(1, 2) - dhh).max() < 1e-12
+    m = np.array([mo, mh, mh])
+    D = (x - x1) * m[None, :, None]
+    assert np.abs(D.sum(1)).max() < 1e-11
+    r0, r1, r2 = x0[:, 0] - x0[:, 1], x0[:, 0] - x0[:, 2], x0[:, 1] - x0[:, 2]
+    for i in range(n):
+        A = np.zeros((9, 3))
+        A[0:3, 0], A[0:3, 1] = r0[i], r1[i]
+        A[3:6, 0], A[3:6, 2] = -r0[i], r2[i]
+        A[6:9, 1], A[6:9, 2] = -r1[i], -r2[i]
+        lam = np.linalg.lstsq(A, D[i].reshape(9), rcond=None)[0]
+        assert np.abs(A @ lam - D[i].reshape(9)).max() < 1e-10 * np.abs(D[i]).max()
+    # the reference's iterative SHAKE on the same step (zero force, v = (x1 - x0) / dt)
+    dt, k2a = np.float32(2.0), S.KCAL2AMU
+    pos = x0.reshape(-1, 3).astype(np.float32)
+    v = ((x1 - x0) / float(dt)).reshape(-1, 3).astype(np.float32)
+    types = np.tile(np.array([11, 12], np.int32)[[0, 1, 1]], n)
+    mass = np.zeros(S.NTYPES); mass[10], mass[11] = mo, mh
+    atoms = np.stack([np.arange(n) * 3 + 1, np.arange(n) * 3 + 2, np.arange(n) * 3 + 3,
+                      np.zeros(n, int)], axis=1).astype(np.int32)
+    bonds = np.tile([doh, doh, dhh], (n, 1))
+    ff, _, st, _ = O.shake(1, pos, types, v, np.zeros_like(pos), mass, dt, k2a, np.float32(60.0),
+                           np.float32(1.0) / np.float32(60.0), np.full(n, -3, np.int32), atoms,
+                           bonds, 1e-10, 200)
+    assert st == 0
+    # settle's constraint force on the inputs the oracle saw: shake_update forms x + dt v in
+    # real*4 (Source/mod_shake.f90), so the prediction is the float sum
+    x0f = pos.reshape(n, 3, 3).astype(np.float64)
+    x1f = (pos + dt * v).astype(np.float64).reshape(n, 3, 3)
+    xs = O.settle(x0f, x1f, mo, mh, doh, dhh)
+    dtsq = float(np.float32(float(dt) * float(dt)) * np.float32(k2a))
+    fs = ((xs - x1f) * m[None, :, None] / dtsq).reshape(-1, 3)
+    assert np.sqrt(((ff - fs) ** 2).sum() / (fs ** 2).sum()) < 1e-5
