@@ -198,15 +198,29 @@ __global__ void k_copyback(int32_t n, const int32_t *__restrict__ rep,
     }
 }
 
-// Re-index a bonded list through tt: `width` ints per item, the first `nidx` are atom ids
-// (-1 = unused slot).  Unflagged replicas have tt = identity.
-__global__ void k_remap(int32_t items, int32_t width, int32_t nidx, int32_t *__restrict__ lst,
-                        const int32_t *__restrict__ tt) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= items) return;
-    for (int k = 0; k < nidx; ++k) {
-        const int v = lst[(size_t)m * width + k];
-        if (v >= 0) lst[(size_t)m * width + k] = tt[v];
+// Every bonded list in one launch: item t of the concatenation falls in list L of S, whose
+// items are `width` ints with the first `nidx` atom ids (-1 = unused).  Items of replicas
+// that rebuild nothing this step are skipped (their tt is the identity).
+struct RemapSet {
+    int32_t *lst[8];
+    int32_t end[8];           // cumulative item counts
+    int32_t width[8], nidx[8];
+    int32_t nlists;
+};
+
+__global__ void k_remap_all(RemapSet S, const int32_t *__restrict__ tt, const int32_t *__restrict__ rep,
+                            const int32_t *__restrict__ rebuild) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (S.nlists == 0 || t >= S.end[S.nlists - 1]) return;
+    int L = 0;
+    while (t >= S.end[L]) ++L;
+    const int m = t - (L ? S.end[L - 1] : 0);
+    int32_t *item = S.lst[L] + (size_t)m * S.width[L];
+    const int a0 = item[0];
+    if (a0 < 0 || !rebuild[rep[a0]]) return;
+    for (int k = 0; k < S.nidx[L]; ++k) {
+        const int v = item[k];
+        if (v >= 0) item[k] = tt[v];
     }
 }
 
@@ -386,14 +400,28 @@ void launch_gather(sofia_ctx *c, bool engine) {
 }
 
 void launch_remap_topology(sofia_ctx *c) {
-    if (c->nb) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nb, 256)), dim3(256), 0, c->stream, c->nb, 3, 2, c->bondlist, c->tt);
-    if (c->na) hipLaunchKernelGGL(k_remap, dim3(nblk(c->na, 256)), dim3(256), 0, c->stream, c->na, 4, 3, c->anglelist, c->tt);
-    if (c->nd) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nd, 256)), dim3(256), 0, c->stream, c->nd, 5, 4, c->dihedlist, c->tt);
-    if (c->ntor) hipLaunchKernelGGL(k_remap, dim3(nblk(c->ntor, 256)), dim3(256), 0, c->stream, c->ntor, 5, 4, c->torsions, c->tt);
-    if (c->nimp) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nimp, 256)), dim3(256), 0, c->stream, c->nimp, 5, 4, c->improlist, c->tt);
-    if (c->ns) hipLaunchKernelGGL(k_remap, dim3(nblk(c->ns, 256)), dim3(256), 0, c->stream, c->ns, 4, 4, c->shake_atoms, c->tt);
-    if (c->nbe) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nbe, 256)), dim3(256), 0, c->stream, c->nbe, 3, 2, c->ebond, c->tt);
-    if (c->nae) hipLaunchKernelGGL(k_remap, dim3(nblk(c->nae, 256)), dim3(256), 0, c->stream, c->nae, 4, 3, c->eangle, c->tt);
+    RemapSet S{};
+    int total = 0;
+    auto add = [&](int32_t *lst, int items, int width, int nidx) {
+        if (!items) return;
+        S.lst[S.nlists] = lst;
+        total += items;
+        S.end[S.nlists] = total;
+        S.width[S.nlists] = width;
+        S.nidx[S.nlists] = nidx;
+        ++S.nlists;
+    };
+    add(c->bondlist, c->nb, 3, 2);
+    add(c->anglelist, c->na, 4, 3);
+    add(c->dihedlist, c->nd, 5, 4);
+    add(c->torsions, c->ntor, 5, 4);
+    add(c->improlist, c->nimp, 5, 4);
+    add(c->shake_atoms, c->ns, 4, 4);
+    add(c->ebond, c->nbe, 3, 2);
+    add(c->eangle, c->nae, 4, 3);
+    if (total)
+        hipLaunchKernelGGL(k_remap_all, dim3(nblk(total, 256)), dim3(256), 0, c->stream, S, c->tt,
+                           c->rep, c->rebuild);
 }
 
 void launch_build_list(sofia_ctx *c) {
