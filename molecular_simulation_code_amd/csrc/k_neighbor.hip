@@ -59,11 +59,9 @@ __global__ void k_bin(int32_t n, const float4 *__restrict__ pos4, const int32_t 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int r = rep[i];
-    if (!rebuild[r]) {
-        perm[i] = i;
-        tt[i] = i;
-        return;
-    }
+    // perm and tt of a replica that rebuilds nothing are stale: every reader (scatter, cell
+    // sort, gather, copy-back, topology remap) skips such replicas
+    if (!rebuild[r]) return;
     const RepInfo R = reps[r];
     const int c = R.cell0 + xyz2bin_dev(pos4[i], R);
     acell[i] = c;
@@ -200,7 +198,7 @@ __global__ void k_copyback(int32_t n, const int32_t *__restrict__ rep,
 
 // Every bonded list in one launch: item t of the concatenation falls in list L of S, whose
 // items are `width` ints with the first `nidx` atom ids (-1 = unused).  Items of replicas
-// that rebuild nothing this step are skipped (their tt is the identity).
+// that rebuild nothing this step are skipped (their atoms keep their places).
 struct RemapSet {
     int32_t *lst[8];
     int32_t end[8];           // cumulative item counts
