@@ -743,24 +743,34 @@ __device__ inline int settle_water(int m, const ShakeArgs &A, double vir[3], lon
     return rr;
 }
 
-// All engine constraints in one launch over kidx, the cluster indices sorted by solver
-// (kinds 1, 3, 4, -3, then the SETTLE waters; e1..e4 are the ends of the first four
-// groups), so that a wavefront almost always runs a single solver.
+// All engine constraints in one launch.  The first nb_it blocks run the iteratively solved
+// clusters of kidx (kinds 1, 3, 4, -3 in that order; e1..e4 are the ends of the groups), so
+// that a wavefront almost always runs a single solver; the other blocks run over the atoms
+// in their current (cell-sorted) order, each oxygen of a SETTLE water solving its water, so
+// that a wavefront's waters come from one or two cells and share cache lines.
 __global__ void __launch_bounds__(128)
-k_constraints(int32_t count, const int32_t *__restrict__ kidx, int32_t e1, int32_t e2, int32_t e3,
-              int32_t e4, ShakeArgs A, long long *__restrict__ eacc) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = t < count;
+k_constraints(const int32_t *__restrict__ kidx, int32_t e1, int32_t e2, int32_t e3, int32_t e4,
+              int32_t nb_it, const int32_t *__restrict__ wclus, ShakeArgs A,
+              long long *__restrict__ eacc) {
     double vir[3] = {0.0, 0.0, 0.0};
     long long eb = 0, ea = 0;
     int rr = 0;
-    if (live) {
-        const int m = kidx[t];
-        if (t >= e4) rr = settle_water(m, A, vir, eb, ea);
-        else if (t >= e3) rr = shake_cluster<-3>(m, A, vir, eb, ea);
-        else if (t >= e2) rr = shake_cluster<4>(m, A, vir, eb, ea);
-        else if (t >= e1) rr = shake_cluster<3>(m, A, vir, eb, ea);
-        else rr = shake_cluster<1>(m, A, vir, eb, ea);
+    bool live;
+    if ((int)blockIdx.x < nb_it) {
+        const int t = blockIdx.x * blockDim.x + threadIdx.x;
+        live = t < e4;
+        if (live) {
+            const int m = kidx[t];
+            if (t >= e3) rr = shake_cluster<-3>(m, A, vir, eb, ea);
+            else if (t >= e2) rr = shake_cluster<4>(m, A, vir, eb, ea);
+            else if (t >= e1) rr = shake_cluster<3>(m, A, vir, eb, ea);
+            else rr = shake_cluster<1>(m, A, vir, eb, ea);
+        }
+    } else {
+        const int i = (blockIdx.x - nb_it) * blockDim.x + threadIdx.x;
+        const int m = i < A.n ? wclus[i] : -1;
+        live = m >= 0;
+        if (live) rr = settle_water(m, A, vir, eb, ea);
     }
     wave_add(to_efix(vir[0]), live, rr, eacc, V_X);
     wave_add(to_efix(vir[1]), live, rr, eacc, V_Y);
@@ -931,9 +941,12 @@ ShakeArgs shake_args(sofia_ctx *c) {
 void launch_shake64(sofia_ctx *c) {
     if (!c->ns) return;
     const int32_t *n = c->shake_kcount;
-    const int e1 = n[0], e2 = e1 + n[1], e3 = e2 + n[2], e4 = e3 + n[3], total = e4 + n[4];
-    hipLaunchKernelGGL(k_constraints, dim3(nblocks(total, 128)), dim3(128), 0, c->stream, total,
-                       c->shake_kidx, e1, e2, e3, e4, shake_args(c), c->eacc);
+    const int e1 = n[0], e2 = e1 + n[1], e3 = e2 + n[2], e4 = e3 + n[3];
+    const int nb_it = nblocks(e4, 128);
+    const int nb_w = n[4] ? nblocks(c->N, 128) : 0;
+    if (nb_it + nb_w == 0) return;
+    hipLaunchKernelGGL(k_constraints, dim3(nb_it + nb_w), dim3(128), 0, c->stream, c->shake_kidx,
+                       e1, e2, e3, e4, nb_it, c->a.wclus, shake_args(c), c->eacc);
 }
 
 void launch_copy16(sofia_ctx *c, size_t n16, const void *src, void *dst) {

@@ -149,6 +149,7 @@ __global__ void k_gather(int32_t n, const int32_t *__restrict__ rep,
     dst.num2[k] = src.num2[o];
     dst.num3[k] = src.num3[o];
     dst.mol[k] = src.mol[o];
+    dst.wclus[k] = src.wclus[o];
 #pragma unroll
     for (int l = 0; l < SPEC; ++l) {
         const int s = src.spec[(size_t)o * SPEC + l];
@@ -179,6 +180,7 @@ __global__ void k_copyback(int32_t n, const int32_t *__restrict__ rep,
     dst.num2[k] = src.num2[k];
     dst.num3[k] = src.num3[k];
     dst.mol[k] = src.mol[k];
+    dst.wclus[k] = src.wclus[k];
 #pragma unroll
     for (int l = 0; l < SPEC; ++l) dst.spec[(size_t)k * SPEC + l] = src.spec[(size_t)k * SPEC + l];
 #pragma unroll

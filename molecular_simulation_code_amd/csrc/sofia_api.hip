@@ -35,7 +35,10 @@ int dalloc(T **p, size_t count) {
     *p = nullptr;
     if (count == 0) count = 1;
     hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-    if (e != hipSuccess) return check_hip(e, "hipMalloc");
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   // reported here; keep it out of later status checks
+        return check_hip(e, "hipMalloc");
+    }
     return SOFIA_OK;
 }
 
@@ -63,7 +66,9 @@ int alloc_atoms(sofia::AtomArrays &A, int n) {
     TRY(dalloc(&A.xd, (size_t)3 * n));
     TRY(dalloc(&A.vd, (size_t)3 * n));
     TRY(dalloc(&A.mol, n));
+    TRY(dalloc(&A.wclus, n));
     SOFIA_HIP(hipMemset(A.mol, 0xff, sizeof(int32_t) * n));
+    SOFIA_HIP(hipMemset(A.wclus, 0xff, sizeof(int32_t) * n));
     SOFIA_HIP(hipMemset(A.spec, 0xff, sizeof(int32_t) * (size_t)n * SPEC));
     SOFIA_HIP(hipMemset(A.num1, 0, sizeof(int32_t) * n));
     SOFIA_HIP(hipMemset(A.num2, 0, sizeof(int32_t) * n));
@@ -74,7 +79,7 @@ int alloc_atoms(sofia::AtomArrays &A, int n) {
 
 void free_atoms(sofia::AtomArrays &A) {
     dfree(A.pos4); dfree(A.q); dfree(A.gid); dfree(A.num1); dfree(A.num2); dfree(A.num3);
-    dfree(A.spec); dfree(A.v32); dfree(A.xd); dfree(A.vd); dfree(A.mol);
+    dfree(A.spec); dfree(A.v32); dfree(A.xd); dfree(A.vd); dfree(A.mol); dfree(A.wclus);
 }
 
 // check_allocation's grid, Source/mod_neighbor.f90:1446-1451 (real*4), vol = box**3
@@ -707,6 +712,8 @@ int sofia_set_shake(sofia_ctx *c, int32_t ns, const int32_t *ns_rep, const int32
     dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_bond); dfree(c->water);
     dfree(c->shake_kidx);
     for (int q = 0; q < 5; ++q) c->shake_kcount[q] = 0;
+    TRY(sync_check(c));
+    SOFIA_HIP(hipMemset(c->a.wclus, 0xff, sizeof(int32_t) * c->N));   // no SETTLE waters yet
     c->ns = 0;
     if (!ns) return SOFIA_OK;
     for (int k = 0; k < ns; ++k)
@@ -1286,10 +1293,12 @@ int engine_regrid(sofia_ctx *c) {
 
 // Build every replica's list with 50 % headroom (so an MD segment never has to regrow it)
 // and reset the engine's counters and flags.
-// Engine SHAKE launches: one per cluster kind (1, 3, 4, -3), each over its own index list so
-// that every kind gets the register budget of its own solver, and a fifth list of the -3
-// clusters that are symmetric rigid waters (|O-H1| = |O-H2|, equal H masses), which SETTLE
-// solves in closed form.  Needs the atom types, so it runs when the engine sets up.
+// Engine constraint work: the clusters solved iteratively (kinds 1, 3, 4, -3), their indices
+// grouped by kind so that a wavefront almost always runs one solver, and the -3 clusters that
+// are symmetric rigid waters (|O-H1| = |O-H2|, equal H masses), which SETTLE solves in closed
+// form.  A SETTLE water is found through its oxygen (AtomArrays::wclus, permuted with the
+// atoms), so its solver runs in atom order, next to the waters of the same cell.  Needs the
+// atom types, so it runs when the engine sets up.
 int classify_shake(sofia_ctx *c) {
     dfree(c->shake_kidx);
     for (int q = 0; q < 5; ++q) c->shake_kcount[q] = 0;
@@ -1316,12 +1325,16 @@ int classify_shake(sofia_ctx *c) {
     }
     std::vector<int32_t> kidx;
     for (int q = 0; q < 5; ++q) {
-        kidx.insert(kidx.end(), lists[q].begin(), lists[q].end());
+        if (q < 4) kidx.insert(kidx.end(), lists[q].begin(), lists[q].end());
         c->shake_kcount[q] = (int32_t)lists[q].size();
     }
-    TRY(dalloc(&c->shake_kidx, kidx.size()));
-    TRY(upload(c->shake_kidx, kidx.data(), sizeof(int32_t) * kidx.size()));
-    return SOFIA_OK;
+    if (!kidx.empty()) {
+        TRY(dalloc(&c->shake_kidx, kidx.size()));
+        TRY(upload(c->shake_kidx, kidx.data(), sizeof(int32_t) * kidx.size()));
+    }
+    std::vector<int32_t> wclus(c->N, -1);
+    for (int k : lists[4]) wclus[at[4 * (size_t)k]] = k;
+    return upload(c->a.wclus, wclus.data(), sizeof(int32_t) * wclus.size());
 }
 
 int engine_lists(sofia_ctx *c) {

@@ -68,6 +68,7 @@ struct AtomArrays {
     double *xd = nullptr;       // SoA x | y | z (3N)
     double *vd = nullptr;       // SoA (3N)
     int32_t *mol = nullptr;     // molecule id (cluster analysis), -1 = solvent
+    int32_t *wclus = nullptr;   // engine: SETTLE water whose oxygen this atom is, -1 = none
 };
 
 }  // namespace sofia
@@ -122,8 +123,9 @@ struct sofia_ctx {
     int32_t *torsions = nullptr, ntor = 0;
     int32_t *improlist = nullptr, nimp = 0;   // (i1, i2, i3 central, i4, type) per improper
     int32_t *shake_num = nullptr, *shake_atoms = nullptr, ns = 0;
-    // engine SHAKE launches: cluster indices grouped by kind 1, 3, 4, -3 (iterative) and
-    // symmetric rigid waters (SETTLE); built when the engine sets up its lists
+    // engine constraints: indices of the iteratively solved clusters grouped by kind 1, 3,
+    // 4, -3, and the count of symmetric rigid waters (SETTLE, found through AtomArrays::wclus);
+    // built when the engine sets up its lists
     int32_t *shake_kidx = nullptr;
     int32_t shake_kcount[5] = {0, 0, 0, 0, 0};
     double *shake_bond = nullptr;
