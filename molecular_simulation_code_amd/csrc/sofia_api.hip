@@ -475,6 +475,169 @@ int rebuild_now(sofia_ctx *c, bool engine, bool reference_growth) {
 }
 
 }  // namespace
+
+void drop_bonded_bundles(sofia_ctx *c) {
+    dfree(c->bund_tstart); dfree(c->bund_sstart); dfree(c->bund_slots); dfree(c->bund_terms);
+    c->nbundle = 0;
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+}
+
+// The engine's bonded items (1-4 pairs, bonds, angles, torsions, impropers, as
+// launch_bonded_engine walks them) grouped into molecules, the connected components of the
+// atoms they touch, and the molecules of each replica packed in their current atom order
+// into bundles of at most BUNDLE_SLOTS atoms and (when several share one) BUNDLE_TERMS
+// items.  Items and list columns do not change when the atoms are re-sorted, so the bundles
+// hold until the topology changes.  Leaves nbundle = 0 (per-kind kernels) when a molecule is
+// too large or a list holds an atom index out of range.
+int build_bonded_bundles(sofia_ctx *c) {
+    drop_bonded_bundles(c);
+    TRY(sync_check(c));
+    const int n = c->N;
+    struct Src { const int32_t *lst; int items; };
+    Src src[BK_COUNT] = {
+        {c->dihedlist, c->nd},
+        {c->water_split ? c->ebond : c->bondlist, c->bond_k ? (c->water_split ? c->nbe : c->nb) : 0},
+        {c->water_split ? c->eangle : c->anglelist, c->angle_k ? (c->water_split ? c->nae : c->na) : 0},
+        {c->torsions, c->tors_k ? c->ntor : 0},
+        {c->improlist, c->nimp},
+    };
+    std::vector<int32_t> lst[BK_COUNT];
+    for (int k = 0; k < BK_COUNT; ++k) {
+        lst[k].resize((size_t)src[k].items * bonded_width(k));
+        if (src[k].items) TRY(download(lst[k].data(), src[k].lst, sizeof(int32_t) * lst[k].size()));
+    }
+    // the atom columns an item touches
+    auto cols = [](int kind, int *out) -> int {
+        if (kind == BK_PAIR14) { out[0] = 0; out[1] = 3; return 2; }
+        const int m = kind == BK_BOND ? 2 : (kind == BK_ANGLE ? 3 : 4);
+        for (int k = 0; k < m; ++k) out[k] = k;
+        return m;
+    };
+    struct Term { int kind, item; };
+    std::vector<Term> terms;
+    for (int k = 0; k < BK_COUNT; ++k)
+        for (int m = 0; m < src[k].items; ++m) {
+            const int32_t *it = lst[k].data() + (size_t)m * bonded_width(k);
+            if (k == BK_BOND && it[2] < 1) continue;    // rigid-water items of the SHAKE kernel
+            if (k == BK_ANGLE && it[3] < 1) continue;
+            if (m >= (1 << 29)) return SOFIA_OK;
+            int cc[4];
+            const int nc = cols(k, cc);
+            for (int q = 0; q < nc; ++q)
+                if (it[cc[q]] < 0 || it[cc[q]] >= n) return SOFIA_OK;
+            terms.push_back({k, m});
+        }
+    if (terms.empty()) return SOFIA_OK;
+    // molecules: union-find over the touched atoms
+    std::vector<int32_t> parent(n);
+    for (int i = 0; i < n; ++i) parent[i] = i;
+    auto root = [&](int i) {
+        while (parent[i] != i) { parent[i] = parent[parent[i]]; i = parent[i]; }
+        return i;
+    };
+    for (const Term &t : terms) {
+        const int32_t *it = lst[t.kind].data() + (size_t)t.item * bonded_width(t.kind);
+        int cc[4];
+        const int nc = cols(t.kind, cc);
+        for (int q = 1; q < nc; ++q) {
+            const int a = root(it[cc[0]]), b = root(it[cc[q]]);
+            if (a != b) parent[std::max(a, b)] = std::min(a, b);
+        }
+    }
+    // molecules in order of their smallest (current) atom index, items and atoms per molecule
+    std::vector<int32_t> mol_of(n, -1), mol_min;
+    std::vector<std::vector<int32_t>> mterms, matoms;
+    for (size_t k = 0; k < terms.size(); ++k) {
+        const Term &t = terms[k];
+        const int32_t *it = lst[t.kind].data() + (size_t)t.item * bonded_width(t.kind);
+        const int rt = root(it[0]);
+        if (mol_of[rt] < 0) {
+            mol_of[rt] = (int32_t)mterms.size();
+            mterms.emplace_back();
+            matoms.emplace_back();
+            mol_min.push_back(rt);
+        }
+        mterms[mol_of[rt]].push_back((int32_t)k);
+    }
+    std::vector<int32_t> slot_src(n, -1);   // first (kind, offset) that names each atom
+    for (size_t k = 0; k < terms.size(); ++k) {
+        const Term &t = terms[k];
+        const int32_t *it = lst[t.kind].data() + (size_t)t.item * bonded_width(t.kind);
+        int cc[4];
+        const int nc = cols(t.kind, cc);
+        for (int q = 0; q < nc; ++q) {
+            const int a = it[cc[q]];
+            if (slot_src[a] >= 0) continue;
+            const size_t off = (size_t)t.item * bonded_width(t.kind) + cc[q];
+            if (off >= (1u << 29)) return SOFIA_OK;
+            slot_src[a] = (int32_t)(((uint32_t)t.kind << 29) | (uint32_t)off);
+            matoms[mol_of[root(a)]].push_back(a);
+        }
+    }
+    const int nmol = (int)mterms.size();
+    std::vector<int32_t> order(nmol);
+    for (int m = 0; m < nmol; ++m) {
+        order[m] = m;
+        if ((int)matoms[m].size() > BUNDLE_SLOTS) return SOFIA_OK;
+    }
+    auto rep_of = [&](int atom) {
+        int lo = 0, hi = c->nrep - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (c->reps_h[mid].atom0 <= atom) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    };
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return mol_min[a] < mol_min[b]; });
+    // pack; a bundle's items go out sorted by kind (the record's top bits)
+    std::vector<int32_t> tstart{0}, sstart{0}, slots, local(n, -1);
+    std::vector<int2> recs, open_recs;
+    int cur_rep = -1, cur_slots = 0;
+    auto close = [&]() {
+        std::stable_sort(open_recs.begin(), open_recs.end(), [](const int2 &a, const int2 &b) {
+            return ((uint32_t)a.x >> 29) < ((uint32_t)b.x >> 29);
+        });
+        recs.insert(recs.end(), open_recs.begin(), open_recs.end());
+        open_recs.clear();
+        tstart.push_back((int32_t)recs.size());
+        sstart.push_back((int32_t)slots.size());
+        cur_slots = 0;
+    };
+    for (int m : order) {
+        const int r = rep_of(mol_min[m]);
+        const int mt = (int)mterms[m].size(), ms = (int)matoms[m].size();
+        if (!open_recs.empty() && (r != cur_rep || (int)open_recs.size() + mt > BUNDLE_TERMS ||
+                                   cur_slots + ms > BUNDLE_SLOTS))
+            close();
+        cur_rep = r;
+        for (int a : matoms[m]) {
+            local[a] = cur_slots++;
+            slots.push_back(slot_src[a]);
+        }
+        for (int k : mterms[m]) {
+            const Term &t = terms[k];
+            const int32_t *it = lst[t.kind].data() + (size_t)t.item * bonded_width(t.kind);
+            int cc[4];
+            const int nc = cols(t.kind, cc);
+            uint32_t packed = 0;
+            for (int q = 0; q < nc; ++q) packed |= (uint32_t)local[it[cc[q]]] << (8 * cc[q]);
+            open_recs.push_back(int2{(int)(((uint32_t)t.kind << 29) | (uint32_t)t.item), (int)packed});
+        }
+    }
+    if (!open_recs.empty()) close();
+    const int nb = (int)tstart.size() - 1;
+    TRY(dalloc(&c->bund_tstart, tstart.size()));
+    TRY(dalloc(&c->bund_sstart, sstart.size()));
+    TRY(dalloc(&c->bund_slots, slots.size()));
+    TRY(dalloc(&c->bund_terms, recs.size()));
+    TRY(upload(c->bund_tstart, tstart.data(), sizeof(int32_t) * tstart.size()));
+    TRY(upload(c->bund_sstart, sstart.data(), sizeof(int32_t) * sstart.size()));
+    TRY(upload(c->bund_slots, slots.data(), sizeof(int32_t) * slots.size()));
+    TRY(upload(c->bund_terms, recs.data(), sizeof(int2) * recs.size()));
+    c->nbundle = nb;
+    return SOFIA_OK;
+}
+
 }  // namespace sofia
 
 using namespace sofia;
@@ -516,6 +679,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep); dfree(c->acell);
     dfree(c->aslot); dfree(c->perm); dfree(c->tt); dfree(c->atombin); dfree(c->sorted_ids);
     dfree(c->nlist); dfree(c->numneigh); dfree(c->maxcnt); dfree(c->rebuild); dfree(c->nbuilds);
+    dfree(c->bund_tstart); dfree(c->bund_sstart); dfree(c->bund_slots); dfree(c->bund_terms);
     dfree(c->shake_err); dfree(c->shake_iters); dfree(c->water); dfree(c->dsf_tab); dfree(c->pfix); dfree(c->ebond); dfree(c->eangle); dfree(c->bondlist); dfree(c->anglelist);
     dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_kidx);
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
@@ -587,6 +751,7 @@ int sofia_set_bonded(sofia_ctx *c, const int32_t *bl, int32_t nb, const int32_t 
                      int32_t nd, const int32_t *nd_rep) {
     if (!c || nb < 0 || na < 0 || nd < 0) return fail(SOFIA_E_ARG, "bad argument");
     TRY(unsplit_water(c));
+    drop_bonded_bundles(c);
     std::vector<int32_t> g;
     dfree(c->bondlist); dfree(c->anglelist); dfree(c->dihedlist);
     c->nb = c->na = c->nd = 0;
@@ -617,6 +782,7 @@ int sofia_set_bonded_coeffs(sofia_ctx *c, int32_t nbt, const double *bk, const d
                             const int32_t *tors, int32_t ntor, const int32_t *ntor_rep) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     TRY(unsplit_water(c));
+    drop_bonded_bundles(c);
     dfree(c->bond_k); dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0);
     dfree(c->tors_k); dfree(c->tors_n); dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->torsions);
     c->ntor = 0;
@@ -664,6 +830,7 @@ int set_impropers(sofia_ctx *c, int style, int32_t nit, const double *k, const d
             if (mult[t] < 0 || mult[t] > 6) return fail(SOFIA_E_ARG, "cvff improper multiplicity must be 0..6");
         }
     TRY(unsplit_water(c));
+    drop_bonded_bundles(c);
     dfree(c->impro_k); dfree(c->impro_chi0); dfree(c->impro_dn); dfree(c->improlist);
     c->nimp = 0;
     c->impro_style = style;
@@ -709,6 +876,7 @@ int sofia_set_shake(sofia_ctx *c, int32_t ns, const int32_t *ns_rep, const int32
                     const int32_t *atoms, const double *bond) {
     if (!c || ns < 0) return fail(SOFIA_E_ARG, "bad argument");
     TRY(unsplit_water(c));
+    drop_bonded_bundles(c);
     dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_bond); dfree(c->water);
     dfree(c->shake_kidx);
     for (int q = 0; q < 5; ++q) c->shake_kcount[q] = 0;
@@ -1353,6 +1521,7 @@ int engine_lists(sofia_ctx *c) {
         launch_set_rebuild(c, 1);
         TRY(rebuild_now(c, true, false));
     }
+    TRY(build_bonded_bundles(c));   // molecules packed in the order the sort just made
     SOFIA_HIP(hipMemsetAsync(c->nbuilds, 0, sizeof(int32_t) * c->nrep, c->stream));
     SOFIA_HIP(hipMemsetAsync(c->shake_err, 0, sizeof(int32_t), c->stream));
     SOFIA_HIP(hipMemsetAsync(c->maxcnt, 0, sizeof(int32_t), c->stream));

@@ -57,6 +57,15 @@ struct DevParams {
 
 constexpr int DSF_TAB_BINS = 1024;
 
+// Engine bonded work in bundles of whole molecules: item kinds and the widths of their lists.
+enum BondedKind { BK_PAIR14 = 0, BK_BOND = 1, BK_ANGLE = 2, BK_TORSION = 3, BK_IMPRO = 4, BK_COUNT = 5 };
+constexpr int BUNDLE_SLOTS = 192;   // atoms per bundle (its forces sit in LDS)
+constexpr int BUNDLE_TERMS = 256;   // items per bundle when molecules are packed together
+constexpr int BUNDLE_THREADS = 256; // one workgroup per bundle
+__host__ __device__ inline int bonded_width(int kind) {
+    return kind == BK_BOND ? 3 : (kind == BK_ANGLE ? 4 : 5);
+}
+
 // Per-atom arrays that the cell sort permutes (one set; the context keeps two).
 struct AtomArrays {
     float4 *pos4 = nullptr;     // x, y, z, type (int bits)
@@ -133,6 +142,11 @@ struct sofia_ctx {
     bool water_split = false;    // water bonds/angles moved into the SHAKE kernel
     int32_t *ebond = nullptr, nbe = 0;    // engine bond / angle lists without the fused waters
     int32_t *eangle = nullptr, nae = 0;
+    // engine bonded bundles (k_bonded_bundles): per bundle the ranges of its term and atom-slot
+    // records; nbundle = 0 runs the per-kind kernels instead
+    int32_t nbundle = 0;
+    int32_t *bund_tstart = nullptr, *bund_sstart = nullptr, *bund_slots = nullptr;
+    int2 *bund_terms = nullptr;
 
     // force field
     float4 *ljtab = nullptr;    // (lj1, lj2, lj3, lj4) per type pair
@@ -195,6 +209,8 @@ void launch_count_pairs(sofia_ctx *c, unsigned long long *out);
 void launch_correct(sofia_ctx *c, int which, long long *fout);
 void launch_pair14(sofia_ctx *c, long long *fout);
 void launch_bonded_engine(sofia_ctx *c);
+int build_bonded_bundles(sofia_ctx *c);   // host: molecules of the engine's bonded items
+void drop_bonded_bundles(sofia_ctx *c);
 void launch_dsf_mixed(sofia_ctx *c, bool energies, float *ff32, bool from_xd);
 
 // integration (k_integrate.hip)
