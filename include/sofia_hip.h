@@ -285,6 +285,12 @@ int sofia_set_tuning(sofia_ctx *ctx, int32_t lanes_per_atom, int32_t block, int3
  * only speed (longer lists against more frequent rebuilds), never the trajectory.  Takes
  * effect at once on a running engine (lists are rebuilt). */
 int sofia_set_skin(sofia_ctx *ctx, double skin);
+/* How the engine evaluates its bonded items (1-4 pairs, bonds, angles, torsions, impropers;
+ * no counterpart in the reference, which has one serial loop per kind): 0 = one kernel per
+ * kind, one thread per item, forces added with atomics (the default); 1 = whole molecules
+ * in bundles, one workgroup per bundle summing its forces in LDS.  Both give the same
+ * integer sums, so the mode changes speed only (autotuned). */
+int sofia_set_bonded_mode(sofia_ctx *ctx, int32_t mode);
 /* Union-find clusters of solute molecules joined by an O-N contact < rcontact (A):
  * atoms whose type is in o_types/n_types, molecule id per atom in mol (0-based within a
  * replica, -1 = solvent).  largest[r] = largest cluster (molecules) per replica. */

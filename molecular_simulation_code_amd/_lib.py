@@ -106,6 +106,7 @@ SIGNATURES = {
     "sofia_profile_nonbond": [_vp, _i32, _vp, _vp],
     "sofia_set_tuning": [_vp, _i32, _i32, _i32],
     "sofia_set_skin": [_vp, _d],
+    "sofia_set_bonded_mode": [_vp, _i32],
     "sofia_profile_step": [_vp, _i32, _i32, _vp],
     "sofia_profile_copy": [_vp, C.c_int64, _i32, _vp],
     "sofia_set_molecules": [_vp, _vp],

@@ -578,7 +578,7 @@ void launch_pair14(sofia_ctx *c, long long *fout) {
 
 void launch_bonded_engine(sofia_ctx *c) {
     const int n = c->N;
-    if (c->nbundle > 0) {
+    if (c->bonded_bundles && c->nbundle > 0) {
         BondedArgs A{};
         A.lst[BK_PAIR14] = c->dihedlist;
         A.lst[BK_BOND] = c->water_split ? c->ebond : c->bondlist;

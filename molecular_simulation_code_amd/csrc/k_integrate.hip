@@ -745,20 +745,22 @@ __device__ inline int settle_water(int m, const ShakeArgs &A, double vir[3], lon
 
 // All engine constraints in one launch.  The first nb_it blocks run the iteratively solved
 // clusters of kidx (kinds 1, 3, 4, -3 in that order; e1..e4 are the ends of the groups), so
-// that a wavefront almost always runs a single solver; the other blocks run over the atoms
-// in their current (cell-sorted) order, each oxygen of a SETTLE water solving its water, so
-// that a wavefront's waters come from one or two cells and share cache lines.
+// that a wavefront almost always runs a single solver.  Each wavefront of the other blocks
+// owns WSPAN consecutive atoms in their current (cell-sorted) order: it queues the SETTLE
+// waters whose oxygen lies there in LDS and solves them 64 at a time, so its waters come
+// from a few neighbouring cells and share cache lines, with every lane busy.
+constexpr int WSPAN = 512;
+
 __global__ void __launch_bounds__(128)
 k_constraints(const int32_t *__restrict__ kidx, int32_t e1, int32_t e2, int32_t e3, int32_t e4,
               int32_t nb_it, const int32_t *__restrict__ wclus, ShakeArgs A,
               long long *__restrict__ eacc) {
-    double vir[3] = {0.0, 0.0, 0.0};
-    long long eb = 0, ea = 0;
-    int rr = 0;
-    bool live;
     if ((int)blockIdx.x < nb_it) {
+        double vir[3] = {0.0, 0.0, 0.0};
+        long long eb = 0, ea = 0;
+        int rr = 0;
         const int t = blockIdx.x * blockDim.x + threadIdx.x;
-        live = t < e4;
+        const bool live = t < e4;
         if (live) {
             const int m = kidx[t];
             if (t >= e3) rr = shake_cluster<-3>(m, A, vir, eb, ea);
@@ -766,17 +768,38 @@ k_constraints(const int32_t *__restrict__ kidx, int32_t e1, int32_t e2, int32_t 
             else if (t >= e1) rr = shake_cluster<3>(m, A, vir, eb, ea);
             else rr = shake_cluster<1>(m, A, vir, eb, ea);
         }
-    } else {
-        const int i = (blockIdx.x - nb_it) * blockDim.x + threadIdx.x;
-        const int m = i < A.n ? wclus[i] : -1;
-        live = m >= 0;
-        if (live) rr = settle_water(m, A, vir, eb, ea);
+        wave_add(to_efix(vir[0]), live, rr, eacc, V_X);
+        wave_add(to_efix(vir[1]), live, rr, eacc, V_Y);
+        wave_add(to_efix(vir[2]), live, rr, eacc, V_Z);
+        wave_add(eb, live, rr, eacc, E_BOND);
+        wave_add(ea, live, rr, eacc, E_ANGLE);
+        return;
     }
-    wave_add(to_efix(vir[0]), live, rr, eacc, V_X);
-    wave_add(to_efix(vir[1]), live, rr, eacc, V_Y);
-    wave_add(to_efix(vir[2]), live, rr, eacc, V_Z);
-    wave_add(eb, live, rr, eacc, E_BOND);
-    wave_add(ea, live, rr, eacc, E_ANGLE);
+    __shared__ int32_t s_q[2][WSPAN];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int a0 = ((blockIdx.x - nb_it) * 2 + wave) * WSPAN;
+    int cnt = 0;
+    for (int k = 0; k < WSPAN; k += 64) {
+        const int i = a0 + k + lane;
+        const int m = i < A.n ? wclus[i] : -1;
+        const uint64_t b = __ballot(m >= 0);
+        if (m >= 0) s_q[wave][cnt + __popcll(b & lt_mask)] = m;
+        cnt += __popcll(b);
+    }
+    __builtin_amdgcn_wave_barrier();   // this wavefront's LDS writes precede its reads
+    for (int base = 0; base < cnt; base += 64) {
+        double vir[3] = {0.0, 0.0, 0.0};
+        long long eb = 0, ea = 0;
+        int rr = 0;
+        const bool live = base + lane < cnt;
+        if (live) rr = settle_water(s_q[wave][base + lane], A, vir, eb, ea);
+        wave_add(to_efix(vir[0]), live, rr, eacc, V_X);
+        wave_add(to_efix(vir[1]), live, rr, eacc, V_Y);
+        wave_add(to_efix(vir[2]), live, rr, eacc, V_Z);
+        wave_add(eb, live, rr, eacc, E_BOND);
+        wave_add(ea, live, rr, eacc, E_ANGLE);
+    }
 }
 
 // Plain streaming copy (16 B per thread per iteration): the HBM reference for traffic
@@ -943,7 +966,7 @@ void launch_shake64(sofia_ctx *c) {
     const int32_t *n = c->shake_kcount;
     const int e1 = n[0], e2 = e1 + n[1], e3 = e2 + n[2], e4 = e3 + n[3];
     const int nb_it = nblocks(e4, 128);
-    const int nb_w = n[4] ? nblocks(c->N, 128) : 0;
+    const int nb_w = n[4] ? nblocks(c->N, 2 * WSPAN) : 0;
     if (nb_it + nb_w == 0) return;
     hipLaunchKernelGGL(k_constraints, dim3(nb_it + nb_w), dim3(128), 0, c->stream, c->shake_kidx,
                        e1, e2, e3, e4, nb_it, c->a.wclus, shake_args(c), c->eacc);

@@ -1838,6 +1838,14 @@ int sofia_profile_step(sofia_ctx *c, int32_t nsteps, int32_t full_rebuild, doubl
     return check_engine_flags(c);
 }
 
+int sofia_set_bonded_mode(sofia_ctx *c, int32_t mode) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (mode != 0 && mode != 1) return fail(SOFIA_E_ARG, "bonded mode must be 0 (per-kind kernels) or 1 (molecule bundles)");
+    c->bonded_bundles = mode == 1;
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return SOFIA_OK;
+}
+
 int sofia_set_skin(sofia_ctx *c, double skin) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     if (!(skin > 0.05) || skin > 10.0) return fail(SOFIA_E_ARG, "skin must be in (0.05, 10] A");

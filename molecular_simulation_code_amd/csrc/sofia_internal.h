@@ -145,6 +145,7 @@ struct sofia_ctx {
     // engine bonded bundles (k_bonded_bundles): per bundle the ranges of its term and atom-slot
     // records; nbundle = 0 runs the per-kind kernels instead
     int32_t nbundle = 0;
+    bool bonded_bundles = false;   // run the bundles (tuning choice) rather than per-kind kernels
     int32_t *bund_tstart = nullptr, *bund_sstart = nullptr, *bund_slots = nullptr;
     int2 *bund_terms = nullptr;
 

@@ -472,8 +472,9 @@ class Engine:
         """Pick the Verlet skin and the pair kernel's lanes per atom on the current state.
         For each skin the lanes are chosen by timing the pair kernel alone; skins are
         compared on whole MD steps (list length against rebuild rate); then the list build's
-        waves per workgroup on full rebuilds.  The knobs change speed only.  Returns
-        (skin, lanes, ms/step, list waves)."""
+        waves per workgroup on full rebuilds, and the bonded evaluation (per-kind kernels
+        or molecule bundles) on eager steps.  The knobs change speed only.  Returns
+        (skin, lanes, ms/step, list waves, bonded mode)."""
         import time
         best = None
         self.tune_log = {"pair_ms": {}, "step_ms": {}, "list_ms": {}}
@@ -504,7 +505,19 @@ class Engine:
         self.tune_log["list_ms"] = t_w
         w = min(t_w, key=t_w.get)
         self.set_tuning(best[1], 256, w)
-        return best[0], best[1], best[2], w
+        t_b = {}
+        for mode in (0, 1):
+            self.set_bonded_mode(mode)
+            t_b[mode] = self.profile_step(6)["bonded"]
+        self.tune_log["bonded_ms"] = t_b
+        bmode = min(t_b, key=t_b.get)
+        self.set_bonded_mode(bmode)
+        return best[0], best[1], best[2], w, bmode
+
+    def set_bonded_mode(self, mode: int):
+        """Bonded items by kind (0, one thread per item) or by molecule bundles (1); changes
+        speed only."""
+        check(lib().sofia_set_bonded_mode(self.ctx, int(mode)))
 
     def set_skin(self, skin: float):
         """Engine Verlet skin (A); changes speed only, never the trajectory."""

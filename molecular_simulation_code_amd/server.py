@@ -104,9 +104,11 @@ class SegmentServer:
                 for s in slots]
         eng = Engine(reps, precision=self.precision)
         if self.tuning:
-            skin, lanes, waves = self.tuning
+            skin, lanes, waves = self.tuning[:3]
             eng.set_skin(skin)
             eng.set_tuning(lanes, 256, waves)
+            if len(self.tuning) > 3:
+                eng.set_bonded_mode(self.tuning[3])
         eng.put_state([(s.x, s.v) for s in slots])
         eng.set_steps([s.step for s in slots])
         eng.resume([s.f for s in slots])

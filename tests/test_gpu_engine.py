@@ -232,6 +232,42 @@ def test_skin_changes_speed_not_trajectory():
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
+def test_bonded_bundles_match_per_kind_kernels():
+    """The bonded items by molecule bundles (forces summed in LDS) and by per-kind kernels
+    (global atomics) give the same integer sums: trajectories and energies agree bit for
+    bit, also when the mode is switched mid-run, in both precisions."""
+    host = _engine()
+    specs = [host.ReplicaSpec(_sys(100, 600, 83), temp=300.0, gamma_ps=1.0, seed=93),
+             host.ReplicaSpec(_sys(140, 900, 84), temp=310.0, gamma_ps=1.0, seed=94),
+             host.ReplicaSpec(_sys(90, 700, 85), temp=290.0, gamma_ps=0.5, seed=95)]
+    for prec in (host.PREC_MIXED, host.PREC_FP64):
+        outs = {}
+        for name, plan in (("kinds", [(0, 0)]), ("bundles", [(0, 1)]),
+                           ("switch", [(0, 1), (15, 0), (30, 1)])):
+            say(f"[engine bonded] precision {prec} plan {name}")
+            e = host.Engine(specs, precision=prec)
+            try:
+                e.init()
+                done = 0
+                for at, mode in plan + [(45, None)]:
+                    if at > done:
+                        e.run(at - done)
+                        done = at
+                    if mode is not None:
+                        e.set_bonded_mode(mode)
+                outs[name] = (e.state(), e.energies(), e.forces())
+            finally:
+                e.close()
+        ref = outs["kinds"]
+        for name in ("bundles", "switch"):
+            out = outs[name]
+            for k in range(len(specs)):
+                assert np.array_equal(out[0][k][0], ref[0][k][0]), f"{name}: replica {k} positions differ"
+                assert np.array_equal(out[0][k][1], ref[0][k][1]), f"{name}: replica {k} velocities differ"
+                assert np.array_equal(out[2][k], ref[2][k]), f"{name}: replica {k} forces differ"
+            assert np.array_equal(out[1][:, 2:9], ref[1][:, 2:9]), f"{name}: bonded energies differ"
+
+
 def test_minimize_then_nve_energy_is_stable():
     host = _engine()
     s = _sys(100, 600, 31)
