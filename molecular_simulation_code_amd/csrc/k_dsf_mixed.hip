@@ -10,7 +10,9 @@
 // exactly), so the pair loop has no rint/fma wrap; the integer difference is converted to
 // float and scaled by box 2^-32.  Fixed point also removes the cancellation of FP32
 // coordinates near the box edge: separations carry full FP32 relative precision.  One
-// gather per list entry brings position, type and charge.
+// gather per list entry brings position, type and charge.  When every atom of a type carries
+// the same charge the record drops the charge (12 bytes) and C q_i q_j comes from a
+// per-type-pair table in LDS, formed as the kernel would form it from the charges.
 //
 // Pair math (force over r):
 //   dr6i = (1/r^2)^3, 0 beyond rcut2;  F_LJ/r = dr6i (lj1 dr6i - lj2) / r^2
@@ -48,6 +50,10 @@ __device__ __forceinline__ uint32_t ld32(__amdgpu_buffer_rsrc_t r, uint32_t off)
 __device__ __forceinline__ int4 ld128(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
     return make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+}
+__device__ __forceinline__ int4 ld96(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)off, 0, 0);
+    return make_int4((int)v[0], (int)v[1], (int)v[2], 0);
 }
 __device__ __forceinline__ uint32_t clamp_bytes(size_t b) {
     return b > 0xffffffffull ? 0xffffffffu : (uint32_t)b;
@@ -109,28 +115,42 @@ __global__ void k_fixpos(int32_t n, const double *__restrict__ xd, const float4 
         x = p.x; y = p.y; z = p.z;
     }
     const uint32_t fx = (uint32_t)(unsigned long long)__double2ll_rn(x * s);
-    pf[i] = make_int4((int)((fx & ~TYPE_BITS) | ((uint32_t)__float_as_int(p.w) & TYPE_BITS)),
-                      (int)(uint32_t)(unsigned long long)__double2ll_rn(y * s),
-                      (int)(uint32_t)(unsigned long long)__double2ll_rn(z * s), __float_as_int(q[i]));
+    const int X = (int)((fx & ~TYPE_BITS) | ((uint32_t)__float_as_int(p.w) & TYPE_BITS));
+    const int Y = (int)(uint32_t)(unsigned long long)__double2ll_rn(y * s);
+    const int Z = (int)(uint32_t)(unsigned long long)__double2ll_rn(z * s);
+    if (q) {
+        pf[i] = make_int4(X, Y, Z, __float_as_int(q[i]));
+    } else {   // charges by type: 12-byte records
+        int *p3 = (int *)pf + 3 * (size_t)i;
+        p3[0] = X; p3[1] = Y; p3[2] = Z;
+    }
 }
 
 // L lanes per atom; lane s of a group walks list entries s, s+L, ... with the loads of the
 // next entry in flight during this entry's math.  Blocks are dealt to the XCDs in runs.
-template <int L, bool ENERGY>
+template <int L, bool ENERGY, bool QT>
 __global__ void __launch_bounds__(256)
 k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
             const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh, int32_t na,
-            const float4 *__restrict__ ljtab, const float4 *__restrict__ tab, DevParams P,
+            const float4 *__restrict__ ljtab, const float4 *__restrict__ ptab,
+            const float4 *__restrict__ tab, DevParams P,
             double *__restrict__ fnb, float *__restrict__ ff32, long long *__restrict__ eacc) {
+    // QT: charges by atom type; 12-byte records (x, y, z) and a pair table (lj1, lj2,
+    // C q_i q_j) in place of the charge in the record
     extern __shared__ float4 s_dyn[];
     const int nt2 = P.ntypes * P.ntypes;
     const int tn = P.tab_n;
     float4 *s_lj = s_dyn;
-    float4 *s_tab = s_dyn + nt2;
-    for (int t = threadIdx.x; t < nt2; t += blockDim.x) s_lj[t] = ljtab[t];
+    float4 *s_pt = s_dyn + nt2;
+    float4 *s_tab = s_dyn + (QT ? 2 : 1) * nt2;
+    for (int t = threadIdx.x; t < nt2; t += blockDim.x) {
+        s_lj[t] = ljtab[t];
+        if (QT) s_pt[t] = ptab[t];
+    }
     for (int t = threadIdx.x; t < (ENERGY ? 2 : 1) * (tn + 1); t += blockDim.x) s_tab[t] = tab[t];
     __syncthreads();
+    constexpr uint32_t RB = QT ? 12u : 16u;   // record bytes
 
     const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = gt / L;
@@ -141,10 +161,18 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     // smallest index), so its 32-bit offsets stay small whatever the list size
     const int iw = __builtin_amdgcn_readfirstlane(ii);
     const auto lrs = rsrc(nlist + (size_t)iw * na, clamp_bytes((size_t)(n - iw) * na * 4));
-    const auto prs = rsrc(pf, clamp_bytes((size_t)n * 16));
+    const auto prs = rsrc(pf, clamp_bytes((size_t)n * RB));
 
-    const int4 pi = pf[ii];
-    const float4 *ljrow = s_lj + ((int)((uint32_t)pi.x & TYPE_BITS) - 1) * P.ntypes - 1;   // by jtype
+    int4 pi;
+    if (QT) {
+        const int *p3 = (const int *)pf + 3 * (size_t)ii;
+        pi = make_int4(p3[0], p3[1], p3[2], 0);
+    } else {
+        pi = pf[ii];
+    }
+    const int trow = ((int)((uint32_t)pi.x & TYPE_BITS) - 1) * P.ntypes - 1;   // by jtype
+    const float4 *ljrow = (QT ? s_pt : s_lj) + trow;
+    const float4 *erow = s_lj + trow;
     const int xi = (int)((uint32_t)pi.x & ~TYPE_BITS);
     const int r = rep[ii];
     const float sbox = reps[r].box * TWO_M32;                 // box 2^-32 (exact)
@@ -177,7 +205,7 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        PJ[u] = ld128(prs, JJ[u] * 16u);
+        PJ[u] = QT ? ld96(prs, JJ[u] * RB) : ld128(prs, JJ[u] * RB);
         JJ[u] = ld32(lrs, off + (u + 4) * step);
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -192,7 +220,7 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             const float dy = sep_f(pi.y, pj.y), dz = sep_f(pi.z, pj.z);
             const float dr2 = norm2_f(dx, dy, dz) * s2;
             const float4 lj = ljrow[valid ? tj : 1u];
-            const float qq = qc * __int_as_float(pj.w);
+            const float qq = QT ? lj.z : qc * __int_as_float(pj.w);
             const PairOut o = pair_dsf(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2);
             const float fs = valid ? o.force * fq : 0.0f;
             fxq += (double)__builtin_truncf(dx * fs);
@@ -200,7 +228,8 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             fzq += (double)__builtin_truncf(dz * fs);
             if (ENERGY) {
                 if (valid) {
-                    pot += (double)(o.dr6i * __builtin_fmaf(o.dr6i, lj.z, -lj.w));
+                    const float4 le = erow[tj];
+                    pot += (double)(o.dr6i * __builtin_fmaf(o.dr6i, le.z, -le.w));
                     ecl += (double)(qq * cubic_f(s_etab[o.kb], o.t));
                     const float fv = o.force * s2;
                     vx += (double)(dx * dx * fv);
@@ -211,7 +240,7 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             // entries past the end of the row (their index is whatever follows in the
             // list buffer) gather the atom's own record instead, a cache hit
             const uint32_t jg = (k + (u + 4) * L < nn) ? JJ[u] : (uint32_t)ii;
-            PJ[u] = ld128(prs, jg * 16u);
+            PJ[u] = QT ? ld96(prs, jg * RB) : ld128(prs, jg * RB);
             // likewise list reads past the row re-read its first entry instead of pulling the
             // unused tail of the row from memory
             JJ[u] = ld32(lrs, (k + (u + 8) * L < nn) ? off + u * step : row0);
@@ -392,20 +421,22 @@ k_dsf_compact(int32_t n, const int4 *__restrict__ pf,
 }
 
 template <int L>
-void launch_lanes(sofia_ctx *c, bool energies, float *ff32) {
+void launch_lanes(sofia_ctx *c, bool energies, float *ff32, bool qt) {
     const int block = c->block;
     const long long threads = (long long)c->N * L;
     const int grid = (int)((threads + block - 1) / block);
     const int nt2 = c->dp.ntypes * c->dp.ntypes;
-    const size_t lds = sizeof(float4) * (nt2 + (energies ? 2 : 1) * (c->dp.tab_n + 1));
-    if (energies)
-        hipLaunchKernelGGL((k_dsf_mixed<L, true>), dim3(grid), dim3(block), lds, c->stream, c->N,
-                           c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh,
-                           c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
-    else
-        hipLaunchKernelGGL((k_dsf_mixed<L, false>), dim3(grid), dim3(block), lds, c->stream, c->N,
-                           c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh,
-                           c->neigh_alloc, c->ljtab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc);
+    const size_t lds = sizeof(float4) * ((qt ? 2 : 1) * nt2 + (energies ? 2 : 1) * (c->dp.tab_n + 1));
+#define DSF_LAUNCH(E, Q)                                                                          \
+    hipLaunchKernelGGL((k_dsf_mixed<L, E, Q>), dim3(grid), dim3(block), lds, c->stream, c->N,       \
+                       c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh, c->neigh_alloc,          \
+                       c->ljtab, c->ptab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc)
+    if (energies) {
+        if (qt) DSF_LAUNCH(true, true); else DSF_LAUNCH(true, false);
+    } else {
+        if (qt) DSF_LAUNCH(false, true); else DSF_LAUNCH(false, false);
+    }
+#undef DSF_LAUNCH
 }
 
 void launch_compact(sofia_ctx *c, bool energies, float *ff32) {
@@ -426,15 +457,19 @@ void launch_compact(sofia_ctx *c, bool energies, float *ff32) {
 }  // namespace
 
 void launch_dsf_mixed(sofia_ctx *c, bool energies, float *ff32, bool from_xd) {
+    // 12-byte records while every atom of a type carries the type's charge (the compacting
+    // kernel reads 16-byte records)
+    const bool qt = c->q_by_type && c->ptab && c->lanes != 0;
     hipLaunchKernelGGL(k_fixpos, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N,
-                       from_xd ? c->a.xd : nullptr, c->a.pos4, c->a.q, c->rep, c->reps_d, c->pfix);
+                       from_xd ? c->a.xd : nullptr, c->a.pos4, qt ? nullptr : c->a.q, c->rep,
+                       c->reps_d, c->pfix);
     switch (c->lanes) {
         case 0: launch_compact(c, energies, ff32); break;
-        case 4: launch_lanes<4>(c, energies, ff32); break;
-        case 8: launch_lanes<8>(c, energies, ff32); break;
-        case 32: launch_lanes<32>(c, energies, ff32); break;
-        case 64: launch_lanes<64>(c, energies, ff32); break;
-        default: launch_lanes<16>(c, energies, ff32); break;
+        case 4: launch_lanes<4>(c, energies, ff32, qt); break;
+        case 8: launch_lanes<8>(c, energies, ff32, qt); break;
+        case 32: launch_lanes<32>(c, energies, ff32, qt); break;
+        case 64: launch_lanes<64>(c, energies, ff32, qt); break;
+        default: launch_lanes<16>(c, energies, ff32, qt); break;
     }
 }
 

@@ -682,6 +682,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->bund_tstart); dfree(c->bund_sstart); dfree(c->bund_slots); dfree(c->bund_terms);
     dfree(c->shake_err); dfree(c->shake_iters); dfree(c->water); dfree(c->dsf_tab); dfree(c->pfix); dfree(c->ebond); dfree(c->eangle); dfree(c->bondlist); dfree(c->anglelist);
     dfree(c->dihedlist); dfree(c->torsions); dfree(c->shake_num); dfree(c->shake_atoms); dfree(c->shake_kidx);
+    dfree(c->ptab);
     dfree(c->shake_bond); dfree(c->ljtab); dfree(c->lj14); dfree(c->mass); dfree(c->bond_k);
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
     dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->improlist); dfree(c->impro_k);
@@ -715,6 +716,8 @@ int sofia_set_forcefield(sofia_ctx *c, const float *lj1, const float *lj2, const
         TRY(dalloc(&c->mass, c->p.ntypes));
     }
     TRY(upload(c->ljtab, lj.data(), sizeof(float4) * nt2));
+    c->lj_host = lj;
+    c->q_by_type = false;   // the pair table is rebuilt with the engine's lists
     TRY(upload(c->lj14, l14.data(), sizeof(double4) * nt2));
     TRY(upload(c->mass, mass, sizeof(double) * c->p.ntypes));
     c->have_ff = true;
@@ -914,8 +917,12 @@ int sofia_put_state(sofia_ctx *c, const sofia_ainfo *pos, const float *q, const 
         for (int i = 0; i < n; ++i)
             if (pos[i].type < 1 || pos[i].type > c->p.ntypes) return fail(SOFIA_E_ARG, "atom type out of range");
         TRY(upload(c->a.pos4, pos, sizeof(float4) * n));
+        c->q_by_type = false;   // types may have changed
     }
-    if (q) TRY(upload(c->a.q, q, sizeof(float) * n));
+    if (q) {
+        TRY(upload(c->a.q, q, sizeof(float) * n));
+        c->q_by_type = false;   // until the engine's lists are set up again
+    }
     if (v) TRY(upload(c->a.v32, v, sizeof(float) * 3 * (size_t)n));
     if (gid) TRY(upload(c->a.gid, gid, sizeof(int32_t) * n));
     return SOFIA_OK;
@@ -1505,8 +1512,48 @@ int classify_shake(sofia_ctx *c) {
     return upload(c->a.wclus, wclus.data(), sizeof(int32_t) * wclus.size());
 }
 
+// Charges by atom type?  Then the mixed pair kernel reads 12-byte position records and takes
+// C q_i q_j from a per-type-pair table, formed in float32 exactly as the kernel forms it
+// from per-atom charges ((C q_i) q_j), so the forces do not change.
+int classify_charges(sofia_ctx *c) {
+    c->q_by_type = false;
+    if (c->p.precision != SOFIA_PREC_MIXED || c->p.coul_flag == SOFIA_COUL_EWALD || c->lj_host.empty())
+        return SOFIA_OK;
+    TRY(sync_check(c));
+    const int n = c->N, nt = c->p.ntypes;
+    std::vector<float> q(n);
+    std::vector<float4> pos(n);
+    TRY(download(q.data(), c->a.q, sizeof(float) * n));
+    TRY(download(pos.data(), c->a.pos4, sizeof(float4) * n));
+    std::vector<float> qt(nt, 0.0f);
+    std::vector<char> seen(nt, 0);
+    for (int i = 0; i < n; ++i) {
+        const int t = __builtin_bit_cast(int, pos[i].w) - 1;
+        if (t < 0 || t >= nt) return SOFIA_OK;
+        if (!seen[t]) {
+            seen[t] = 1;
+            qt[t] = q[i];
+        } else if (__builtin_bit_cast(uint32_t, qt[t]) != __builtin_bit_cast(uint32_t, q[i])) {
+            return SOFIA_OK;
+        }
+    }
+    std::vector<float4> pt((size_t)nt * nt);
+    for (int a = 0; a < nt; ++a)
+        for (int b = 0; b < nt; ++b) {
+            const float4 l = c->lj_host[(size_t)a * nt + b];
+            volatile float qc = c->dp.coulpre * qt[a];
+            volatile float qq = qc * qt[b];
+            pt[(size_t)a * nt + b] = make_float4(l.x, l.y, qq, 0.0f);
+        }
+    if (!c->ptab) TRY(dalloc(&c->ptab, pt.size()));
+    TRY(upload(c->ptab, pt.data(), sizeof(float4) * pt.size()));
+    c->q_by_type = true;
+    return SOFIA_OK;
+}
+
 int engine_lists(sofia_ctx *c) {
     TRY(engine_regrid(c));
+    TRY(classify_charges(c));
     TRY(split_water(c));
     TRY(classify_shake(c));
     launch_set_rebuild(c, 1);

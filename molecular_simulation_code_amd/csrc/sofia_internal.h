@@ -161,6 +161,11 @@ struct sofia_ctx {
     int32_t impro_style = 0;       // 0 harmonic (lammps14), 1 cvff (amber wrapper)
     float4 *dsf_tab = nullptr;  // cubic pieces of the DSF Coulomb force / energy (mixed kernel)
     int4 *pfix = nullptr;       // mixed kernel positions: x 2^32 / box mod 2^32, type (per launch)
+    // charges by atom type (every atom of a type carries the same charge): the mixed kernel
+    // then reads 12-byte records and takes C q_i q_j from ptab (lj1, lj2, C q_i q_j, 0)
+    bool q_by_type = false;
+    float4 *ptab = nullptr;
+    std::vector<float4> lj_host;   // lj1..lj4 per type pair, as uploaded
     int32_t nbt = 0, nat = 0, ntt = 0;
     bool have_ff = false;
 

@@ -268,6 +268,31 @@ def test_bonded_bundles_match_per_kind_kernels():
             assert np.array_equal(out[1][:, 2:9], ref[1][:, 2:9]), f"{name}: bonded energies differ"
 
 
+def test_charge_by_type_records_keep_the_trajectory():
+    """Mixed precision: with charges set by atom type the per-atom pair kernels read 12-byte
+    records and take C q_i q_j from a type-pair table; the compacting kernel (lanes 0) keeps
+    16-byte records with the charges.  Both give the same trajectory bit for bit."""
+    host = _engine()
+    specs = [host.ReplicaSpec(_sys(100, 600, 86), temp=300.0, gamma_ps=1.0, seed=96),
+             host.ReplicaSpec(_sys(110, 800, 87), temp=310.0, gamma_ps=1.0, seed=97)]
+    outs = {}
+    for lanes in (16, 0):
+        say(f"[engine charge records] lanes {lanes}")
+        e = host.Engine(specs, precision=host.PREC_MIXED)
+        try:
+            e.set_tuning(lanes, 256, 4)
+            e.init()
+            e.minimize(20)
+            e.run(40)
+            outs[lanes] = (e.state(), e.forces())
+        finally:
+            e.close()
+    for k in range(len(specs)):
+        assert np.array_equal(outs[16][0][k][0], outs[0][0][k][0]), f"replica {k} positions differ"
+        assert np.array_equal(outs[16][0][k][1], outs[0][0][k][1]), f"replica {k} velocities differ"
+        assert np.array_equal(outs[16][1][k], outs[0][1][k]), f"replica {k} forces differ"
+
+
 def test_minimize_then_nve_energy_is_stable():
     host = _engine()
     s = _sys(100, 600, 31)
