@@ -614,3 +614,34 @@ def test_full_size_batch_properties():
                 assert net <= 1e-9 * scale
     finally:
         e.close()
+
+
+def test_full_size_mixed_forces_and_energies_track_fp64():
+    """The same 1.27 M-atom batch (23 copies of the 64-replica workload's largest replica) in
+    mixed precision and in FP64, NVE, at the generated positions: per replica the mixed forces
+    are within 1e-5 relative RMS of the FP64 forces and the potential energy within 1e-6."""
+    host = _engine()
+    comps = S.replica_sizes(64)
+    k = int(np.argmax([10 * g + 3 * w for g, w in comps]))
+    s = S.make_system(*comps[k], seed=20261210 + k)
+    ncopy = 23
+    specs = [host.ReplicaSpec(s, temp=300.0, gamma_ps=0.0, seed=700 + c) for c in range(ncopy)]
+    out = {}
+    for name, prec in (("fp64", host.PREC_FP64), ("mixed", host.PREC_MIXED)):
+        say("[full size precision]", name, "atoms", ncopy * s.np)
+        e = host.Engine(specs, precision=prec)
+        try:
+            e.init()
+            out[name] = (e.forces(), e.energies())
+        finally:
+            e.close()
+    (fd, ed), (fm, em) = out["fp64"], out["mixed"]
+    pot = [0, 1, 2, 3, 4, 5, 6, 8]          # every potential term; column 7 is the kinetic energy
+    for c in (0, ncopy - 1):
+        rms = np.sqrt(((fm[c] - fd[c]) ** 2).sum() / (fd[c] ** 2).sum())
+        ud, um = ed[c, pot].sum(), em[c, pot].sum()
+        say(f"[full size precision] copy {c}: force rel RMS {rms:.3e}, potential "
+            f"{ud:.6f} vs {um:.6f} (rel {abs(um - ud) / abs(ud):.2e}), terms rel "
+            + " ".join(f"{abs(em[c, t] - ed[c, t]) / max(abs(ed[c, t]), 1e-30):.1e}" for t in pot))
+        assert rms <= 1e-5
+        assert abs(um - ud) <= 1e-6 * abs(ud)
