@@ -38,7 +38,8 @@ enum {
     SOFIA_E_ARG = -3,        /* invalid argument or size */
     SOFIA_E_OVERFLOW = -4,   /* Verlet list overflow that could not be grown */
     SOFIA_E_HIP = -5,        /* HIP runtime error */
-    SOFIA_E_SHAKE = -6,      /* SHAKE determinant <= 0 (the reference stops) */
+    SOFIA_E_SHAKE = -6,      /* constraint solve failed: SHAKE determinant <= 0 (the
+                                reference stops) or a rigid water beyond SETTLE's reach */
     SOFIA_E_STATE = -7       /* call out of order (e.g. no list yet) */
 };
 
@@ -238,6 +239,22 @@ int sofia_md_resume(sofia_ctx *ctx);
 /* Total force on every atom after the last step (3N doubles, the order of
  * sofia_get_state_f64): the force the next step's first half-kick uses. */
 int sofia_get_forces_f64(sofia_ctx *ctx, double *f);
+/* The engine's pair kernel alone (lj_cut_coul_dsf_nonewton over the engine's own lists,
+ * Source/mod_force_nonbond.f90:105-196, in the context's precision) evaluated at the
+ * current state, for checking the engine against the FP64 reference routine: f receives
+ * 3N doubles (x, y, z per atom, device order as sofia_get_state_f64), e (optional) per
+ * replica 5 doubles: LJ energy, Coulomb energy, virial x, y, z.  The forces the last step
+ * carries are left as they were. */
+int sofia_md_pair_forces(sofia_ctx *ctx, double *f, double *e);
+/* sofia_get_nlist for replica rep alone: its natoms rows (1-based ids within the
+ * replica, stride *neigh_alloc) and counts. */
+int sofia_get_nlist_replica(sofia_ctx *ctx, int32_t rep, int32_t *nlist, int32_t *numneigh,
+                            int32_t *neigh_alloc);
+/* The engine's current knobs: pair-kernel lanes per atom, list-build waves, bonded mode,
+ * bytes of the mixed pair kernel's position record (12 when charges go by atom type, 16
+ * otherwise, 0 outside mixed DSF) and the Verlet skin in A.  Any pointer may be NULL. */
+int sofia_get_engine_config(sofia_ctx *ctx, int32_t *lanes, int32_t *list_waves,
+                            int32_t *bonded_mode, int32_t *record_bytes, double *skin);
 /* sofia_md_resume, then the forces of every replica r with have[r] != 0 replaced by f
  * (3N doubles in sofia_put_state_f64's order, as returned by sofia_get_forces_f64 and
  * re-ordered with the state).  The state a step carries over is (x, v, F): F was evaluated

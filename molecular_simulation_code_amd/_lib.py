@@ -97,6 +97,9 @@ SIGNATURES = {
     "sofia_md_finish": [_vp],
     "sofia_md_resume": [_vp],
     "sofia_get_forces_f64": [_vp, _vp],
+    "sofia_md_pair_forces": [_vp, _vp, _vp],
+    "sofia_get_nlist_replica": [_vp, _i32, _vp, _vp, _vp],
+    "sofia_get_engine_config": [_vp] * 6,
     "sofia_md_resume_forces": [_vp, _vp, _vp],
     "sofia_minimize": [_vp, _i32, _d, _d],
     "sofia_md_energies": [_vp, _vp],

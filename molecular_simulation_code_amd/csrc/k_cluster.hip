@@ -27,14 +27,18 @@ __device__ inline int uf_find(int32_t *parent, int x) {
     }
 }
 
-__device__ inline void uf_union(int32_t *parent, int a, int b) {
-    for (int it = 0; it < 1 << 20; ++it) {   // bounded: every successful CAS shrinks a tree
+// A failed CAS means another thread linked a root first, so the loop is lock-free; the
+// bound only guards against a corrupted forest, and running out of it is reported through
+// err (the host turns it into an error) rather than leaving a union undone unnoticed.
+__device__ inline void uf_union(int32_t *parent, int a, int b, int32_t *err) {
+    for (int it = 0; it < 1 << 20; ++it) {
         a = uf_find(parent, a);
         b = uf_find(parent, b);
         if (a == b) return;
         if (a < b) { const int t = a; a = b; b = t; }
         if (atomicCAS(&parent[a], a, b) == a) return;
     }
+    atomicOr(err, 1);
 }
 
 __global__ void k_uf_init(int32_t nmol, int32_t *__restrict__ parent, int32_t *__restrict__ size) {
@@ -49,7 +53,7 @@ __global__ void k_uf_contacts(int32_t n, const float4 *__restrict__ pos4, const 
                               const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
                               const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh,
                               int32_t na, uint32_t nmask, uint32_t omask, float rc2,
-                              int32_t *__restrict__ parent) {
+                              int32_t *__restrict__ parent, int32_t *__restrict__ err) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int mi = mol[i];
@@ -71,7 +75,7 @@ __global__ void k_uf_contacts(int32_t n, const float4 *__restrict__ pos4, const 
         dx -= R.box * __builtin_rintf(dx * R.ibox);
         dy -= R.box * __builtin_rintf(dy * R.ibox);
         dz -= R.box * __builtin_rintf(dz * R.ibox);
-        if (dx * dx + dy * dy + dz * dz < rc2) uf_union(parent, mi, mj);
+        if (dx * dx + dy * dy + dz * dz < rc2) uf_union(parent, mi, mj, err);
     }
 }
 
@@ -104,21 +108,26 @@ int cluster_largest(sofia_ctx *c, float rcontact, const std::vector<int> &o_type
     }
     if (rcontact * rcontact >= (float)c->dp.rcut2)
         return fail(SOFIA_E_ARG, "contact distance must be inside the pair cutoff");
-    SOFIA_HIP(hipMemsetAsync(c->uf_largest, 0, sizeof(int32_t) * c->nrep, c->stream));
+    // uf_largest holds nrep maxima and, after them, the give-up flag of uf_union
+    int32_t *err = c->uf_largest + c->nrep;
+    SOFIA_HIP(hipMemsetAsync(c->uf_largest, 0, sizeof(int32_t) * (c->nrep + 1), c->stream));
     if (c->nmol > 0) {
         hipLaunchKernelGGL(k_uf_init, dim3(nblocks(c->nmol, 256)), dim3(256), 0, c->stream, c->nmol,
                            c->uf_parent, c->uf_size);
         hipLaunchKernelGGL(k_uf_contacts, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N,
                            c->a.pos4, c->a.mol, c->rep, c->reps_d, c->nlist, c->numneigh,
-                           c->neigh_alloc, nm, om, rcontact * rcontact, c->uf_parent);
+                           c->neigh_alloc, nm, om, rcontact * rcontact, c->uf_parent, err);
         hipLaunchKernelGGL(k_uf_count, dim3(nblocks(c->nmol, 256)), dim3(256), 0, c->stream, c->nmol,
                            c->uf_parent, c->uf_size);
         hipLaunchKernelGGL(k_uf_largest, dim3(nblocks(c->nmol, 256)), dim3(256), 0, c->stream,
                            c->nmol, c->uf_parent, c->uf_size, c->mol_rep, c->uf_largest);
     }
-    SOFIA_HIP(hipMemcpyAsync(largest, c->uf_largest, sizeof(int32_t) * c->nrep, hipMemcpyDeviceToHost,
-                             c->stream));
+    std::vector<int32_t> out(c->nrep + 1);
+    SOFIA_HIP(hipMemcpyAsync(out.data(), c->uf_largest, sizeof(int32_t) * (c->nrep + 1),
+                             hipMemcpyDeviceToHost, c->stream));
     SOFIA_HIP(hipStreamSynchronize(c->stream));
+    if (out[c->nrep]) return fail(SOFIA_E_STATE, "union-find gave up on a union (corrupted forest)");
+    for (int r = 0; r < c->nrep; ++r) largest[r] = out[r];
     return SOFIA_OK;
 }
 

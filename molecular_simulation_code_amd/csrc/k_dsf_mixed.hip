@@ -18,7 +18,7 @@
 //   dr6i = (1/r^2)^3, 0 beyond rcut2;  F_LJ/r = dr6i (lj1 dr6i - lj2) / r^2
 //   F_C/r = C qi qj g(r), g from a cubic Hermite table in r over [0.5 A, cut_coul] built
 //   in FP64 from the reference's A&S 7.1.26 erfc; one extra all-zero bin past the last
-//   takes every r >= cut_coul (and anything the clamp of the bin index sends there).
+//   takes every r >= cut_coul; below 0.5 A, g is evaluated directly (dsf_direct).
 // Energies (energy launches only) use the matching table for E_C and lj3/lj4 for E_LJ,
 // halved (full list); the virial is 0.5 sum dx^2 F/r as in the PME kernel.
 //
@@ -76,11 +76,27 @@ struct PairOut {
     float dr6i;    // LJ r^-6 (0 beyond rcut2)
     float t;       // position inside the table bin
     int kb;        // table bin (tn = the zero bin)
+    float ebelow;  // Coulomb energy per C qi qj below the table start (0 elsewhere)
 };
+
+// DSF Coulomb terms evaluated directly with the A&S 7.1.26 erfc (Source/mod_force_nonbond.f90:
+// 160-183), for separations below the table start: (force over r, energy) per C qi qj.
+__device__ __noinline__ float2 dsf_direct(float r, float dri, const DevParams &P) {
+    const float al = (float)P.alpha;
+    const float erfcd = expf(-(al * al) * (r * r));
+    const float t = 1.0f / (1.0f + (float)(P.ewald_p * P.alpha) * r);
+    const float erfcc =
+        t * ((float)P.a1 + t * ((float)P.a2 + t * ((float)P.a3 + t * ((float)P.a4 + t * (float)P.a5)))) *
+        erfcd;
+    const float fs = (float)P.f_shift;
+    const float g = (erfcc * dri + (float)(2.0 * P.alpha * P.my_pis_inv) * erfcd + r * fs) * (dri * dri);
+    const float e = erfcc * dri - (float)P.e_shift - r * fs;
+    return make_float2(g, e);
+}
 
 __device__ __forceinline__ PairOut pair_dsf(float dr2, float qq, const float4 &lj,
                                             const float4 *__restrict__ s_tab, float tinv,
-                                            float u0, int tn, float rcut2) {
+                                            float u0, int tn, float rcut2, const DevParams &P) {
     PairOut o;
     const float dri = __builtin_amdgcn_rsqf(dr2);
     const float r = dr2 * dri;
@@ -89,10 +105,20 @@ __device__ __forceinline__ PairOut pair_dsf(float dr2, float qq, const float4 &l
     if (dr2 > rcut2) dr6i = 0.0f;
     const float flj = dr6i * __builtin_fmaf(lj.x, dr6i, -lj.y);
     const float u = __builtin_fmaf(r, tinv, u0);
-    // a negative u converts to a huge unsigned index and lands in the zero bin too
-    o.kb = (int)min((uint32_t)(int)u, (uint32_t)tn);
+    // r beyond the table (u >= tn) reads the zero bin; r below its start (u < 0) reads the
+    // zero bin too and takes the direct form instead (the reference evaluates DSF at every
+    // r < cut_coul)
+    const bool below = u < 0.0f;
+    o.kb = below ? tn : (int)min((uint32_t)(int)u, (uint32_t)tn);
     o.t = __builtin_amdgcn_fractf(u);
-    o.force = __builtin_fmaf(qq, cubic_f(s_tab[o.kb], o.t), flj * dr2i);
+    float gc = cubic_f(s_tab[o.kb], o.t);
+    o.ebelow = 0.0f;
+    if (below) {
+        const float2 d = dsf_direct(r, dri, P);
+        gc = d.x;
+        o.ebelow = d.y;
+    }
+    o.force = __builtin_fmaf(qq, gc, flj * dr2i);
     o.dr6i = dr6i;
     return o;
 }
@@ -221,7 +247,7 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             const float dr2 = norm2_f(dx, dy, dz) * s2;
             const float4 lj = ljrow[valid ? tj : 1u];
             const float qq = QT ? lj.z : qc * __int_as_float(pj.w);
-            const PairOut o = pair_dsf(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2);
+            const PairOut o = pair_dsf(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2, P);
             const float fs = valid ? o.force * fq : 0.0f;
             fxq += (double)__builtin_truncf(dx * fs);
             fyq += (double)__builtin_truncf(dy * fs);
@@ -230,7 +256,7 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
                 if (valid) {
                     const float4 le = erow[tj];
                     pot += (double)(o.dr6i * __builtin_fmaf(o.dr6i, le.z, -le.w));
-                    ecl += (double)(qq * cubic_f(s_etab[o.kb], o.t));
+                    ecl += (double)(qq * (cubic_f(s_etab[o.kb], o.t) + o.ebelow));
                     const float fv = o.force * s2;
                     vx += (double)(dx * dx * fv);
                     vy += (double)(dy * dy * fv);
@@ -344,14 +370,14 @@ k_dsf_compact(int32_t n, const int4 *__restrict__ pf,
             const float dx = e.x, dy = e.y, dz = e.z;
             const float dr2 = norm2_f(dx, dy, dz) * s2;
             const float qq = qc * e.w;
-            const PairOut o = pair_dsf(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2);
+            const PairOut o = pair_dsf(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2, P);
             const float fs = o.force * fq;
             fxq += (double)__builtin_truncf(dx * fs);
             fyq += (double)__builtin_truncf(dy * fs);
             fzq += (double)__builtin_truncf(dz * fs);
             if (ENERGY) {
                 pot += (double)(o.dr6i * __builtin_fmaf(o.dr6i, lj.z, -lj.w));
-                ecl += (double)(qq * cubic_f(s_etab[o.kb], o.t));
+                ecl += (double)(qq * (cubic_f(s_etab[o.kb], o.t) + o.ebelow));
                 const float fv = o.force * s2;
                 vx += (double)(dx * dx * fv);
                 vy += (double)(dy * dy * fv);

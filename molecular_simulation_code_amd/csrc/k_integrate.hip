@@ -700,10 +700,16 @@ __device__ inline int settle_water(int m, const ShakeArgs &A, double vir[3], lon
         const double za1 = to(a1, 2);
         const double xb1 = to(b1, 0), yb1 = to(b1, 1), zb1 = to(b1, 2);
         const double xc1 = to(c1, 0), yc1 = to(c1, 1), zc1 = to(c1, 2);
+        // The closed form exists only while the unconstrained step leaves the water within
+        // reach of the rigid triangle: each radicand below must be non-negative (and every
+        // value finite).  Otherwise the step is rejected as a failed constraint solve, as the
+        // iterative solver rejects a singular one, instead of writing NaN into the state.
         const double sinphi = za1 / ra;
-        const double cosphi = sqrt(1.0 - sinphi * sinphi);
+        const double rad_phi = 1.0 - sinphi * sinphi;
+        const double cosphi = sqrt(fmax(rad_phi, 0.0));
         const double sinpsi = (zb1 - zc1) / (2.0 * rc * cosphi);
-        const double cospsi = sqrt(1.0 - sinpsi * sinpsi);
+        const double rad_psi = 1.0 - sinpsi * sinpsi;
+        const double cospsi = sqrt(fmax(rad_psi, 0.0));
         const double ya2 = ra * cosphi, xb2 = -rc * cospsi;
         const double t1 = -rb * cosphi, t2 = rc * sinpsi * sinphi;
         const double yb2 = t1 - t2, yc2 = t1 + t2;
@@ -711,8 +717,16 @@ __device__ inline int settle_water(int m, const ShakeArgs &A, double vir[3], lon
         const double beta = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
         const double gamma = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
         const double ab = alpha * alpha + beta * beta;
-        const double sint = (alpha * gamma - beta * sqrt(ab - gamma * gamma)) / ab;
-        const double cost = sqrt(1.0 - sint * sint);
+        const double rad_ab = ab - gamma * gamma;
+        const double sint = (alpha * gamma - beta * sqrt(fmax(rad_ab, 0.0))) / ab;
+        const double rad_t = 1.0 - sint * sint;
+        // written so that a NaN in any operand also fails
+        if (!(rad_phi >= 0.0 && cosphi > 0.0 && rad_psi >= 0.0 && ab > 0.0 && rad_ab >= 0.0 &&
+              rad_t >= 0.0)) {
+            atomicOr(A.err, 1);
+            return rr;
+        }
+        const double cost = sqrt(rad_t);
         const double p3[3][3] = {{-ya2 * sint, ya2 * cost, za1},
                                  {xb2 * cost - yb2 * sint, xb2 * sint + yb2 * cost, zb1},
                                  {-xb2 * cost - yc2 * sint, -xb2 * sint + yc2 * cost, zc1}};

@@ -54,6 +54,14 @@ void dfree(T *&p) {
         if (rc_ != SOFIA_OK) return rc_; \
     } while (0)
 
+// The captured MD step holds buffer addresses and by-value parameters (list stride, tuning,
+// record layout, thermostat constants) as they were at capture; anything that changes one
+// of them drops the graph, and the next graph launch captures it again.
+void drop_graph(sofia_ctx *c) {
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+}
+
 int alloc_atoms(sofia::AtomArrays &A, int n) {
     TRY(dalloc(&A.pos4, n));
     TRY(dalloc(&A.q, n));
@@ -126,6 +134,7 @@ int grow_alloc(int mx) {   // Source/mod_neighbor.f90:260-263
 
 int ensure_list(sofia_ctx *c, int na) {
     const size_t need = (size_t)c->N * (size_t)na;
+    if (need > c->nlist_cap || na != c->neigh_alloc) drop_graph(c);
     if (need > c->nlist_cap) {
         dfree(c->nlist);
         TRY(dalloc(&c->nlist, need));
@@ -479,7 +488,7 @@ int rebuild_now(sofia_ctx *c, bool engine, bool reference_growth) {
 void drop_bonded_bundles(sofia_ctx *c) {
     dfree(c->bund_tstart); dfree(c->bund_sstart); dfree(c->bund_slots); dfree(c->bund_terms);
     c->nbundle = 0;
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graph(c);
 }
 
 // The engine's bonded items (1-4 pairs, bonds, angles, torsions, impropers, as
@@ -642,6 +651,10 @@ int build_bonded_bundles(sofia_ctx *c) {
 
 using namespace sofia;
 
+namespace {
+int classify_charges(sofia_ctx *c);   // the engine's charge records, defined with the engine set-up
+}  // namespace
+
 extern "C" {
 
 int sofia_last_error(char *buf, size_t len) {
@@ -670,8 +683,7 @@ int sofia_ctx_create_batch(const sofia_params *p, int32_t nrep, const int32_t *n
 int sofia_ctx_destroy(sofia_ctx *c) {
     if (!c) return SOFIA_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    if (c->graph_raw) (void)hipGraphDestroy(c->graph_raw);
+    drop_graph(c);
     free_atoms(c->a);
     free_atoms(c->b);
     dfree(c->rep); dfree(c->dr32); dfree(c->drd); dfree(c->fnb); dfree(c->fix); dfree(c->fscratch); dfree(c->stage32);
@@ -715,12 +727,16 @@ int sofia_set_forcefield(sofia_ctx *c, const float *lj1, const float *lj2, const
         TRY(dalloc(&c->lj14, nt2));
         TRY(dalloc(&c->mass, c->p.ntypes));
     }
+    TRY(sync_check(c));   // kernels in flight may still read the tables
     TRY(upload(c->ljtab, lj.data(), sizeof(float4) * nt2));
     c->lj_host = lj;
-    c->q_by_type = false;   // the pair table is rebuilt with the engine's lists
+    c->q_by_type = false;
+    drop_graph(c);          // a captured step may hold the 12-byte record layout
     TRY(upload(c->lj14, l14.data(), sizeof(double4) * nt2));
     TRY(upload(c->mass, mass, sizeof(double) * c->p.ntypes));
     c->have_ff = true;
+    // a running engine takes the new coefficients into its type-pair table at once
+    if (c->engine_ready) TRY(classify_charges(c));
     return SOFIA_OK;
 }
 
@@ -913,18 +929,23 @@ int sofia_put_state(sofia_ctx *c, const sofia_ainfo *pos, const float *q, const 
                     const int32_t *gid) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     const int n = c->N;
+    TRY(sync_check(c));   // kernels in flight may still read the arrays replaced here
     if (pos) {
         for (int i = 0; i < n; ++i)
             if (pos[i].type < 1 || pos[i].type > c->p.ntypes) return fail(SOFIA_E_ARG, "atom type out of range");
         TRY(upload(c->a.pos4, pos, sizeof(float4) * n));
         c->q_by_type = false;   // types may have changed
+        drop_graph(c);
     }
     if (q) {
         TRY(upload(c->a.q, q, sizeof(float) * n));
-        c->q_by_type = false;   // until the engine's lists are set up again
+        c->q_by_type = false;
+        drop_graph(c);
     }
     if (v) TRY(upload(c->a.v32, v, sizeof(float) * 3 * (size_t)n));
     if (gid) TRY(upload(c->a.gid, gid, sizeof(int32_t) * n));
+    // a running engine re-derives its charge records from the new types and charges
+    if ((pos || q) && c->engine_ready) TRY(classify_charges(c));
     return SOFIA_OK;
 }
 
@@ -1296,7 +1317,7 @@ int sofia_set_thermostat(sofia_ctx *c, const sofia_thermostat *t) {
     }
     c->has_thermostat = any;
     TRY(upload(c->reps_d, c->reps_h.data(), sizeof(sofia::RepInfo) * c->nrep));
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graph(c);
     return SOFIA_OK;
 }
 
@@ -1350,7 +1371,9 @@ int check_engine_flags(sofia_ctx *c) {
     SOFIA_HIP(hipMemcpyAsync(&mx, c->maxcnt, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     TRY(sync_check(c));
     if (mx > c->neigh_alloc) return fail(SOFIA_E_OVERFLOW, "Verlet list overflow during an MD segment");
-    if (err) return fail(SOFIA_E_SHAKE, "SHAKE determinant <= 0 during an MD segment");
+    if (err)
+        return fail(SOFIA_E_SHAKE, "constraint solve failed during an MD segment (SHAKE "
+                                   "determinant <= 0, or a rigid water moved beyond SETTLE's reach)");
     return SOFIA_OK;
 }
 
@@ -1362,6 +1385,7 @@ namespace {
 // of every replica from its current box, per-cell arrays re-sized when the cell count grows.
 int reference_regrid(sofia_ctx *c) {
     TRY(sync_check(c));
+    drop_graph(c);
     int cell0 = 0;
     for (auto &R : c->reps_h) {
         cell_grid(R.box, c->p.rcut, c->p.cut_coul, &R.rn, &R.ncellT, &R.ncellD);
@@ -1413,6 +1437,7 @@ int reference_regrid(sofia_ctx *c) {
 int engine_regrid(sofia_ctx *c) {
     if (c->engine_grid) return SOFIA_OK;
     TRY(sync_check(c));
+    drop_graph(c);
     const double rc = std::sqrt(std::max(c->p.rcut2, c->p.cut_coulsq));
     const double rv = c->skin >= 0.0 ? rc + c->skin : std::sqrt(c->p.rv2);
     int cell0 = 0, n5 = 0;
@@ -1572,7 +1597,7 @@ int engine_lists(sofia_ctx *c) {
     SOFIA_HIP(hipMemsetAsync(c->nbuilds, 0, sizeof(int32_t) * c->nrep, c->stream));
     SOFIA_HIP(hipMemsetAsync(c->shake_err, 0, sizeof(int32_t), c->stream));
     SOFIA_HIP(hipMemsetAsync(c->maxcnt, 0, sizeof(int32_t), c->stream));
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graph(c);
     return SOFIA_OK;
 }
 
@@ -1618,6 +1643,70 @@ int sofia_get_forces_f64(sofia_ctx *c, double *f) {
     // the sum k_nve_v64 forms (FIX_INV is a power of two, so the product is exact)
     for (size_t i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d) f[3 * i + d] = nb[d * n + i] + (double)fx[d * n + i] * FIX_INV;
+    return SOFIA_OK;
+}
+
+int sofia_md_pair_forces(sofia_ctx *c, double *f, double *e) {
+    if (!c || !f) return fail(SOFIA_E_ARG, "null argument");
+    if (!c->engine_ready || !c->have_list) return fail(SOFIA_E_STATE, "call sofia_md_init first");
+    const size_t n = (size_t)c->N, fb = sizeof(double) * 3 * n;
+    // the pair forces the last step carries are set aside and put back afterwards
+    launch_zero_forces(c, c->fscratch, true);
+    SOFIA_HIP(hipMemcpyAsync(c->fscratch, c->fnb, fb, hipMemcpyDeviceToDevice, c->stream));
+    launch_nonbond(c, nb_mode(c), true);
+    std::vector<double> nb(3 * n);
+    std::vector<long long> eq((size_t)c->nrep * NE);
+    SOFIA_HIP(hipMemcpyAsync(nb.data(), c->fnb, fb, hipMemcpyDeviceToHost, c->stream));
+    SOFIA_HIP(hipMemcpyAsync(eq.data(), c->eacc, sizeof(long long) * eq.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    SOFIA_HIP(hipMemcpyAsync(c->fnb, c->fscratch, fb, hipMemcpyDeviceToDevice, c->stream));
+    TRY(sync_check(c));
+    for (size_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) f[3 * i + d] = nb[d * n + i];
+    if (e)
+        for (int r = 0; r < c->nrep; ++r) {
+            double *o = e + (size_t)r * 5;
+            o[0] = E(eq, r, E_LJ); o[1] = E(eq, r, E_COUL);
+            o[2] = E(eq, r, V_X); o[3] = E(eq, r, V_Y); o[4] = E(eq, r, V_Z);
+        }
+    return SOFIA_OK;
+}
+
+int sofia_get_nlist_replica(sofia_ctx *c, int32_t rep, int32_t *nlist, int32_t *numneigh,
+                            int32_t *na) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (rep < 0 || rep >= c->nrep) return fail(SOFIA_E_ARG, "replica index out of range");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no list built yet");
+    TRY(sync_check(c));
+    const auto &R = c->reps_h[rep];
+    const size_t stride = (size_t)c->neigh_alloc;
+    if (na) *na = c->neigh_alloc;
+    std::vector<int32_t> nn(R.natom);
+    TRY(download(nn.data(), c->numneigh + R.atom0, sizeof(int32_t) * R.natom));
+    if (numneigh) std::memcpy(numneigh, nn.data(), sizeof(int32_t) * R.natom);
+    if (nlist) {
+        TRY(download(nlist, c->nlist + (size_t)R.atom0 * stride, sizeof(int32_t) * R.natom * stride));
+        for (int k = 0; k < R.natom; ++k) {
+            int32_t *row = nlist + (size_t)k * stride;
+            const int m = std::min<int>(nn[k], (int)stride);
+            for (int q = 0; q < m; ++q) row[q] = row[q] - R.atom0 + 1;
+            for (size_t q = m; q < stride; ++q) row[q] = 0;
+        }
+    }
+    return SOFIA_OK;
+}
+
+int sofia_get_engine_config(sofia_ctx *c, int32_t *lanes, int32_t *list_waves,
+                            int32_t *bonded_mode, int32_t *record_bytes, double *skin) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (lanes) *lanes = c->lanes;
+    if (list_waves) *list_waves = c->list_waves;
+    if (bonded_mode) *bonded_mode = c->bonded_bundles ? 1 : 0;
+    if (record_bytes)
+        *record_bytes = c->p.precision != SOFIA_PREC_MIXED || c->p.coul_flag == SOFIA_COUL_EWALD
+                            ? 0
+                            : ((c->q_by_type && c->ptab && c->lanes != 0) ? 12 : 16);
+    if (skin) *skin = c->skin >= 0.0 ? c->skin : std::sqrt(c->p.rv2) - std::sqrt(std::max(c->p.rcut2, c->p.cut_coulsq));
     return SOFIA_OK;
 }
 
@@ -1687,9 +1776,13 @@ int sofia_md_launch(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
         SOFIA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         step_sequence(c, false);
         SOFIA_HIP(hipStreamEndCapture(c->stream, &g));
-        if (c->graph_raw) (void)hipGraphDestroy(c->graph_raw);
-        c->graph_raw = g;
-        SOFIA_HIP(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+        // the executable graph is independent of the captured one, which goes at once
+        const hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            c->graph = nullptr;
+            return check_hip(ie, "hipGraphInstantiate");
+        }
     }
     for (int s = 0; s < nsteps - 1; ++s) {
         if (use_graph) {
@@ -1742,7 +1835,7 @@ int sofia_minimize(sofia_ctx *c, int32_t nsteps, double gamma, double max_disp) 
     c->has_thermostat = thermo;
     force_sequence(c, true);
     TRY(collect_energies(c));
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graph(c);
     return check_engine_flags(c);
 }
 
@@ -1840,7 +1933,7 @@ int sofia_set_tuning(sofia_ctx *c, int32_t lanes, int32_t block, int32_t list_wa
     c->lanes = lanes;
     c->block = block;
     c->list_waves = list_waves;
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graph(c);
     return SOFIA_OK;
 }
 
@@ -1889,7 +1982,7 @@ int sofia_set_bonded_mode(sofia_ctx *c, int32_t mode) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     if (mode != 0 && mode != 1) return fail(SOFIA_E_ARG, "bonded mode must be 0 (per-kind kernels) or 1 (molecule bundles)");
     c->bonded_bundles = mode == 1;
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graph(c);
     return SOFIA_OK;
 }
 
@@ -1934,7 +2027,7 @@ int sofia_set_molecules(sofia_ctx *c, const int32_t *mol) {
     TRY(dalloc(&c->mol_rep, std::max(1, base)));
     TRY(dalloc(&c->uf_parent, std::max(1, base)));
     TRY(dalloc(&c->uf_size, std::max(1, base)));
-    TRY(dalloc(&c->uf_largest, c->nrep));
+    TRY(dalloc(&c->uf_largest, c->nrep + 1));   // per-replica maxima, then an error flag
     if (base) TRY(upload(c->mol_rep, mrep.data(), sizeof(int32_t) * base));
     return upload(c->a.mol, g.data(), sizeof(int32_t) * c->N);
 }

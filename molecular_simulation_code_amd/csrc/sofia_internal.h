@@ -182,7 +182,6 @@ struct sofia_ctx {
     uint64_t step_h = 0;
     int lanes = 16, block = 256, list_waves = 4;
     hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_raw = nullptr;
     bool has_thermostat = false;
 
     // host staging (pinned) for the asynchronous pair-kernel drop-in

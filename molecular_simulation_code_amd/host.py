@@ -611,6 +611,41 @@ class Engine:
         check(lib().sofia_get_forces_f64(self.ctx, ptr(f)))
         return [t[0] for t in self._by_gid((f,))]
 
+    def pair_forces(self):
+        """The pair kernel alone at the current state: forces (N, 3) in device order and,
+        per replica, (LJ energy, Coulomb energy, virial x, y, z)."""
+        f = np.empty((self.N, 3), f64)
+        e = np.empty((len(self.reps), 5), f64)
+        check(lib().sofia_md_pair_forces(self.ctx, ptr(f), ptr(e)))
+        return f, e
+
+    def device_view(self):
+        """FP64 positions (N, 3) and global ids (N,) in the current device order."""
+        x = np.empty((self.N, 3), f64)
+        gid = np.empty(self.N, i32)
+        check(lib().sofia_get_state_f64(self.ctx, ptr(x), None))
+        check(lib().sofia_get_state(self.ctx, None, None, ptr(gid)))
+        return x, gid
+
+    def replica_list(self, r: int):
+        """Replica r's Verlet list: (nlist flat, numneigh, stride), 1-based ids within the
+        replica in device order."""
+        na = C.c_int32()
+        n = int(self.natoms[r])
+        check(lib().sofia_get_nlist_replica(self.ctx, r, None, None, C.byref(na)))
+        nl = np.empty(n * na.value, i32)
+        nn = np.empty(n, i32)
+        check(lib().sofia_get_nlist_replica(self.ctx, r, ptr(nl), ptr(nn), C.byref(na)))
+        return nl, nn, na.value
+
+    def config(self) -> dict:
+        lanes, waves, bmode, rec = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        skin = C.c_double()
+        check(lib().sofia_get_engine_config(self.ctx, C.byref(lanes), C.byref(waves), C.byref(bmode),
+                                            C.byref(rec), C.byref(skin)))
+        return {"lanes": lanes.value, "list_waves": waves.value, "bonded_mode": bmode.value,
+                "record_bytes": rec.value, "skin": skin.value}
+
     def put_state(self, states):
         """Upload FP64 (x, v) per replica, each in its own global_id order; valid before
         the first sort (a fresh engine holds every replica in that order)."""

@@ -185,6 +185,24 @@ def dsf(pos, types, q, nlist, numneigh, neigh_alloc, lj, prm: OrcParams):
     return ff, ff64, pot.value, ec.value, vir
 
 
+def dsf_x64(xyz, types, q, nlist, numneigh, neigh_alloc, lj, prm: OrcParams):
+    """orc_dsf on FP64 positions (n, 3): returns (ff64, potential, e_coul, virial)."""
+    n = len(xyz)
+    x = np.ascontiguousarray(xyz, _d)
+    t = np.ascontiguousarray(types, _i)
+    ff64 = np.zeros((n, 3), _d)
+    pot = C.c_double()
+    ec = C.c_double()
+    vir = np.zeros(3, _d)
+    l1, l2, l3, l4 = _lj(lj[:4])
+    nlist = np.ascontiguousarray(nlist, _i)
+    numneigh = np.ascontiguousarray(numneigh, _i)
+    lib().orc_dsf_x64(C.c_int32(n), P(x), P(t), P(np.ascontiguousarray(q, _f)), P(nlist),
+                      P(numneigh), C.c_int32(neigh_alloc), P(l1), P(l2), P(l3), P(l4),
+                      C.byref(prm), P(ff64), C.byref(pot), C.byref(ec), P(vir))
+    return ff64, pot.value, ec.value, vir
+
+
 def coul_long(pos, types, q, nlist, numneigh, neigh_alloc, lj, prm: OrcParams):
     n = len(pos)
     a = ainfo(pos, types)
@@ -487,6 +505,10 @@ def settle(x0, x1, m_o, m_h, d_oh, d_hh):
     |H-H| = d_hh with displacements m_i (x - x1)_i along the old bonds: the solution that
     SHAKE (Source/mod_shake.f90 shake3angle) iterates towards, in closed form.  Checker
     for the engine's water constraint path; not a restatement of a reference routine.
+
+    The closed form exists only while every radicand is non-negative (the unconstrained
+    step left each water within reach of the rigid triangle); otherwise ValueError names
+    the waters, as the reference's SHAKE stops on a singular solve.
     """
     x0 = np.asarray(x0, _d); x1 = np.asarray(x1, _d)
     w = m_o + 2.0 * m_h
@@ -504,10 +526,16 @@ def settle(x0, x1, m_o, m_h, d_oh, d_hh):
                   zk / np.linalg.norm(zk, axis=1)[:, None]], axis=1)        # rows: frame axes
     p = lambda v: np.einsum("nij,nj->ni", T, v)
     b0d, c0d, a1d, b1d, c1d = p(b0), p(c0), p(a1), p(b1), p(c1)
-    sinphi = a1d[:, 2] / ra
-    cosphi = np.sqrt(1.0 - sinphi ** 2)
-    sinpsi = (b1d[:, 2] - c1d[:, 2]) / (2.0 * rc * cosphi)
-    cospsi = np.sqrt(1.0 - sinpsi ** 2)
+    bad = np.zeros(len(x1), bool)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        sinphi = a1d[:, 2] / ra
+        rad_phi = 1.0 - sinphi ** 2
+        cosphi = np.sqrt(np.maximum(rad_phi, 0.0))
+        bad |= ~(rad_phi >= 0.0) | ~(cosphi > 0.0)
+        sinpsi = (b1d[:, 2] - c1d[:, 2]) / (2.0 * rc * cosphi)
+        rad_psi = 1.0 - sinpsi ** 2
+        bad |= ~(rad_psi >= 0.0)
+    cospsi = np.sqrt(np.maximum(rad_psi, 0.0))
     ya2 = ra * cosphi
     xb2 = -rc * cospsi
     t1, t2 = -rb * cosphi, rc * sinpsi * sinphi
@@ -516,8 +544,15 @@ def settle(x0, x1, m_o, m_h, d_oh, d_hh):
     beta = xb2 * (c0d[:, 1] - b0d[:, 1]) + b0d[:, 0] * yb2 + c0d[:, 0] * yc2
     gamma = b0d[:, 0] * b1d[:, 1] - b1d[:, 0] * b0d[:, 1] + c0d[:, 0] * c1d[:, 1] - c1d[:, 0] * c0d[:, 1]
     ab = alpha ** 2 + beta ** 2
-    sint = (alpha * gamma - beta * np.sqrt(ab - gamma ** 2)) / ab
-    cost = np.sqrt(1.0 - sint ** 2)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rad_ab = ab - gamma ** 2
+        sint = (alpha * gamma - beta * np.sqrt(np.maximum(rad_ab, 0.0))) / ab
+        rad_t = 1.0 - sint ** 2
+    bad |= ~(ab > 0.0) | ~(rad_ab >= 0.0) | ~(rad_t >= 0.0)
+    if bad.any():
+        raise ValueError(f"SETTLE has no solution for waters {np.flatnonzero(bad).tolist()}: "
+                         "the unconstrained step moved them beyond the rigid triangle's reach")
+    cost = np.sqrt(rad_t)
     a3 = np.stack([-ya2 * sint, ya2 * cost, a1d[:, 2]], axis=1)
     b3 = np.stack([xb2 * cost - yb2 * sint, xb2 * sint + yb2 * cost, b1d[:, 2]], axis=1)
     c3 = np.stack([-xb2 * cost - yc2 * sint, -xb2 * sint + yc2 * cost, c1d[:, 2]], axis=1)

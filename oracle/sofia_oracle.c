@@ -236,16 +236,21 @@ int32_t orc_check_neighbor(int32_t np, const float *drx, const float *dry, const
  * :376-390); the reference DSF kernel itself computes no virial.
  * Energies are summed per atom and then over atoms in index order.
  * ---------------------------------------------------------------------------------- */
-void orc_dsf(int32_t np, const orc_ainfo *pos, const float *q, const int32_t *nlist,
-             const int32_t *numneigh, int32_t neigh_alloc, const float *lj1, const float *lj2,
-             const float *lj3, const float *lj4, const orc_params *p, orc_coords *ff,
-             double *ff64, double *potential, double *e_coul, double *virial) {
+/* The pair loop on positions already promoted to real*8 (xyz: x, y, z per atom).  The
+ * reference promotes each real*4 coordinate where it forms dx (`x1 - position(j)%x`), so on
+ * promoted real*4 inputs this is the reference's arithmetic exactly. */
+static void dsf_core(int32_t np, const double *xyz, const int32_t *type, const float *q,
+                     const int32_t *nlist, const int32_t *numneigh, int32_t neigh_alloc,
+                     const float *lj1, const float *lj2, const float *lj3, const float *lj4,
+                     const orc_params *p, orc_coords *ff, double *ff64, double *potential,
+                     double *e_coul, double *virial) {
     double *pe = (double *)malloc(sizeof(double) * (size_t)np * 5);
     const double two_a_pis = (2.0 * p->alpha) * p->my_pis_inv;
 #pragma omp parallel for schedule(dynamic, 32)
     for (int i = 1; i <= np; ++i) {
-        const double x1 = pos[i - 1].x, y1 = pos[i - 1].y, z1 = pos[i - 1].z;
-        const int itype = pos[i - 1].type;
+        const double x1 = xyz[3 * (size_t)(i - 1)], y1 = xyz[3 * (size_t)(i - 1) + 1];
+        const double z1 = xyz[3 * (size_t)(i - 1) + 2];
+        const int itype = type[i - 1];
         const double qtmp = q[i - 1];
         const int ioffset = (itype - 1) * p->ntypes;
         const int32_t *lst = nlist + (size_t)(i - 1) * neigh_alloc;
@@ -254,10 +259,10 @@ void orc_dsf(int32_t np, const orc_ainfo *pos, const float *q, const int32_t *nl
         double vx = 0.0, vy = 0.0, vz = 0.0;
         for (int j = 0; j < num; ++j) {
             const int nb = lst[j] - 1;
-            double dx = x1 - (double)pos[nb].x;
-            double dy = y1 - (double)pos[nb].y;
-            double dz = z1 - (double)pos[nb].z;
-            const int jtype = pos[nb].type;
+            double dx = x1 - xyz[3 * (size_t)nb];
+            double dy = y1 - xyz[3 * (size_t)nb + 1];
+            double dz = z1 - xyz[3 * (size_t)nb + 2];
+            const int jtype = type[nb];
             dx = minimg_f64(dx, p->box, p->ibox);
             dy = minimg_f64(dy, p->box, p->ibox);
             dz = minimg_f64(dz, p->box, p->ibox);
@@ -307,6 +312,37 @@ void orc_dsf(int32_t np, const orc_ainfo *pos, const float *q, const int32_t *nl
     *potential = 0.5 * pot;
     *e_coul = 0.5 * ec;
     if (virial) { virial[0] = 0.5 * vx; virial[1] = 0.5 * vy; virial[2] = 0.5 * vz; }
+}
+
+void orc_dsf(int32_t np, const orc_ainfo *pos, const float *q, const int32_t *nlist,
+             const int32_t *numneigh, int32_t neigh_alloc, const float *lj1, const float *lj2,
+             const float *lj3, const float *lj4, const orc_params *p, orc_coords *ff,
+             double *ff64, double *potential, double *e_coul, double *virial) {
+    double *xyz = (double *)malloc(sizeof(double) * 3 * (size_t)(np > 0 ? np : 1));
+    int32_t *type = (int32_t *)malloc(sizeof(int32_t) * (size_t)(np > 0 ? np : 1));
+    for (int i = 0; i < np; ++i) {
+        xyz[3 * (size_t)i] = pos[i].x;
+        xyz[3 * (size_t)i + 1] = pos[i].y;
+        xyz[3 * (size_t)i + 2] = pos[i].z;
+        type[i] = pos[i].type;
+    }
+    dsf_core(np, xyz, type, q, nlist, numneigh, neigh_alloc, lj1, lj2, lj3, lj4, p, ff, ff64,
+             potential, e_coul, virial);
+    free(type);
+    free(xyz);
+}
+
+/* The same pair sum on FP64 positions (the engine's state), for checking the engine's pair
+ * kernels on their own lists and coordinates. */
+void orc_dsf_x64(int32_t np, const double *xyz, const int32_t *type, const float *q,
+                 const int32_t *nlist, const int32_t *numneigh, int32_t neigh_alloc,
+                 const float *lj1, const float *lj2, const float *lj3, const float *lj4,
+                 const orc_params *p, double *ff64, double *potential, double *e_coul,
+                 double *virial) {
+    orc_coords *ff = (orc_coords *)malloc(sizeof(orc_coords) * (size_t)(np > 0 ? np : 1));
+    dsf_core(np, xyz, type, q, nlist, numneigh, neigh_alloc, lj1, lj2, lj3, lj4, p, ff, ff64,
+             potential, e_coul, virial);
+    free(ff);
 }
 
 /* lj_cut_coul_long_nonewton_NOMIC, Source/mod_force_nonbond.f90:300-400: the PME

@@ -67,6 +67,14 @@ void orc_dsf(int32_t np, const orc_ainfo *pos, const float *q, const int32_t *nl
              const int32_t *numneigh, int32_t neigh_alloc, const float *lj1, const float *lj2,
              const float *lj3, const float *lj4, const orc_params *p, orc_coords *ff,
              double *ff64, double *potential, double *e_coul, double *virial);
+/* orc_dsf on FP64 positions (xyz: x, y, z per atom; type per atom): the same arithmetic
+ * from the promotion of the coordinates on, so on real*4-representable positions it equals
+ * orc_dsf bit for bit.  ff64 receives the per-atom sums. */
+void orc_dsf_x64(int32_t np, const double *xyz, const int32_t *type, const float *q,
+                 const int32_t *nlist, const int32_t *numneigh, int32_t neigh_alloc,
+                 const float *lj1, const float *lj2, const float *lj3, const float *lj4,
+                 const orc_params *p, double *ff64, double *potential, double *e_coul,
+                 double *virial);
 void orc_coul_long(int32_t np, const orc_ainfo *pos, const float *q, const int32_t *nlist,
                    const int32_t *numneigh, int32_t neigh_alloc, const float *lj1,
                    const float *lj2, const float *lj3, const float *lj4, const orc_params *p,

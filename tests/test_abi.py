@@ -36,15 +36,14 @@ def test_binding_covers_header():
     assert sorted(_lib.SIGNATURES) == header_symbols()
 
 
-def test_params_struct_layout_matches_header():
+def test_params_struct_layout_matches_header(tmp_path):
     # offsets of the ctypes mirror vs a C compile of the header
     from molecular_simulation_code_amd._lib import SofiaParams, SofiaThermostat
     src = ('#include "sofia_hip.h"\n#include <stdio.h>\n#include <stddef.h>\n'
            'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(sofia_params), '
            'offsetof(sofia_params, alpha), offsetof(sofia_params, shake_maxiter), '
            'offsetof(sofia_params, neigh_alloc0), sizeof(sofia_thermostat));return 0;}\n')
-    exe = os.path.join(ROOT, "_work", "abi_probe")
-    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    exe = str(tmp_path / "abi_probe")
     c = exe + ".c"
     with open(c, "w") as f:
         f.write(src)

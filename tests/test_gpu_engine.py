@@ -282,6 +282,8 @@ def test_charge_by_type_records_keep_the_trajectory():
         try:
             e.set_tuning(lanes, 256, 4)
             e.init()
+            # the per-atom kernels really take the 12-byte records here
+            assert e.config()["record_bytes"] == (12 if lanes else 16), e.config()
             e.minimize(20)
             e.run(40)
             outs[lanes] = (e.state(), e.forces())
@@ -645,3 +647,57 @@ def test_full_size_mixed_forces_and_energies_track_fp64():
             + " ".join(f"{abs(em[c, t] - ed[c, t]) / max(abs(ed[c, t]), 1e-30):.1e}" for t in pot))
         assert rms <= 1e-5
         assert abs(um - ud) <= 1e-6 * abs(ud)
+
+
+def test_settle_failure_is_reported():
+    """A rigid water whose unconstrained step leaves SETTLE's reach (one H launched at
+    2 A/fs) makes the MD call fail with SOFIA_E_SHAKE instead of writing NaN positions."""
+    host = _engine()
+    from molecular_simulation_code_amd._lib import SofiaError
+    s = _sys(4, 300, 71)
+    h = 10 * s.n_gly + 1                      # first water's first H (0-based)
+    s.v[h] = np.array([0.0, 0.0, 2.0], np.float32)
+    e = host.Engine([host.ReplicaSpec(s, temp=300.0, gamma_ps=0.0, seed=1)], precision=host.PREC_FP64)
+    try:
+        with pytest.raises(SofiaError) as ei:
+            e.init()
+            e.run(2)
+        say("[settle failure]", ei.value)
+        assert ei.value.code == -6
+    finally:
+        e.close()
+
+
+def test_forcefield_change_mid_run_graph_equals_eager():
+    """New LJ coefficients uploaded into a running mixed-precision engine (charges by type,
+    12-byte records and a type-pair table) reach the captured step: graph and eager runs
+    stay bit-identical after the change, and differ from a run without it."""
+    host = _engine()
+    import ctypes as C
+    from molecular_simulation_code_amd._lib import check, lib, ptr
+    s = _sys(40, 500, 72)
+    lj = S.lj_tables()
+    mod = [np.ascontiguousarray(x * np.float32(1.05) if k < 2 else x, dtype=np.float32)
+           for k, x in enumerate(lj[:4])]
+    l14 = [np.ascontiguousarray(x, np.float64) for x in lj[4:]]
+    mass = np.ascontiguousarray(S.type_mass(), np.float64)
+    outs = {}
+    for tag, use_graph, change in (("graph", True, True), ("eager", False, True),
+                                   ("unchanged", True, False)):
+        e = host.Engine([host.ReplicaSpec(s, temp=300.0, gamma_ps=1.0, seed=9)],
+                        precision=host.PREC_MIXED)
+        try:
+            e.init()
+            e.minimize(20)
+            e.run(20, use_graph=use_graph)
+            assert e.config()["record_bytes"] == 12
+            if change:
+                check(lib().sofia_set_forcefield(e.ctx, *[ptr(a) for a in mod + l14 + [mass]]))
+                assert e.config()["record_bytes"] == 12   # the pair table was rebuilt
+            e.run(20, use_graph=use_graph)
+            outs[tag] = e.state()[0]
+        finally:
+            e.close()
+    assert np.array_equal(outs["graph"][0], outs["eager"][0])
+    assert np.array_equal(outs["graph"][1], outs["eager"][1])
+    assert not np.array_equal(outs["graph"][0], outs["unchanged"][0])

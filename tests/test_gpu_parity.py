@@ -135,6 +135,38 @@ def test_dsf_mixed(case, gpu_sys):
     assert g.e_coul == pytest.approx(ec, rel=1e-5)
 
 
+@pytest.mark.parametrize("r_close", [0.32, 0.41, 0.49, 0.55])
+def test_dsf_mixed_below_the_table_start(r_close):
+    """A water H placed r_close from another water's O (no LJ between them): below 0.5 A the
+    mixed kernel leaves its Coulomb table for the direct A&S form, and the pair still
+    matches the oracle (the reference evaluates DSF at every r < cut_coul)."""
+    SofiaGPU = _gpu()
+    s = S.make_system(0, 300, seed=23, params=S.Params(rcut=5.5, cut_coul=5.5, pad=1.0, alpha=0.25))
+    o, h = 0, 3 * 150 + 1                      # O of water 0, first H of water 150
+    d = s.pos[h].astype(np.float64) - s.pos[o]
+    d -= float(s.box) * np.round(d / float(s.box))
+    s.pos[h] = (s.pos[h] - d + r_close * d / np.linalg.norm(d)).astype(np.float32)
+    s.pos[h] = np.mod(s.pos[h], s.box).astype(np.float32)
+    g = SofiaGPU(s)
+    try:
+        g.neighbor_wrapper(0)
+        nl, nn, na = g.get_nlist()
+        prm = O.params_from(s.derived(), S.NTYPES)
+        _, ff64, pot, ec, _ = O.dsf(g.position, g.type, g.q, nl, nn, na, S.lj_tables(), prm)
+        g.potential = g.e_coul = 0.0
+        g.lj_cut_coul_dsf_nonewton_mixed(0)
+        gid = g.global_id
+        io, ih = int(np.flatnonzero(gid == o + 1)[0]), int(np.flatnonzero(gid == h + 1)[0])
+        for i in (io, ih):
+            e = np.linalg.norm(g.ff64[i] - ff64[i]) / np.linalg.norm(ff64[i])
+            say(f"[close pair r={r_close}] atom {i}: |F| {np.linalg.norm(ff64[i]):.1f}, rel err {e:.2e}")
+            assert e < FORCE_RTOL
+        assert rel_rms(g.ff64, ff64) < FORCE_RTOL
+        assert g.e_coul == pytest.approx(ec, rel=ENERGY_RTOL)
+    finally:
+        g.close()
+
+
 def test_dsf_mixed_kernels_agree(case, gpu_sys):
     """Every lanes-per-atom layout of the mixed kernel, and the compacting kernel (0), give
     the same per-atom sums: each pair is quantised to 2^-24 before an exact integer sum."""

@@ -50,6 +50,24 @@ def test_golden_dsf_bit_exact(golden):
     assert ec == pytest.approx(float(golden["dsf_ecoul"]), rel=1e-12)
 
 
+def test_dsf_on_fp64_positions_is_the_golden_dsf(golden):
+    """O.dsf_x64 (the checker of the engine, which keeps FP64 positions) on the golden
+    system's real*4 positions promoted to real*8: the reference's pair sum bit for bit,
+    and a shift of every position by a whole box leaves the forces unchanged to rounding."""
+    d = _derived(golden)
+    nl, nn, na = _list(golden, d)
+    prm = O.params_from(d, S.NTYPES)
+    lj = S.lj_tables()
+    _, ff64, pot, ec, vir = O.dsf(golden["pos"], golden["types"], golden["q"], nl, nn, na, lj, prm)
+    x = golden["pos"].astype(np.float64)
+    f2, pot2, ec2, vir2 = O.dsf_x64(x, golden["types"], golden["q"], nl, nn, na, lj, prm)
+    assert np.array_equal(f2, ff64) and pot2 == pot and ec2 == ec and np.array_equal(vir2, vir)
+    assert np.array_equal(f2.astype(np.float32), golden["dsf_f"])
+    f3, pot3, _, _ = O.dsf_x64(x + float(d["box"]), golden["types"], golden["q"], nl, nn, na, lj, prm)
+    assert np.sqrt(((f3 - ff64) ** 2).sum() / (ff64 ** 2).sum()) < 1e-9
+    assert pot3 == pytest.approx(pot, rel=1e-9)
+
+
 def test_golden_coul_long_bit_exact(golden):
     d = _derived(golden)
     nl, nn, na = _list(golden, d)
@@ -597,3 +615,21 @@ def test_settle_is_the_converged_shake_solution():
     dtsq = float(np.float32(float(dt) * float(dt)) * np.float32(k2a))
     fs = ((xs - x1f) * m[None, :, None] / dtsq).reshape(-1, 3)
     assert np.sqrt(((ff - fs) ** 2).sum() / (fs ** 2).sum()) < 1e-5
+
+
+def test_settle_rejects_a_water_beyond_reach():
+    """A step that carries an H farther than the rigid triangle can follow (the O above the
+    centre of mass by more than ra) has no SETTLE solution: the oracle names the water
+    instead of returning NaN positions, and leaves the reachable waters solvable."""
+    mo, mh, doh, dhh = 15.999, 1.008, 1.0, 1.633
+    th = np.arccos(1 - dhh ** 2 / (2 * doh ** 2))
+    geo = np.array([[0, 0, 0], [doh, 0, 0], [doh * np.cos(th), doh * np.sin(th), 0]])
+    x0 = np.stack([geo + 10.0, geo + 20.0])
+    x1 = x0.copy()
+    x1[1, 1, 2] += 3.0            # water 1: one H leaves the molecular plane by 3 A
+    assert np.isfinite(O.settle(x0[:1], x1[:1], mo, mh, doh, dhh)).all()
+    with pytest.raises(ValueError, match=r"waters \[1\]"):
+        O.settle(x0, x1, mo, mh, doh, dhh)
+    # the same water, moved slightly, is solvable
+    x1[1, 1, 2] = x0[1, 1, 2] + 0.05
+    assert np.isfinite(O.settle(x0, x1, mo, mh, doh, dhh)).all()
