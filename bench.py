@@ -9,7 +9,10 @@ latency of served MD-segment requests.
              each, same precision and tuning, resumed from the batched run's state):
              `batched_over_plain` is the gain of batching.
 * plain_c2 -- 1,200 glycine + 64,000 water (204,000 atoms), FP64 pair arithmetic, FP64
-             state; `plain_over_cpu` compares it with the CPU baseline.
+             state, device-resident engine (one system).
+* plain_dropin_c2 -- the same system through the drop-in C ABI, the reference's DSF step
+             routine by routine on host arrays (PCIe copies per call); `plain_over_cpu`
+             compares it with the CPU baseline on the same system and state.
 * batched -- 64 replicas, 5,000-60,000 atoms (log-uniform, seeded), glycine mole
              fraction 0.01-0.04, FP32 pair arithmetic with int64 fixed-point sums, FP64
              state; all replicas advance in one launch sequence (hipGraph-captured step).
@@ -17,7 +20,9 @@ latency of served MD-segment requests.
   (list maintenance, LJ+DSF pair forces, 1-4 pairs, bonded terms, Langevin, SHAKE).
 * served  -- Poisson arrivals of 500-step requests (3,000-150,000 atoms, 280-320 K),
              continuous batching, union-find cluster analysis every 100 steps.
-* cpu_baseline -- the C oracle (OpenMP) on the 13,500-atom system, same step.
+* cpu_baseline -- the C oracle (OpenMP) on the 204,000-atom system at the plain run's
+             equilibrated state, the same step, at least 20 steps; cpu_baseline_c1 the same
+             on the 13,500-atom system.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -37,20 +42,32 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Matom-steps/s batched vs plain @ force rel.err≤1e-5; p50/p99 segment ms"
 FLOP_PER_PAIR = 42.0          # per ordered in-cutoff list entry (84 per unordered pair)
-PEAK_FP32 = 157.3e12          # MI355X FP32 vector/matrix peak, FLOP/s
+PEAK_FP32 = 157.3e12          # MI355X FP32 vector peak with packed FP32 (v_pk_fma_f32), FLOP/s
+PEAK_FP32_UNPACKED = 78.6e12  # the same without packed FP32 (this build disables it)
 PEAK_FP64 = 78.6e12
+PAIR_KERNEL = "k_dsf_mixed pair kernel"
+
+
+def _profile_json(name):
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def traffic_bytes():
     """HBM bytes per pair-kernel launch on the batched workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by the calibration copy), or
     None when no such summary exists."""
-    path = os.path.join(ROOT, "profiles", "pair_kernel_batched64_traffic.json")
-    try:
-        with open(path) as f:
-            return json.load(f)["traffic_bytes_per_launch"]
-    except (OSError, ValueError, KeyError):
-        return None
+    d = _profile_json("pair_kernel_batched64_traffic.json")
+    return None if d is None else d.get("traffic_bytes_per_launch")
+
+
+def valu_counters():
+    """VALU busy % and the FP32 / FP64 / integer instruction mix of the pair kernel on the
+    batched workload, from the committed rocprofv3 counter summary, or None."""
+    return _profile_json("pair_kernel_batched64_valu.json")
 
 
 def log(*a):
@@ -119,11 +136,83 @@ def force_error_mixed_vs_fp64(system):
     return float(np.sqrt(((fm - f64) ** 2).sum() / (f64 ** 2).sum()))
 
 
-def cpu_baseline(system, budget_s=12.0):
-    """Oracle MD steps (list rebuild when triggered, DSF pair forces, 1-4 pairs, bonded,
-    SHAKE, velocity Verlet) on the host cores; returns (atom-steps/s, steps, threads)."""
+def cpu_model():
+    """The host CPU's model name and logical CPU count, as /proc/cpuinfo reports them."""
+    name, n = "unknown", 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and name == "unknown":
+                    name = line.split(":", 1)[1].strip()
+                if line.startswith("processor"):
+                    n += 1
+    except OSError:
+        pass
+    return name, n
+
+
+def wrapped_f32(x, box):
+    """FP64 positions wrapped into (0, box] as pbc() leaves them, in real*4."""
+    p = np.mod(x, float(box)).astype(np.float32)
+    p = np.where(p <= 0, p + np.float32(box), p).astype(np.float32)
+    return np.where(p > np.float32(box), p - np.float32(box), p).astype(np.float32)
+
+
+def dropin_rate(system, state, steps, warmup):
+    """The plain GPU drop-in: the reference's DSF time loop (Source/mod_integrate.f90:270-308,
+    force_wrapper_DSF_NOMIC Source/mod_force.f90:510-554) with every routine of the step
+    replaced by its C-ABI drop-in on host-owned real*4 arrays, as a Fortran host would call
+    them through the ISO_C_BINDING shim: nve_v, nve_x, pbc, check_neighbor (and
+    neighbor_wrapper when triggered), the FP64 DSF pair kernel, the 1-4 pair, the bonded
+    terms, shake, nve_v.  Each call copies its arrays over PCIe, as the reference's MIC
+    offload did.  Returns (atom-steps/s, list builds during the timed steps)."""
+    from molecular_simulation_code_amd.host import SofiaGPU
+    s = system.copy()
+    s.pos = wrapped_f32(state[0], s.box)
+    s.v = np.ascontiguousarray(state[1], np.float32)
+    g = SofiaGPU(s)
+    dthalf = float(s.derived()["dthalf"])
+    try:
+        def forces():
+            g.potential = g.e_coul = g.e_coul_14 = g.pot_14 = 0.0
+            g.e_bond = g.e_angle = g.e_dihed = g.e_improper = 0.0
+            f = g.lj_cut_coul_dsf_nonewton(0).astype(np.float64)
+            f += g.pair14_dsf()
+            f += g.bonded_forces()
+            g.ff = f.astype(np.float32)          # the ff + ffh merge
+            g.shake(1)
+
+        def step():
+            g.nve_v(dthalf)
+            g.nve_x()
+            g.pbc()
+            if g.check_neighbor():
+                g.neighbor_wrapper()
+            forces()
+            g.nve_v(dthalf)
+
+        g.neighbor_wrapper(0)
+        forces()
+        for _ in range(max(1, warmup)):
+            step()
+        b0 = g.nn_builds
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        el = time.perf_counter() - t0
+        return s.np * steps / el, g.nn_builds - b0
+    finally:
+        g.close()
+
+
+def cpu_baseline(system, budget_s=12.0, min_steps=20, max_steps=1000):
+    """Oracle MD steps (list rebuild when triggered, DSF pair forces, 1-4 pairs, bond, angle,
+    torsion and improper terms, SHAKE, velocity Verlet) on the host cores: at least
+    min_steps steps, then until budget_s has passed or max_steps are done.  Returns
+    (atom-steps/s, steps, threads)."""
     from oracle import oracle as O
     from molecular_simulation_code_amd import system as S
+    from molecular_simulation_code_amd.host import torsion_list
     s = system.copy()
     d = s.derived()
     prm = O.params_from(d, S.NTYPES)
@@ -131,9 +220,12 @@ def cpu_baseline(system, budget_s=12.0):
     mass = S.type_mass()
     kb = np.array([c[0] for c in S.BOND_COEFF]); r0 = np.array([c[1] for c in S.BOND_COEFF])
     ka = np.array([c[0] for c in S.ANGLE_COEFF]); th = np.radians([c[1] for c in S.ANGLE_COEFF])
+    tk = np.array([c[0] for c in S.DIHED_COEFF]); tm = np.array([c[1] for c in S.DIHED_COEFF], float)
+    ts = np.array([c[2] for c in S.DIHED_COEFF], float)
+    ik = np.array([c[0] for c in S.IMPRO_K_CHI0]); ic = np.array([c[1] for c in S.IMPRO_K_CHI0])
     rn, nT, nD = O.cell_grid(d["box"], d["rcut"], d["cut_coul"])
     cnum = O.neighcell(nT, nD)
-    st = {}
+    st = {"tors": torsion_list(s).copy(), "impro": s.improlist.copy()}
 
     def rebuild():
         tt, perm, start, endp, atombin = O.sort_cells(s.pos, s.types, d["box"], rn, nT, nD)
@@ -145,6 +237,8 @@ def cpu_baseline(system, budget_s=12.0):
         s.bondlist[:, :2] = tt[s.bondlist[:, :2] - 1]
         s.anglelist[:, :3] = tt[s.anglelist[:, :3] - 1]
         s.dihedlist[:, :4] = tt[s.dihedlist[:, :4] - 1]
+        st["tors"][:, :4] = tt[st["tors"][:, :4] - 1]
+        st["impro"][:, :4] = tt[st["impro"][:, :4] - 1]
         sa = s.shake_atoms
         s.shake_atoms = np.where(sa > 0, tt[np.maximum(sa, 1) - 1], 0).astype(np.int32)
         st["nl"], st["nn"], st["na"] = O.build_neighbor(s.pos, s.types, atombin, start, endp, cnum,
@@ -157,7 +251,9 @@ def cpu_baseline(system, budget_s=12.0):
         fh, _, _ = O.pair14_dsf(s.dihedlist, s.pos, s.types, s.q, lj, prm)
         fb, _ = O.bond_harmonic(s.bondlist, s.pos, s.types, kb, r0, d["box"], d["ibox"])
         fa, _ = O.angle_harmonic(s.anglelist, s.pos, s.types, ka, th, d["box"], d["ibox"])
-        f = (ff.astype(np.float64) + fh + fb + fa).astype(np.float32)
+        ft, _ = O.torsion_cos(st["tors"], s.pos, s.types, tk, tm, ts, d["box"], d["ibox"])
+        fi, _, _ = O.improper_harmonic(st["impro"], s.pos, s.types, ik, ic, d["box"], d["ibox"])
+        f = (ff.astype(np.float64) + fh + fb + fa + ft + fi).astype(np.float32)
         f, _, _, _ = O.shake(1, s.pos, s.types, s.v, f, mass, d["dt"], S.KCAL2AMU, d["box"],
                              d["ibox"], s.shake_num, s.shake_atoms, s.shake_bond,
                              float(S.SHAKE_TOL), S.SHAKE_MAXITER)
@@ -182,7 +278,7 @@ def cpu_baseline(system, budget_s=12.0):
         f = step(f)
         n += 1
         el = time.perf_counter() - t0
-        if el > budget_s or n >= 200:
+        if n >= max_steps or (n >= min_steps and el > budget_s):
             break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return s.np * n / el, n, threads
@@ -379,9 +475,16 @@ def main():
     ms_p, pairs_p = eng.profile_nonbond(10)
     temp_p = float(eng.temperatures()[0])
     phases_p = eng.profile_step(30)
+    state_c2 = eng.state()[0]
     log(f"[plain] {plain_rate / 1e6:.1f} Matom-steps/s  ({1e3 * dt_plain / K:.3f} ms/step, "
         f"pair kernel {ms_p:.3f} ms, {pairs_p / plain_sys.np:.1f} pairs/atom in cutoff, T {temp_p:.1f} K)")
     eng.close()
+
+    # ---------------------------------------------------------------- plain drop-in: C2 through the C ABI
+    kd = max(10, min(K, 100))
+    dropin, dropin_builds = dropin_rate(plain_sys, state_c2, kd, max(1, W // 2))
+    log(f"[plain drop-in] {dropin / 1e6:.1f} Matom-steps/s ({kd} steps, {dropin_builds} list builds, "
+        "host arrays, PCIe copies per call)")
 
     # ---------------------------------------------------------------- batched: 64 replicas mixed
     natoms = sum(r.np for r in reps)
@@ -454,16 +557,28 @@ def main():
             f"{srv.rebatches} re-batches, {srv.segments_run} segments")
 
     # ---------------------------------------------------------------- CPU baseline
-    cpu = None
+    cpu = cpu_c1 = None
     if not args.no_cpu:
-        rate, nsteps, threads = cpu_baseline(small, args.cpu_seconds)
+        model, ncpu = cpu_model()
+        c2 = plain_sys.copy()
+        c2.pos = wrapped_f32(state_c2[0], c2.box)
+        c2.v = np.ascontiguousarray(state_c2[1], np.float32)
+        rate, nsteps, threads = cpu_baseline(c2, args.cpu_seconds, min_steps=20)
         cpu = {"value": rate / 1e6, "unit": "Matom-steps/s", "cores": threads, "kind": "port",
-               "sample": f"C oracle (OpenMP pair and list loops), 13,500-atom glycine/water, "
-                         f"{nsteps} full MD steps"}
-        log(f"[cpu] {rate / 1e6:.3f} Matom-steps/s on {threads} threads")
+               "cpu_model": model, "host_logical_cpus": ncpu,
+               "sample": f"C oracle (OpenMP pair and list loops; DSF, 1-4, bond, angle, torsion, "
+                         f"improper, SHAKE, velocity Verlet, list rebuilds when triggered) on the "
+                         f"204,000-atom system at the plain run's equilibrated state, {nsteps} MD "
+                         "steps"}
+        log(f"[cpu] C2: {rate / 1e6:.3f} Matom-steps/s on {threads} threads ({model})")
+        r1, n1, _ = cpu_baseline(small, args.cpu_seconds, min_steps=20, max_steps=1000)
+        cpu_c1 = {"value": r1 / 1e6, "unit": "Matom-steps/s", "steps": n1,
+                  "sample": "13,500-atom glycine/water (generated state), same step"}
+        log(f"[cpu] C1: {r1 / 1e6:.3f} Matom-steps/s ({n1} steps)")
 
     flops = FLOP_PER_PAIR * pairs_b
     achieved = flops / (ms_b * 1e-3)
+    valu = valu_counters()
     line = {
         "metric": METRIC,
         "value": round(batched_rate / 1e6, 3),
@@ -507,19 +622,29 @@ def main():
         },
         "batched_over_plain": round(batched_rate / per_launch_rate, 3),
         "batched_over_plain_c2_fp64": round(batched_rate / plain_rate, 3),
-        "plain_over_cpu": None if cpu is None else round(plain_rate / 1e6 / cpu["value"], 1),
+        # the plain GPU drop-in (C2 through the C ABI, host arrays) over the CPU port on C2
+        "plain_dropin_c2": {"value": round(dropin / 1e6, 3), "unit": "Matom-steps/s",
+                            "steps": kd, "list_builds": int(dropin_builds),
+                            "workload": "204,000-atom glycine/water, FP64: the reference's DSF step "
+                                        "with each routine replaced by its C-ABI drop-in on host "
+                                        "real*4 arrays (PCIe copies per call included)"},
+        "plain_over_cpu": None if cpu is None else round(dropin / 1e6 / cpu["value"], 1),
+        "plain_engine_c2_over_cpu": None if cpu is None else round(plain_rate / 1e6 / cpu["value"], 1),
+        "cpu_baseline_c1": cpu_c1,
         "p50_ms": p50,
         "p99_ms": p99,
         "force_rel_rms_mixed_vs_fp64": ferr,
         "list_rebuilds_per_replica_mean": float(np.mean(rebuilds)),
         "list_rebuild_window_steps": K + max(1, W),
         "hbm_copy_GBps": round(2.0 * (1 << 30) / (ms_copy * 1e-3) / 1e9, 1),
-        # priced against the compute roofline (MI355X FP32 peak); the counters put the
-        # kernel's actual limit in the vector-memory data path (texture data unit busy)
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic_bytes(),
-                     "limiter": "vector-memory data path (gathers)",
-                     "kernel": "k_dsf_mixed pair kernel", "ms_per_launch": ms_b,
+        # the pair kernel is FP32 vector-ALU work (no matrix cores): priced against the
+        # packed FP32 vector peak (frac) and the non-packed one this build issues (frac_unpacked)
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "peak_unpacked": PEAK_FP32_UNPACKED / 1e12,
+                     "frac_unpacked": achieved / PEAK_FP32_UNPACKED, "traffic": traffic_bytes(),
+                     "valu_counters": valu,
+                     "kernel": PAIR_KERNEL, "ms_per_launch": ms_b,
                      "pairs_in_cutoff_per_launch": pairs_b, "flop_per_pair": FLOP_PER_PAIR},
         "cpu_baseline": cpu,
     }

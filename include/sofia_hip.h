@@ -187,6 +187,15 @@ int sofia_correct_bonds(sofia_ctx *ctx, int32_t which, sofia_coords *ff, double 
 /* The 1-4 pair inside dihedral_opls_DSF, Source/mod_force_dihed.f90:1003-1048;
  * ffh (np) receives the pair forces, e_coul_14/pot_14 are ADDED to. */
 int sofia_pair14_dsf(sofia_ctx *ctx, sofia_coords *ffh, double *e_coul_14, double *pot_14);
+/* The bonded terms of force_wrapper_DSF_NOMIC (Source/mod_force.f90:510-554):
+ * bond_harmonic (Source/mod_force_bond.f90:6-75), angle_harmonic
+ * (Source/mod_force_angle.f90:10-122), the torsion series of the dihedral routine and the
+ * improper term set by sofia_set_impropers / _cvff (improper_harmonic_lammps14,
+ * Source/mod_force_dihed.f90:1546-1724), with the coefficients of sofia_set_bonded_coeffs,
+ * on the uploaded real*4 positions: ff (np) receives their summed forces, the energies are
+ * ADDED to.  Not on a context running the engine, which evaluates these itself. */
+int sofia_bonded_forces(sofia_ctx *ctx, sofia_coords *ff, double *e_bond, double *e_angle,
+                        double *e_dihed, double *e_improper);
 /* nve_v(dt_partial) / nve_x / pbc(step), Source/mod_ensemble.f90:11-87,
  * Source/mod_pbc.f90:6-39, on the real*4 state with force ff (np). */
 int sofia_nve_v(sofia_ctx *ctx, float dt_partial, const sofia_coords *ff, double *ke);

@@ -84,6 +84,7 @@ SIGNATURES = {
     "sofia_nonbond_wait": [_vp] * 5,
     "sofia_correct_bonds": [_vp, _i32, _vp, _vp, _vp],
     "sofia_pair14_dsf": [_vp] * 4,
+    "sofia_bonded_forces": [_vp] * 6,
     "sofia_nve_v": [_vp, _f, _vp, _vp],
     "sofia_nve_x": [_vp],
     "sofia_pbc": [_vp],

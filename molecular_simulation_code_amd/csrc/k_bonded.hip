@@ -466,6 +466,16 @@ __global__ void k_improper_cvff(int32_t items, const int32_t *__restrict__ il,
     wave_add(to_efix(e), live, r, eacc, E_IMPRO);
 }
 
+// real*4 positions promoted to an FP64 SoA copy, the input of the bonded kernels above
+__global__ void k_pos_to_xd(int32_t n, const float4 *__restrict__ pos4, double *__restrict__ xd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos4[i];
+    xd[i] = p.x;
+    xd[(size_t)n + i] = p.y;
+    xd[2 * (size_t)n + i] = p.z;
+}
+
 // ------------------------------------------------------------------ one workgroup per bundle
 // A bundle holds whole molecules (connected components of the bonded items) of one
 // replica, at most BUNDLE_SLOTS atoms, its items sorted by kind so that a wavefront mostly
@@ -574,6 +584,35 @@ void launch_pair14(sofia_ctx *c, long long *fout) {
     hipLaunchKernelGGL(k_pair14<false>, dim3(nblocks(c->nd, 256)), dim3(256), 0, c->stream, c->nd,
                        c->dihedlist, c->a.pos4, c->a.xd, c->a.q, c->rep, c->reps_d, c->lj14, c->dp,
                        c->N, fout, c->eacc);
+}
+
+// The drop-in form of the bonded terms: every bond, angle, torsion and improper of the
+// uploaded lists (rigid waters included, as the reference's bond_harmonic / angle_harmonic
+// loop over them) on the real*4 positions, promoted into the gather buffer's FP64 copy
+// (free outside a sort), forces into fout.
+void launch_bonded_dropin(sofia_ctx *c, long long *fout) {
+    const int n = c->N;
+    double *xd = c->b.xd;
+    hipLaunchKernelGGL(k_pos_to_xd, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, n, c->a.pos4, xd);
+    if (c->nb && c->bond_k)
+        hipLaunchKernelGGL(k_bond, dim3(nblocks(c->nb, 256)), dim3(256), 0, c->stream, c->nb,
+                           c->bondlist, xd, c->rep, c->reps_d, c->bond_k, c->bond_r0, n, fout, c->eacc);
+    if (c->na && c->angle_k)
+        hipLaunchKernelGGL(k_angle, dim3(nblocks(c->na, 256)), dim3(256), 0, c->stream, c->na,
+                           c->anglelist, xd, c->rep, c->reps_d, c->angle_k, c->angle_th0, n, fout,
+                           c->eacc);
+    if (c->ntor && c->tors_k)
+        hipLaunchKernelGGL(k_torsion, dim3(nblocks(c->ntor, 256)), dim3(256), 0, c->stream, c->ntor,
+                           c->torsions, xd, c->rep, c->reps_d, c->tors_k, c->tors_n, c->tors_shift,
+                           c->tors_cs, n, fout, c->eacc);
+    if (c->nimp && c->impro_style == 0)
+        hipLaunchKernelGGL(k_improper, dim3(nblocks(c->nimp, 256)), dim3(256), 0, c->stream, c->nimp,
+                           c->improlist, xd, c->rep, c->reps_d, c->impro_k, c->impro_chi0, n, fout,
+                           c->eacc);
+    else if (c->nimp)
+        hipLaunchKernelGGL(k_improper_cvff, dim3(nblocks(c->nimp, 256)), dim3(256), 0, c->stream,
+                           c->nimp, c->improlist, xd, c->rep, c->reps_d, c->impro_k, c->impro_dn, n,
+                           fout, c->eacc);
 }
 
 void launch_bonded_engine(sofia_ctx *c) {

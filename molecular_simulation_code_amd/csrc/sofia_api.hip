@@ -1205,6 +1205,29 @@ int sofia_pair14_dsf(sofia_ctx *c, sofia_coords *ffh, double *ec14, double *pot1
     return SOFIA_OK;
 }
 
+int sofia_bonded_forces(sofia_ctx *c, sofia_coords *ff, double *e_bond, double *e_angle,
+                        double *e_dihed, double *e_improper) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_ff || !c->bond_k) return fail(SOFIA_E_STATE, "bonded coefficients not set");
+    if (c->engine_ready) return fail(SOFIA_E_STATE, "a running engine evaluates its own bonded terms");
+    launch_zero_forces(c, c->fscratch, true);
+    launch_bonded_dropin(c, c->fscratch);
+    float *st = staging_f32(c);
+    launch_fix_to_coords(c, c->fscratch, st);
+    std::vector<long long> e;
+    TRY(read_energies(c, e));
+    if (ff) TRY(download(ff, st, sizeof(float) * 3 * (size_t)c->N));
+    double sb = 0, sa = 0, sd = 0, si = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        sb += E(e, r, E_BOND); sa += E(e, r, E_ANGLE); sd += E(e, r, E_DIHED); si += E(e, r, E_IMPRO);
+    }
+    if (e_bond) *e_bond += sb;
+    if (e_angle) *e_angle += sa;
+    if (e_dihed) *e_dihed += sd;
+    if (e_improper) *e_improper += si;
+    return SOFIA_OK;
+}
+
 int sofia_nve_v(sofia_ctx *c, float dt_partial, const sofia_coords *ff, double *ke) {
     if (!c || !ff) return fail(SOFIA_E_ARG, "null argument");
     if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");

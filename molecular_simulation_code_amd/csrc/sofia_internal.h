@@ -214,6 +214,7 @@ void launch_count_pairs(sofia_ctx *c, unsigned long long *out);
 void launch_correct(sofia_ctx *c, int which, long long *fout);
 void launch_pair14(sofia_ctx *c, long long *fout);
 void launch_bonded_engine(sofia_ctx *c);
+void launch_bonded_dropin(sofia_ctx *c, long long *fout);
 int build_bonded_bundles(sofia_ctx *c);   // host: molecules of the engine's bonded items
 void drop_bonded_bundles(sofia_ctx *c);
 void launch_dsf_mixed(sofia_ctx *c, bool energies, float *ff32, bool from_xd);

@@ -135,6 +135,10 @@ class SofiaGPU:
         self.neigh_alloc = S.NEIGH_ALLOC0
         self.nn_builds = 0
         self._upload_all()
+        # bond / angle / torsion / improper coefficients for bonded_forces()
+        self._keep += _bonded_coeffs(h, torsion_list(system))
+        self._keep += _impropers(h, [system.improlist])
+        self.e_bond = self.e_angle = self.e_dihed = self.e_improper = 0.0
 
     # ------------------------------------------------------------------ plumbing
     def close(self):
@@ -326,6 +330,21 @@ class SofiaGPU:
         self.ffh = ff
         self.e_coul_14 += e1.value
         self.pot_14 += e2.value
+        return ff
+
+    def bonded_forces(self):
+        """bond_harmonic + angle_harmonic + torsions + improper_harmonic_lammps14 of
+        force_wrapper_DSF_NOMIC (Source/mod_force.f90:510-554) -> their summed forces;
+        e_bond / e_angle / e_dihed / e_improper are added to."""
+        self._put_pos()
+        ff = np.empty((self.np, 3), f32)
+        e = [C.c_double(0.0) for _ in range(4)]
+        check(lib().sofia_bonded_forces(self.ctx, ptr(ff), *[C.byref(x) for x in e]))
+        self.ff_bonded = ff
+        self.e_bond += e[0].value
+        self.e_angle += e[1].value
+        self.e_dihed += e[2].value
+        self.e_improper += e[3].value
         return ff
 
     def nve_v(self, dt_partial: float):

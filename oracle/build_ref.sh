@@ -4,8 +4,8 @@
 # library produces golden vectors in this development container.
 #
 # Built: global, mod_minimg, mod_force_nonbond, mod_force_correction, mod_force_dihed,
-#        mod_force_bond, mod_force_angle, mod_pbc, mod_force_impro_amber  (each compiles
-#        unmodified with amdflang).
+#        mod_force_bond, mod_force_angle, mod_pbc, mod_force_impro_amber,
+#        mod_force_dihed_amber  (each compiles unmodified with amdflang).
 # Not buildable here (would need stand-ins for files the image lacks, so they are not built):
 #   mod_neighbor.f90  -- `use ifport` (Intel Fortran portability module)
 #   mod_shake.f90     -- Intel's `call abort('message')` extension
@@ -24,11 +24,11 @@ mkdir -p "$out/obj"
 FFLAGS="-O2 -fPIC -fopenmp -ffp-contract=off -w"
 cd "$out/obj"
 for m in global mod_minimg mod_force_nonbond mod_force_correction mod_force_dihed \
-         mod_force_bond mod_force_angle mod_pbc mod_force_impro_amber; do
+         mod_force_bond mod_force_angle mod_pbc mod_force_impro_amber mod_force_dihed_amber; do
   "$FC" $FFLAGS -c "$src/$m.f90" -o "$m.o"
 done
 "$FC" $FFLAGS -c "$here/ref_driver.f90" -o ref_driver.o
 "$FC" -shared -fopenmp -o "$out/libsofia_ref.so" ref_driver.o global.o mod_minimg.o \
   mod_force_nonbond.o mod_force_correction.o mod_force_dihed.o mod_force_bond.o \
-  mod_force_angle.o mod_pbc.o mod_force_impro_amber.o
+  mod_force_angle.o mod_pbc.o mod_force_impro_amber.o mod_force_dihed_amber.o
 echo "built $out/libsofia_ref.so"

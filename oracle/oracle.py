@@ -491,6 +491,19 @@ class Ref:
                                  P(mult), P(f), C.byref(e), P(vir))
         return f, e.value, vir
 
+    def dihed_amber(self, torsions, k, mult, shift_deg):
+        """dihed_PME_amber with the 1-4 LJ tables zeroed: (torsion forces, e_dihedral).
+        Leaves the lj14 tables of module global zeroed."""
+        lst = np.ascontiguousarray(torsions, _i)
+        k = np.ascontiguousarray(k, _d)
+        mult = np.ascontiguousarray(mult, _d)
+        sh = np.ascontiguousarray(shift_deg, _d)
+        f = np.zeros((self.n, 3), _f)
+        e = C.c_double(0.0)
+        self.L.ref_dihed_amber(C.c_int32(len(lst)), P(lst), C.c_int32(len(k)), P(k), P(mult), P(sh),
+                               P(f), C.byref(e))
+        return f, e.value
+
     def pbc(self, pos):
         xyz = np.ascontiguousarray(pos, _f).copy()
         self.L.ref_pbc(P(xyz))

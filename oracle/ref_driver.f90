@@ -3,7 +3,7 @@
 ! oracle/build_ref.sh compiles this file together with the reference modules where they
 ! lie (/root/reference/Source: global, mod_minimg, mod_force_nonbond,
 ! mod_force_correction, mod_force_dihed, mod_force_bond, mod_force_angle, mod_pbc,
-! mod_force_impro_amber) into
+! mod_force_impro_amber, mod_force_dihed_amber) into
 ! oracle/_ref/libsofia_ref.so.  The entry points below only copy caller arrays into
 ! `module global`, call one reference subroutine unchanged, and copy its outputs back,
 ! so the golden-vector script in tests/golden/ can compare the C oracle with the
@@ -18,6 +18,7 @@ module sofia_ref_driver
   use mod_force_angle
   use mod_pbc
   use mod_force_impro_amber
+  use mod_force_dihed_amber
   implicit none
 contains
 
@@ -278,6 +279,38 @@ contains
     ei = e_improper
     vir(1) = v_improx; vir(2) = v_improy; vir(3) = v_improz
   end subroutine ref_improper_cvff
+
+  ! dihed_PME_amber (Source/mod_force_dihed_amber.f90:7-252) with the 1-4 LJ tables zeroed,
+  ! which leaves the torsion series alone in ff_dihed and e_dihedral.  The per-type
+  ! tables are filled as the topology reader fills them (Source/mod_readfile.f90:405-410):
+  ! k_dihed, multiplicity, cos_shift / sin_shift of the shift in degrees.
+  subroutine ref_dihed_amber(n, lst, ntt, k, mult, shift_deg, fout, ed) &
+       bind(c, name="ref_dihed_amber")
+    integer(c_int), value :: n, ntt
+    integer(c_int), intent(in) :: lst(5, n)
+    real(c_double), intent(in) :: k(ntt), mult(ntt), shift_deg(ntt)
+    real(c_float), intent(out) :: fout(3, np)
+    real(c_double), intent(out) :: ed
+    integer :: t
+    if (allocated(dihedlist)) deallocate(dihedlist)
+    allocate(dihedlist(5, n)); dihedlist = lst; numdihed = n
+    if (allocated(k_dihed)) deallocate(k_dihed, shift_dihed, cos_shift, sin_shift, multiplicity)
+    allocate(k_dihed(ntt), shift_dihed(ntt), cos_shift(ntt), sin_shift(ntt), multiplicity(ntt))
+    do t = 1, ntt
+       k_dihed(t) = k(t)
+       shift_dihed(t) = shift_deg(t)
+       cos_shift(t) = cos(acos(-1.0d0)*shift_deg(t)/180.0)
+       sin_shift(t) = sin(acos(-1.0d0)*shift_deg(t)/180.0)
+       multiplicity(t) = mult(t)
+    end do
+    lj14_1 = 0.0d0; lj14_2 = 0.0d0; lj14_3 = 0.0d0; lj14_4 = 0.0d0
+    call zero_forces()
+    e_dihedral = 0.0d0; pot_14 = 0.0d0
+    v_dihedx = 0.0d0; v_dihedy = 0.0d0; v_dihedz = 0.0d0
+    call dihed_PME_amber()
+    call get_forces(ff_dihed, fout)
+    ed = e_dihedral
+  end subroutine ref_dihed_amber
 
   ! pbc (Source/mod_pbc.f90:6-39) on xyz in place.
   subroutine ref_pbc(xyz) bind(c, name="ref_pbc")

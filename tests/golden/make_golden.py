@@ -13,6 +13,7 @@ the outputs of the reference Fortran routines on it:
 * bond_harmonic, angle_harmonic        forces, energies
 * improper_harmonic_lammps14           forces, e_improper, virial (two improper types)
 * impro_amber_force (cvff form)        forces, e_improper, virial (two improper types)
+* dihed_PME_amber (1-4 LJ zeroed)      torsion forces, e_dihedral (six torsion types)
 * pbc                                  wrapped coordinates
 
 The Verlet list these routines consume is the oracle's (mod_neighbor.f90 needs Intel's
@@ -36,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 from oracle import oracle as O  # noqa: E402
 from molecular_simulation_code_amd import system as S  # noqa: E402
+from molecular_simulation_code_amd.host import torsion_list  # noqa: E402
 
 CASES = {
     # name: (n_gly, n_wat, seed, rcut, cut_coul, pad, alpha, gewald)
@@ -50,6 +52,10 @@ IMPRO_CHI0 = np.array([0.0, 0.35])
 CVFF_K = np.array([1.1, 0.7])
 CVFF_SIGN = np.array([-1, 1], np.int32)
 CVFF_MULT = np.array([2, 3], np.int32)
+# torsion types (K, n, shift deg): the topology's two and four more
+TORS_K = np.array([c[0] for c in S.DIHED_COEFF] + [0.8, 0.5, 1.3, 0.4])
+TORS_MULT = np.array([c[1] for c in S.DIHED_COEFF] + [1, 2, 3, 0], float)
+TORS_SHIFT = np.array([c[2] for c in S.DIHED_COEFF] + [30.0, 90.0, -45.0, 60.0])
 
 
 def pair_digest(nlist, numneigh, neigh_alloc, ids=None):
@@ -119,6 +125,13 @@ def main():
         g["cvff_k"], g["cvff_sign"], g["cvff_mult"] = CVFF_K, CVFF_SIGN, CVFF_MULT
         g["cvff_f"], g["cvff_e"], g["cvff_vir"] = R.improper_cvff(g["improlist"], CVFF_K, CVFF_SIGN,
                                                                  CVFF_MULT)
+        # torsions in geometric order a-b-c-d through dihed_PME_amber, the types cycling
+        # over the topology's and four with shifts off 0 / 180 degrees and n = 0..3
+        tl = torsion_list(s).copy()
+        tl[:, :4] = g["tt"][tl[:, :4] - 1]
+        tl[:, 4] = (np.arange(len(tl)) % len(TORS_K)) + 1
+        g["tors_list"], g["tors_k"], g["tors_mult"], g["tors_shift"] = tl, TORS_K, TORS_MULT, TORS_SHIFT
+        g["tors_f"], g["tors_e"] = R.dihed_amber(tl, TORS_K, TORS_MULT, TORS_SHIFT)
         rng = np.random.default_rng(3)
         x = g["pos"].copy()
         k = len(x) // 8

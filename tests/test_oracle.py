@@ -129,6 +129,46 @@ def test_golden_bond_angle_pbc(golden):
     assert np.array_equal(O.pbc(golden["pbc_in"], golden["types"], d["box"]), golden["pbc_out"])
 
 
+def test_golden_torsion_is_dihed_pme_amber(golden):
+    """O.torsion_cos against the reference's dihed_PME_amber (1-4 LJ zeroed) on six torsion
+    types: the topology's two and shifts of 30, 90, -45 and 60 degrees with n = 0..3, so a
+    wrong phase sign, shift convention or multiplicity recurrence fails.  The reference
+    stores forces in real*4."""
+    d = _derived(golden)
+    f, e = O.torsion_cos(golden["tors_list"], golden["pos"], golden["types"], golden["tors_k"],
+                         golden["tors_mult"], golden["tors_shift"], d["box"], d["ibox"])
+    ref = golden["tors_f"].astype(np.float64)
+    assert (ref ** 2).sum() > 0
+    assert np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()) < 1e-6
+    assert np.abs(f.astype(np.float32) - golden["tors_f"]).max() <= 4 * np.abs(ref).max() * 2.0 ** -24
+    assert e == pytest.approx(float(golden["tors_e"]), rel=1e-12)
+
+
+@pytest.mark.parametrize("phi_deg", [0.0, 60.0, 90.0, 180.0, -120.0])
+def test_torsion_closed_form(phi_deg):
+    """A single torsion built at a chosen dihedral phi: E = K [1 + cos(n phi - shift)] for
+    n = 0..3 and shifts 0, 30, 90, 180 (phi = 0 cis, 180 trans, the IUPAC sign), zero net
+    force and zero net torque."""
+    phi = np.radians(phi_deg)
+    x = np.array([[10.0, 11.0, 10.0], [10.0, 10.0, 10.0], [11.5, 10.0, 10.0],
+                  [11.5, 10.0 + np.cos(phi), 10.0 + np.sin(phi)]], np.float32)
+    types = np.ones(4, np.int32)
+    box = np.float32(40.0)
+    for n in (0, 1, 2, 3):
+        for sh in (0.0, 30.0, 90.0, 180.0):
+            tl = np.array([[1, 2, 3, 4, 1]], np.int32)
+            f, e = O.torsion_cos(tl, x, types, np.array([0.7]), np.array([float(n)]),
+                                 np.array([sh]), box, np.float32(1.0) / box)
+            xf = x.astype(np.float64)
+            b1, b2, b3 = xf[1] - xf[0], xf[2] - xf[1], xf[3] - xf[2]
+            n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+            ph = np.arctan2(np.dot(np.cross(n1, n2), b2) / np.linalg.norm(b2), np.dot(n1, n2))
+            assert ph == pytest.approx(np.radians(phi_deg) if phi_deg != 180.0 else abs(ph), abs=1e-5)
+            assert e == pytest.approx(0.7 * (1.0 + np.cos(n * ph - np.radians(sh))), abs=1e-6)
+            assert np.abs(f.sum(0)).max() < 1e-9
+            assert np.abs(np.cross(xf, f).sum(0)).max() < 1e-8
+
+
 # ------------------------------------------------------------------ list: brute force and invariants
 def test_list_equals_stencil_restricted_bruteforce(golden):
     d = _derived(golden)
