@@ -367,6 +367,8 @@ def profile_run(args, workers):
             eng.set_tuning(int(knobs[1]), 256, int(knobs[2]))
             if len(knobs) > 3:
                 eng.set_bonded_mode(int(knobs[3]))
+            if len(knobs) > 4:
+                eng.set_graph_steps(int(knobs[4]))
             # the timed run's preparation, with the equilibration stepped eagerly so that the
             # profiled process captures one graph only
             eng.init()
@@ -414,8 +416,8 @@ def main():
                     help="short run of the batched workload for rocprofv3: a few steps, the pair "
                          "kernel and the 1 GiB calibration copy; no JSON line")
     ap.add_argument("--tuning", default=None,
-                    help="profile run: skin,lanes,list_waves[,bonded_mode] to use instead of the "
-                         "autotuner")
+                    help="profile run: skin,lanes,list_waves[,bonded_mode[,steps_per_graph]] to use "
+                         "instead of the autotuner")
     ap.add_argument("--workers", type=int, default=0,
                     help="input-generation processes (0 = one per core up to 16; use 1 under a "
                          "profiler, which initialises the GPU before the program starts)")
@@ -526,6 +528,7 @@ def main():
                 e1.set_skin(tune_b[0])
                 e1.set_tuning(tune_b[1], 256, tune_b[3])
                 e1.set_bonded_mode(tune_b[4])
+                e1.set_graph_steps(tune_b[5])
             e1.resume([snap[1][k]])
             t_rep += time_engine(e1, kp, wp)
             as_rep += spec.system.np * kp
@@ -546,7 +549,7 @@ def main():
         SV.relax(pool, args.minimize)
         reqs = SV.poisson_stream(pool, args.serve_rate, args.serve_seconds)
         nreq = len(reqs)
-        tuning = None if tune_b is None else (tune_b[0], tune_b[1], tune_b[3], tune_b[4])
+        tuning = None if tune_b is None else (tune_b[0], tune_b[1], tune_b[3], tune_b[4], tune_b[5])
         srv = SV.SegmentServer(segment=100, tuning=tuning)
         res, wall, atom_steps = srv.serve(reqs)
         lat = np.array([r.latency_ms for r in res])
@@ -600,7 +603,8 @@ def main():
             "temperature_K_mean": round(float(temps_b.mean()), 1),
             "phase_ms_eager": {k: round(v, 4) for k, v in phases_b.items()},
             "tuning": None if tune_b is None else {"skin": tune_b[0], "lanes": tune_b[1],
-                                                   "list_waves": tune_b[3], "bonded_mode": tune_b[4]},
+                                                   "list_waves": tune_b[3], "bonded_mode": tune_b[4],
+                                                   "steps_per_graph": tune_b[5]},
             "preparation": f"minimize {args.minimize}, Langevin gamma 20/ps for {args.equil} steps "
                            "(untimed)",
             "plain": {"workload": f"the same {len(reps)} replicas, one system per launch (one engine "
@@ -614,7 +618,8 @@ def main():
                       "phase_ms_eager": {k: round(v, 4) for k, v in phases_p.items()},
                       "tuning": None if tune_p is None else {"skin": tune_p[0], "lanes": tune_p[1],
                                                              "list_waves": tune_p[3],
-                                                             "bonded_mode": tune_p[4]}},
+                                                             "bonded_mode": tune_p[4],
+                                                             "steps_per_graph": tune_p[5]}},
             "served": {"workload": f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g}s, "
                                    "500-step requests, 3,000-150,000 atoms",
                        "requests": nreq, "p50_ms": p50, "p99_ms": p99,

@@ -239,6 +239,19 @@ int sofia_md_run(sofia_ctx *ctx, int32_t nsteps, int32_t use_graph);
  * overflow or SHAKE failure.  Several contexts launched back to back run concurrently. */
 int sofia_md_launch(sofia_ctx *ctx, int32_t nsteps, int32_t use_graph);
 int sofia_md_finish(sofia_ctx *ctx);
+/* Grow-and-redo (the engine's form of neighbor_wrapper's grow-and-rebuild,
+ * Source/mod_neighbor.f90:258-298): md_launch keeps the segment's starting state on the
+ * device; when md_finish finds that a list rebuild inside the segment overflowed the
+ * stride, it restores that state, rebuilds every list with a stride 1.25x the overflowing
+ * count and runs the segment again, with the same result as a run that never overflowed.
+ * sofia_md_list_stats reports the current stride and how many segments were redone. */
+int sofia_md_list_stats(sofia_ctx *ctx, int32_t *neigh_alloc, int32_t *redos);
+/* The engine's list stride is headroom x the largest neighbour count when lists are set up
+ * (default 1.5; 1.0 leaves no room, which makes overflow and redo likely). */
+int sofia_set_list_headroom(sofia_ctx *ctx, double headroom);
+/* MD steps captured per hipGraph (1..200, default 1): md_launch replays a k-step graph and
+ * finishes with one-step graphs; k changes launch overhead only, never the trajectory. */
+int sofia_set_graph_steps(sofia_ctx *ctx, int32_t steps_per_graph);
 /* Continue from FP64 state uploaded with sofia_put_state_f64 (and the replica step
  * counters with sofia_set_replica_steps), e.g. a restart: rebuilds the lists and
  * re-evaluates the forces at the state, with the Langevin draw of the step that produced

@@ -96,6 +96,9 @@ SIGNATURES = {
     "sofia_md_run": [_vp, _i32, _i32],
     "sofia_md_launch": [_vp, _i32, _i32],
     "sofia_md_finish": [_vp],
+    "sofia_md_list_stats": [_vp, _vp, _vp],
+    "sofia_set_list_headroom": [_vp, _d],
+    "sofia_set_graph_steps": [_vp, _i32],
     "sofia_md_resume": [_vp],
     "sofia_get_forces_f64": [_vp, _vp],
     "sofia_md_pair_forces": [_vp, _vp, _vp],

@@ -59,7 +59,9 @@ void dfree(T *&p) {
 // of them drops the graph, and the next graph launch captures it again.
 void drop_graph(sofia_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graphk) (void)hipGraphExecDestroy(c->graphk);
     c->graph = nullptr;
+    c->graphk = nullptr;
 }
 
 int alloc_atoms(sofia::AtomArrays &A, int n) {
@@ -88,6 +90,26 @@ int alloc_atoms(sofia::AtomArrays &A, int n) {
 void free_atoms(sofia::AtomArrays &A) {
     dfree(A.pos4); dfree(A.q); dfree(A.gid); dfree(A.num1); dfree(A.num2); dfree(A.num3);
     dfree(A.spec); dfree(A.v32); dfree(A.xd); dfree(A.vd); dfree(A.mol); dfree(A.wclus);
+}
+
+// Every per-atom array of one set into another, ordered on stream s.
+int copy_atoms(sofia::AtomArrays &dst, const sofia::AtomArrays &src, size_t n, hipStream_t s) {
+    auto cp = [&](void *d, const void *p, size_t bytes) {
+        return hipMemcpyAsync(d, p, bytes, hipMemcpyDeviceToDevice, s);
+    };
+    SOFIA_HIP(cp(dst.pos4, src.pos4, sizeof(float4) * n));
+    SOFIA_HIP(cp(dst.q, src.q, sizeof(float) * n));
+    SOFIA_HIP(cp(dst.gid, src.gid, sizeof(int32_t) * n));
+    SOFIA_HIP(cp(dst.num1, src.num1, sizeof(int32_t) * n));
+    SOFIA_HIP(cp(dst.num2, src.num2, sizeof(int32_t) * n));
+    SOFIA_HIP(cp(dst.num3, src.num3, sizeof(int32_t) * n));
+    SOFIA_HIP(cp(dst.spec, src.spec, sizeof(int32_t) * SPEC * n));
+    SOFIA_HIP(cp(dst.v32, src.v32, sizeof(float) * 3 * n));
+    SOFIA_HIP(cp(dst.xd, src.xd, sizeof(double) * 3 * n));
+    SOFIA_HIP(cp(dst.vd, src.vd, sizeof(double) * 3 * n));
+    SOFIA_HIP(cp(dst.mol, src.mol, sizeof(int32_t) * n));
+    SOFIA_HIP(cp(dst.wclus, src.wclus, sizeof(int32_t) * n));
+    return SOFIA_OK;
 }
 
 // check_allocation's grid, Source/mod_neighbor.f90:1446-1451 (real*4), vol = box**3
@@ -130,6 +152,14 @@ int grow_alloc(int mx) {   // Source/mod_neighbor.f90:260-263
     int na = (int)((float)mx * 1.2f);
     while (na % 16) ++na;
     return na;
+}
+
+// The engine's list stride for a largest count mx: headroom x mx rounded up to 16 entries
+// (exactly mx at headroom 1, which leaves no room at all), never below the floor a redo set.
+int engine_stride(const sofia_ctx *c, int mx) {
+    int na = mx;
+    if (c->headroom > 1.0) na = (((int)std::ceil((double)mx * c->headroom)) + 15) / 16 * 16;
+    return std::max({64, na, c->stride_floor});
 }
 
 int ensure_list(sofia_ctx *c, int na) {
@@ -477,7 +507,7 @@ int rebuild_now(sofia_ctx *c, bool engine, bool reference_growth) {
             c->have_list = true;
             return sync_check(c);
         }
-        int na = reference_growth ? grow_alloc(mx) : ((mx * 3 / 2 + 15) / 16) * 16;
+        int na = reference_growth ? grow_alloc(mx) : engine_stride(c, mx);
         TRY(ensure_list(c, na));
     }
     return fail(SOFIA_E_OVERFLOW, "Verlet list kept overflowing");
@@ -686,6 +716,9 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     drop_graph(c);
     free_atoms(c->a);
     free_atoms(c->b);
+    free_atoms(c->snap);
+    dfree(c->snap_topo); dfree(c->snap_fnb); dfree(c->snap_drd); dfree(c->snap_fix);
+    dfree(c->snap_step); dfree(c->snap_nbuilds);
     dfree(c->rep); dfree(c->dr32); dfree(c->drd); dfree(c->fnb); dfree(c->fix); dfree(c->fscratch); dfree(c->stage32);
     dfree(c->mol_rep); dfree(c->uf_parent); dfree(c->uf_size); dfree(c->uf_largest);
     dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep); dfree(c->acell);
@@ -1388,11 +1421,12 @@ int collect_energies(sofia_ctx *c) {
     return SOFIA_OK;
 }
 
-int check_engine_flags(sofia_ctx *c) {
+int check_engine_flags(sofia_ctx *c, int32_t *overflow = nullptr) {
     int32_t err = 0, mx = 0;
     SOFIA_HIP(hipMemcpyAsync(&err, c->shake_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     SOFIA_HIP(hipMemcpyAsync(&mx, c->maxcnt, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     TRY(sync_check(c));
+    if (overflow) *overflow = mx > c->neigh_alloc ? mx : 0;
     if (mx > c->neigh_alloc) return fail(SOFIA_E_OVERFLOW, "Verlet list overflow during an MD segment");
     if (err)
         return fail(SOFIA_E_SHAKE, "constraint solve failed during an MD segment (SHAKE "
@@ -1610,7 +1644,7 @@ int engine_lists(sofia_ctx *c) {
     TRY(download(nn.data(), c->numneigh, sizeof(int32_t) * c->N));
     int32_t mx = 0;
     for (int v : nn) mx = std::max(mx, v);
-    const int want = std::max(64, ((mx * 3 / 2 + 15) / 16) * 16);
+    const int want = engine_stride(c, mx);
     if (want > c->neigh_alloc) {
         TRY(ensure_list(c, want));
         launch_set_rebuild(c, 1);
@@ -1790,31 +1824,129 @@ int sofia_md_resume_forces(sofia_ctx *c, const double *f, const int32_t *have) {
     return put_forces(c, f, slot, have);
 }
 
+namespace {
+
+// The bonded and constraint lists that the cell sort re-indexes in place.
+std::vector<std::pair<int32_t *, size_t>> sorted_lists(sofia_ctx *c) {
+    return {{c->bondlist, (size_t)3 * c->nb},   {c->anglelist, (size_t)4 * c->na},
+            {c->dihedlist, (size_t)5 * c->nd},  {c->torsions, (size_t)5 * c->ntor},
+            {c->improlist, (size_t)5 * c->nimp}, {c->shake_atoms, (size_t)4 * c->ns},
+            {c->ebond, (size_t)3 * c->nbe},      {c->eangle, (size_t)4 * c->nae}};
+}
+
+// The state an MD segment starts from (every per-atom array the sort permutes, the lists it
+// re-indexes, the carried forces, displacements, step counters and build counts), copied
+// on the context stream ahead of the segment's kernels.
+int snapshot_take(sofia_ctx *c) {
+    const size_t n = (size_t)c->N;
+    if (!c->snap.pos4) {
+        TRY(alloc_atoms(c->snap, c->N));
+        TRY(dalloc(&c->snap_fnb, 3 * n));
+        TRY(dalloc(&c->snap_drd, 3 * n));
+        TRY(dalloc(&c->snap_fix, 3 * n));
+        TRY(dalloc(&c->snap_step, c->nrep));
+        TRY(dalloc(&c->snap_nbuilds, c->nrep));
+    }
+    size_t topo = 0;
+    for (auto &l : sorted_lists(c)) topo += l.first ? l.second : 0;
+    if (topo > c->snap_topo_cap) {
+        TRY(sync_check(c));
+        dfree(c->snap_topo);
+        TRY(dalloc(&c->snap_topo, topo));
+        c->snap_topo_cap = topo;
+    }
+    hipStream_t s = c->stream;
+    TRY(copy_atoms(c->snap, c->a, n, s));
+    size_t off = 0;
+    for (auto &l : sorted_lists(c)) {
+        if (!l.first || !l.second) continue;
+        SOFIA_HIP(hipMemcpyAsync(c->snap_topo + off, l.first, sizeof(int32_t) * l.second,
+                                 hipMemcpyDeviceToDevice, s));
+        off += l.second;
+    }
+    SOFIA_HIP(hipMemcpyAsync(c->snap_fnb, c->fnb, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s));
+    SOFIA_HIP(hipMemcpyAsync(c->snap_drd, c->drd, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s));
+    SOFIA_HIP(hipMemcpyAsync(c->snap_fix, c->fix, sizeof(long long) * 3 * n, hipMemcpyDeviceToDevice, s));
+    SOFIA_HIP(hipMemcpyAsync(c->snap_step, c->step_d, sizeof(uint64_t) * c->nrep, hipMemcpyDeviceToDevice, s));
+    SOFIA_HIP(hipMemcpyAsync(c->snap_nbuilds, c->nbuilds, sizeof(int32_t) * c->nrep,
+                             hipMemcpyDeviceToDevice, s));
+    c->snap_valid = true;
+    return SOFIA_OK;
+}
+
+// Back to the segment's starting state, then lists rebuilt for every replica with a stride
+// that holds `overflow` entries with room to spare, the carried forces following the atoms
+// through the re-sort.  The engine's forces depend on positions only (complete lists, exact
+// sums), so the segment run again from here is the one that would have run without the
+// overflow, bit for bit.
+int snapshot_restore(sofia_ctx *c, int32_t overflow) {
+    const size_t n = (size_t)c->N;
+    hipStream_t s = c->stream;
+    TRY(sync_check(c));
+    TRY(copy_atoms(c->a, c->snap, n, s));
+    size_t off = 0;
+    for (auto &l : sorted_lists(c)) {
+        if (!l.first || !l.second) continue;
+        SOFIA_HIP(hipMemcpyAsync(l.first, c->snap_topo + off, sizeof(int32_t) * l.second,
+                                 hipMemcpyDeviceToDevice, s));
+        off += l.second;
+    }
+    SOFIA_HIP(hipMemcpyAsync(c->fnb, c->snap_fnb, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s));
+    SOFIA_HIP(hipMemcpyAsync(c->drd, c->snap_drd, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s));
+    SOFIA_HIP(hipMemcpyAsync(c->fix, c->snap_fix, sizeof(long long) * 3 * n, hipMemcpyDeviceToDevice, s));
+    SOFIA_HIP(hipMemcpyAsync(c->step_d, c->snap_step, sizeof(uint64_t) * c->nrep, hipMemcpyDeviceToDevice, s));
+    TRY(sync_check(c));
+    std::vector<double> f(3 * n);
+    std::vector<int32_t> slot;
+    TRY(sofia_get_forces_f64(c, f.data()));
+    TRY(gid_slots(c, slot));
+    c->stride_floor = std::max(c->stride_floor, ((overflow * 5 / 4 + 15) / 16) * 16);
+    TRY(engine_lists(c));
+    TRY(put_forces(c, f.data(), slot, nullptr));
+    SOFIA_HIP(hipMemcpyAsync(c->nbuilds, c->snap_nbuilds, sizeof(int32_t) * c->nrep,
+                             hipMemcpyDeviceToDevice, s));
+    return sync_check(c);
+}
+
+int capture_steps(sofia_ctx *c, hipGraphExec_t *out, int k) {
+    hipGraph_t g = nullptr;
+    SOFIA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < k; ++s) step_sequence(c, false);
+    SOFIA_HIP(hipStreamEndCapture(c->stream, &g));
+    // the executable graph is independent of the captured one, which goes at once
+    const hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        *out = nullptr;
+        return check_hip(ie, "hipGraphInstantiate");
+    }
+    return SOFIA_OK;
+}
+
+}  // namespace
+
 int sofia_md_launch(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
     if (nsteps <= 0) return SOFIA_OK;
-    if (use_graph && !c->graph) {
-        hipGraph_t g = nullptr;
-        SOFIA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        step_sequence(c, false);
-        SOFIA_HIP(hipStreamEndCapture(c->stream, &g));
-        // the executable graph is independent of the captured one, which goes at once
-        const hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) {
-            c->graph = nullptr;
-            return check_hip(ie, "hipGraphInstantiate");
-        }
+    if (c->redo) {
+        TRY(snapshot_take(c));
+        c->snap_nsteps = nsteps;
+        c->snap_graph = use_graph;
     }
-    for (int s = 0; s < nsteps - 1; ++s) {
-        if (use_graph) {
-            SOFIA_HIP(hipGraphLaunch(c->graph, c->stream));
-        } else {
-            step_sequence(c, false);
-        }
+    const int body = nsteps - 1;   // the last step also reduces energies
+    if (use_graph) {
+        const int k = c->graph_steps;
+        if (!c->graph) TRY(capture_steps(c, &c->graph, 1));
+        if (k > 1 && body >= k && !c->graphk) TRY(capture_steps(c, &c->graphk, k));
+        int s = 0;
+        if (k > 1 && c->graphk)
+            for (; s + k <= body; s += k) SOFIA_HIP(hipGraphLaunch(c->graphk, c->stream));
+        for (; s < body; ++s) SOFIA_HIP(hipGraphLaunch(c->graph, c->stream));
+    } else {
+        for (int s = 0; s < body; ++s) step_sequence(c, false);
     }
-    step_sequence(c, true);   // last step also reduces energies
+    step_sequence(c, true);
     c->step_h += nsteps;
     return SOFIA_OK;
 }
@@ -1822,8 +1954,47 @@ int sofia_md_launch(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {
 int sofia_md_finish(sofia_ctx *c) {
     if (!c) return fail(SOFIA_E_ARG, "null context");
     if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
-    TRY(collect_energies(c));
-    return check_engine_flags(c);
+    for (int attempt = 0;; ++attempt) {
+        int32_t overflow = 0;
+        const int rc = check_engine_flags(c, &overflow);
+        if (rc == SOFIA_E_OVERFLOW && c->snap_valid && attempt < 4) {
+            // grow and redo the segment (the reference grows neigh_alloc and rebuilds)
+            const int32_t nsteps = c->snap_nsteps, graph = c->snap_graph;
+            TRY(snapshot_restore(c, overflow));
+            c->step_h -= nsteps;
+            ++c->redos;
+            TRY(sofia_md_launch(c, nsteps, graph));
+            continue;
+        }
+        c->snap_valid = false;
+        if (rc != SOFIA_OK) return rc;
+        return collect_energies(c);
+    }
+}
+
+int sofia_set_graph_steps(sofia_ctx *c, int32_t k) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (k < 1 || k > 200) return fail(SOFIA_E_ARG, "steps per graph must be 1..200");
+    if (k != c->graph_steps) {
+        TRY(sync_check(c));
+        drop_graph(c);
+        c->graph_steps = k;
+    }
+    return SOFIA_OK;
+}
+
+int sofia_set_list_headroom(sofia_ctx *c, double headroom) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!(headroom >= 1.0) || headroom > 4.0) return fail(SOFIA_E_ARG, "list headroom must be in [1, 4]");
+    c->headroom = headroom;
+    return SOFIA_OK;
+}
+
+int sofia_md_list_stats(sofia_ctx *c, int32_t *neigh_alloc, int32_t *redos) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (neigh_alloc) *neigh_alloc = c->neigh_alloc;
+    if (redos) *redos = c->redos;
+    return SOFIA_OK;
 }
 
 int sofia_md_run(sofia_ctx *c, int32_t nsteps, int32_t use_graph) {

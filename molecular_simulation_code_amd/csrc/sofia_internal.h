@@ -181,7 +181,27 @@ struct sofia_ctx {
     uint64_t *step_d = nullptr;  // per-replica step counters (Langevin key)
     uint64_t step_h = 0;
     int lanes = 16, block = 256, list_waves = 4;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;    // one MD step
+    hipGraphExec_t graphk = nullptr;   // graph_steps MD steps
+    int32_t graph_steps = 1;
+
+    // Grow-and-redo of an MD segment whose Verlet list overflowed (the engine's form of the
+    // reference's grow-and-rebuild, Source/mod_neighbor.f90:258-298): the state at the start
+    // of each segment is kept on the device, and a segment that overflowed is run again
+    // from it with a larger list stride.
+    bool redo = true;
+    sofia::AtomArrays snap{};
+    int32_t *snap_topo = nullptr;
+    size_t snap_topo_cap = 0;
+    double *snap_fnb = nullptr, *snap_drd = nullptr;
+    long long *snap_fix = nullptr;
+    uint64_t *snap_step = nullptr;
+    int32_t *snap_nbuilds = nullptr;
+    bool snap_valid = false;
+    int32_t snap_nsteps = 0, snap_graph = 0;
+    int32_t redos = 0;
+    int32_t stride_floor = 0;          // smallest stride the next engine list set-up uses
+    double headroom = 1.5;             // engine list stride: headroom x the largest count
     bool has_thermostat = false;
 
     // host staging (pinned) for the asynchronous pair-kernel drop-in

@@ -404,6 +404,46 @@ class SofiaGPU:
         return out
 
 
+_TUNE_CACHE: dict = {}
+
+
+def _tune_cache_file():
+    import os
+    return os.environ.get("SOFIA_TUNE_CACHE")
+
+
+def _tune_cache_get(key: str):
+    if key in _TUNE_CACHE:
+        return _TUNE_CACHE[key]
+    path = _tune_cache_file()
+    if path:
+        import json
+        try:
+            with open(path) as f:
+                v = json.load(f).get(key)
+        except (OSError, ValueError):
+            v = None
+        if v is not None:
+            _TUNE_CACHE[key] = tuple(v)
+            return _TUNE_CACHE[key]
+    return None
+
+
+def _tune_cache_put(key: str, value: tuple):
+    _TUNE_CACHE[key] = value
+    path = _tune_cache_file()
+    if path:
+        import json
+        try:
+            with open(path) as f:
+                data = json.load(f)
+        except (OSError, ValueError):
+            data = {}
+        data[key] = list(value)
+        with open(path, "w") as f:
+            json.dump(data, f, indent=1)
+
+
 @dataclass
 class ReplicaSpec:
     system: S.SofiaSystem
@@ -486,14 +526,37 @@ class Engine:
     def set_tuning(self, lanes: int, block: int, list_waves: int):
         check(lib().sofia_set_tuning(self.ctx, lanes, block, list_waves))
 
+    def tune_key(self) -> str:
+        """The autotune cache bucket of this engine: precision, electrostatics, replica
+        count and total atoms (both on a half-octave scale) and the mean number density
+        (to 0.002 atoms/A^3)."""
+        vol = float(sum(float(r.system.box) ** 3 for r in self.reps))
+        rho = self.N / vol
+        return (f"p{self.params.precision}c{self.params.coul_flag}"
+                f"r{round(2 * np.log2(len(self.reps)))}n{round(2 * np.log2(self.N))}"
+                f"d{round(rho / 0.002)}")
+
     def autotune(self, steps: int = 80, skins=(1.0, 1.3, 1.6, 2.0, 2.5), lanes=(0, 8, 16, 32),
-                 waves=(4, 8, 16)):
+                 waves=(4, 8, 16), graph_steps=(1, 20), cache: bool = True):
         """Pick the Verlet skin and the pair kernel's lanes per atom on the current state.
         For each skin the lanes are chosen by timing the pair kernel alone; skins are
         compared on whole MD steps (list length against rebuild rate); then the list build's
-        waves per workgroup on full rebuilds, and the bonded evaluation (per-kind kernels
-        or molecule bundles) on eager steps.  The knobs change speed only.  Returns
-        (skin, lanes, ms/step, list waves, bonded mode)."""
+        waves per workgroup on full rebuilds, the bonded evaluation (per-kind kernels or
+        molecule bundles) on eager steps, and the MD steps per captured graph on whole
+        steps.  The knobs change speed only.  Results are cached per bucket (tune_key: size
+        and density on a coarse scale) for the process and, when SOFIA_TUNE_CACHE names a
+        JSON file, across processes; a cached bucket is applied without timing.  Returns
+        (skin, lanes, ms/step, list waves, bonded mode, steps per graph)."""
+        key = self.tune_key()
+        hit = _tune_cache_get(key) if cache else None
+        if hit is not None:
+            sk, L, ms, w, bmode, gk = hit
+            self.set_skin(sk)
+            self.set_tuning(L, 256, w)
+            self.set_bonded_mode(bmode)
+            self.set_graph_steps(gk)
+            self.tune_log = {"cached": key}
+            return sk, L, ms, w, bmode, gk
         import time
         best = None
         self.tune_log = {"pair_ms": {}, "step_ms": {}, "list_ms": {}}
@@ -531,7 +594,36 @@ class Engine:
         self.tune_log["bonded_ms"] = t_b
         bmode = min(t_b, key=t_b.get)
         self.set_bonded_mode(bmode)
-        return best[0], best[1], best[2], w, bmode
+        t_g = {}
+        for gk in graph_steps:
+            self.set_graph_steps(gk)
+            self.run(max(5, gk + 1))
+            self.synchronize()
+            t0 = time.perf_counter()
+            self.run(steps)
+            self.synchronize()
+            t_g[gk] = 1e3 * (time.perf_counter() - t0) / steps
+        self.tune_log["graph_ms"] = t_g
+        gk = min(t_g, key=t_g.get)
+        self.set_graph_steps(gk)
+        out = (best[0], best[1], min(best[2], t_g[gk]), w, bmode, gk)
+        if cache:
+            _tune_cache_put(key, out)
+        return out
+
+    def set_graph_steps(self, k: int):
+        """MD steps captured per hipGraph (launch overhead only)."""
+        check(lib().sofia_set_graph_steps(self.ctx, int(k)))
+
+    def set_list_headroom(self, headroom: float):
+        """List stride = headroom x the largest neighbour count at list set-up."""
+        check(lib().sofia_set_list_headroom(self.ctx, float(headroom)))
+
+    def list_stats(self):
+        """(list stride, segments redone after an in-segment list overflow)."""
+        na, redos = C.c_int32(), C.c_int32()
+        check(lib().sofia_md_list_stats(self.ctx, C.byref(na), C.byref(redos)))
+        return na.value, redos.value
 
     def set_bonded_mode(self, mode: int):
         """Bonded items by kind (0, one thread per item) or by molecule bundles (1); changes

@@ -109,6 +109,8 @@ class SegmentServer:
             eng.set_tuning(lanes, 256, waves)
             if len(self.tuning) > 3:
                 eng.set_bonded_mode(self.tuning[3])
+            if len(self.tuning) > 4:
+                eng.set_graph_steps(self.tuning[4])
         eng.put_state([(s.x, s.v) for s in slots])
         eng.set_steps([s.step for s in slots])
         eng.resume([s.f for s in slots])

@@ -751,3 +751,64 @@ def test_forcefield_change_mid_run_graph_equals_eager():
     assert np.array_equal(outs["graph"][0], outs["eager"][0])
     assert np.array_equal(outs["graph"][1], outs["eager"][1])
     assert not np.array_equal(outs["graph"][0], outs["unchanged"][0])
+
+
+def test_list_overflow_mid_segment_is_grown_and_redone():
+    """With no headroom in the list stride (headroom 1.0: the stride is the largest count at
+    set-up), a rebuild inside the segment overflows; the segment is run again from its
+    starting state with a grown stride and ends bit-identical to a run with ample room,
+    in one segment and across several, with multi-step graphs."""
+    host = _engine()
+    # hot replicas (600-650 K) rebuild often and their largest neighbour count moves
+    s1, s2 = _sys(20, 1400, 81, temp=600.0), _sys(30, 1700, 82, temp=650.0)
+    specs = [host.ReplicaSpec(s1, temp=600.0, gamma_ps=1.0, seed=21),
+             host.ReplicaSpec(s2, temp=650.0, gamma_ps=1.0, seed=22)]
+    outs = {}
+    for tag, headroom, segs, gsteps in (("ample", 1.5, [200], 1), ("tight", 1.0, [200], 1),
+                                        ("tight-segments", 1.0, [70, 70, 60], 8)):
+        e = host.Engine(specs, precision=host.PREC_MIXED)
+        try:
+            e.set_list_headroom(headroom)
+            e.set_graph_steps(gsteps)
+            e.init()
+            na0, _ = e.list_stats()
+            for n in segs:
+                e.run(n)
+            na, redos = e.list_stats()
+            say(f"[list redo] {tag}: stride {na0} -> {na}, {redos} segments redone, "
+                f"rebuilds {e.rebuilds().tolist()}")
+            outs[tag] = (e.state(), e.forces(), e.energies(), redos, na0, na)
+        finally:
+            e.close()
+    assert outs["ample"][3] == 0
+    assert outs["tight"][3] >= 1 and outs["tight"][5] > outs["tight"][4]
+    assert outs["tight-segments"][3] >= 1
+    for tag in ("tight", "tight-segments"):
+        for k in range(len(specs)):
+            assert np.array_equal(outs[tag][0][k][0], outs["ample"][0][k][0]), f"{tag} {k} x"
+            assert np.array_equal(outs[tag][0][k][1], outs["ample"][0][k][1]), f"{tag} {k} v"
+            assert np.array_equal(outs[tag][1][k], outs["ample"][1][k]), f"{tag} {k} F"
+        np.testing.assert_allclose(outs[tag][2], outs["ample"][2], rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("gsteps", [10, 50])
+def test_multi_step_graphs_equal_one_step_graphs(gsteps):
+    """Replaying k-step graphs (and one-step graphs for the rest of a segment) gives the
+    one-step-graph trajectory bit for bit."""
+    host = _engine()
+    s = _sys(60, 1500, 83)
+    outs = []
+    for k in (1, gsteps):
+        e = host.Engine([host.ReplicaSpec(s, temp=300.0, gamma_ps=1.0, seed=31)],
+                        precision=host.PREC_MIXED)
+        try:
+            e.set_graph_steps(k)
+            e.init()
+            e.run(73)
+            e.run(gsteps + 5)
+            outs.append((e.state()[0], e.forces()[0], e.rebuilds()[0]))
+        finally:
+            e.close()
+    assert np.array_equal(outs[0][0][0], outs[1][0][0]) and np.array_equal(outs[0][0][1], outs[1][0][1])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2]
