@@ -336,6 +336,16 @@ int sofia_set_bonded_mode(sofia_ctx *ctx, int32_t mode);
 int sofia_set_molecules(sofia_ctx *ctx, const int32_t *mol);
 int sofia_cluster_largest(sofia_ctx *ctx, float rcontact, int32_t n_o, const int32_t *o_types,
                           int32_t n_n, const int32_t *n_types, int32_t *largest);
+/* get_greatestCluster (Source/mod_cluster.f90:15-83) on the engine state, per replica, over
+ * the solute molecules of sofia_set_molecules: molecule centres (the mean of the molecule's
+ * atoms made whole, wrapped into the box; xyzcenter_molecule / pbc_molcom), denstrack = other
+ * centres closer than denscut (mol_build_neighbor_nosort, Source/mod_neighbor_cluster.f90:
+ * 56-178), dense molecules (denstrack >= densmin) joined into clusters through centre pairs
+ * closer than denscut (buildClusters / harvestCluster), the first largest cluster and its
+ * non-dense neighbours (append_surface_molecules).  total[r] = core[r] + surface[r], the
+ * reference's return value; core and surface may be NULL. */
+int sofia_cluster_greatest(sofia_ctx *ctx, double denscut, int32_t densmin, int32_t *total,
+                           int32_t *core, int32_t *surface);
 
 /* ---------------------------------------------------------------- files (host only) */
 /* Restart file as dump_global_velocity_id_mpi writes it (Source/mod_datadump.f90:113-151):

@@ -21,8 +21,9 @@ COMMON = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-Wall",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(HERE, "..", "include")]
 # The list build must evaluate the real*4 distance without FMA contraction to match the
-# reference bit for bit; the force kernels may contract.
-PER_FILE = {"k_neighbor.hip": ["-ffp-contract=off"]}
+# reference bit for bit, and the cluster analysis its centres and distances as the host's
+# restatement does (exact integer results); the force kernels may contract.
+PER_FILE = {"k_neighbor.hip": ["-ffp-contract=off"], "k_cluster.hip": ["-ffp-contract=off"]}
 SOURCES = ["sofia_api.hip", "k_neighbor.hip", "k_list_engine.hip", "k_nonbond.hip",
            "k_dsf_mixed.hip", "k_bonded.hip", "k_integrate.hip", "k_cluster.hip", "sofia_io.cpp"]
 

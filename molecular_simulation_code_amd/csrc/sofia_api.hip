@@ -721,6 +721,8 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->snap_step); dfree(c->snap_nbuilds);
     dfree(c->rep); dfree(c->dr32); dfree(c->drd); dfree(c->fnb); dfree(c->fix); dfree(c->fscratch); dfree(c->stage32);
     dfree(c->mol_rep); dfree(c->uf_parent); dfree(c->uf_size); dfree(c->uf_largest);
+    dfree(c->gc_first); dfree(c->gc_refgid); dfree(c->gc_cnt); dfree(c->gc_dens); dfree(c->gc_sum);
+    dfree(c->gc_refx); dfree(c->gc_com); dfree(c->gc_key); dfree(c->gc_surface);
     dfree(c->ccount); dfree(c->cstart); dfree(c->cnum); dfree(c->cell_rep); dfree(c->acell);
     dfree(c->aslot); dfree(c->perm); dfree(c->tt); dfree(c->atombin); dfree(c->sorted_ids);
     dfree(c->nlist); dfree(c->numneigh); dfree(c->maxcnt); dfree(c->rebuild); dfree(c->nbuilds);
@@ -2216,12 +2218,29 @@ int sofia_set_molecules(sofia_ctx *c, const int32_t *mol) {
         for (int k = 0; k <= mx; ++k) mrep.push_back(r);
         base += mx + 1;
     }
+    TRY(sync_check(c));
     dfree(c->mol_rep); dfree(c->uf_parent); dfree(c->uf_size); dfree(c->uf_largest);
+    dfree(c->gc_first); dfree(c->gc_refgid); dfree(c->gc_cnt); dfree(c->gc_dens); dfree(c->gc_sum);
+    dfree(c->gc_refx); dfree(c->gc_com); dfree(c->gc_key); dfree(c->gc_surface);
     c->nmol = base;
-    TRY(dalloc(&c->mol_rep, std::max(1, base)));
-    TRY(dalloc(&c->uf_parent, std::max(1, base)));
-    TRY(dalloc(&c->uf_size, std::max(1, base)));
+    const size_t nm = (size_t)std::max(1, base);
+    TRY(dalloc(&c->mol_rep, nm));
+    TRY(dalloc(&c->uf_parent, nm));
+    TRY(dalloc(&c->uf_size, nm));
     TRY(dalloc(&c->uf_largest, c->nrep + 1));   // per-replica maxima, then an error flag
+    TRY(dalloc(&c->gc_first, c->nrep + 1));
+    TRY(dalloc(&c->gc_refgid, nm));
+    TRY(dalloc(&c->gc_cnt, nm));
+    TRY(dalloc(&c->gc_dens, nm));
+    TRY(dalloc(&c->gc_sum, 3 * nm));
+    TRY(dalloc(&c->gc_refx, 3 * nm));
+    TRY(dalloc(&c->gc_com, 3 * nm));
+    TRY(dalloc(&c->gc_key, c->nrep));
+    TRY(dalloc(&c->gc_surface, c->nrep));
+    std::vector<int32_t> first(c->nrep + 1, base);
+    for (int m = base - 1; m >= 0; --m) first[mrep[m]] = m;
+    for (int r = c->nrep - 1; r >= 0; --r) first[r] = std::min(first[r], first[r + 1]);
+    TRY(upload(c->gc_first, first.data(), sizeof(int32_t) * first.size()));
     if (base) TRY(upload(c->mol_rep, mrep.data(), sizeof(int32_t) * base));
     return upload(c->a.mol, g.data(), sizeof(int32_t) * c->N);
 }
@@ -2233,6 +2252,14 @@ int sofia_cluster_largest(sofia_ctx *c, float rcontact, int32_t n_o, const int32
     if (!c->uf_parent) return fail(SOFIA_E_STATE, "call sofia_set_molecules first");
     std::vector<int> ot(o_types, o_types + n_o), nt(n_types, n_types + n_n);
     return cluster_largest(c, rcontact, ot, nt, largest);
+}
+
+int sofia_cluster_greatest(sofia_ctx *c, double denscut, int32_t densmin, int32_t *total,
+                           int32_t *core, int32_t *surface) {
+    if (!c || !(denscut > 0.0) || densmin < 0) return fail(SOFIA_E_ARG, "bad argument");
+    if (!c->engine_ready) return fail(SOFIA_E_STATE, "call sofia_md_init first");
+    if (!c->gc_first) return fail(SOFIA_E_STATE, "call sofia_set_molecules first");
+    return cluster_greatest(c, denscut, densmin, total, core, surface);
 }
 
 }  // extern "C"

@@ -106,6 +106,13 @@ struct sofia_ctx {
     // cluster analysis: global solute-molecule index per atom lives in AtomArrays::mol
     int32_t nmol = 0;
     int32_t *mol_rep = nullptr, *uf_parent = nullptr, *uf_size = nullptr, *uf_largest = nullptr;
+    // get_greatestCluster: first molecule of each replica (nrep + 1), per-molecule centres,
+    // their construction, the density counts, and per-replica results
+    int32_t *gc_first = nullptr, *gc_refgid = nullptr, *gc_cnt = nullptr, *gc_dens = nullptr;
+    long long *gc_sum = nullptr;
+    double *gc_refx = nullptr, *gc_com = nullptr;
+    unsigned long long *gc_key = nullptr;
+    int32_t *gc_surface = nullptr;
 
     // cells
     int32_t *ccount = nullptr, *cstart = nullptr, *cnum = nullptr, *cell_rep = nullptr;
@@ -265,6 +272,8 @@ void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp);
 // cluster analysis (k_cluster.hip)
 int cluster_largest(sofia_ctx *c, float rcontact, const std::vector<int> &o_types,
                     const std::vector<int> &n_types, int32_t *largest);
+int cluster_greatest(sofia_ctx *c, double denscut, int32_t densmin, int32_t *total, int32_t *core,
+                     int32_t *surface);
 
 // error helpers (sofia_api.cpp)
 int fail(int code, const std::string &msg);

@@ -808,6 +808,15 @@ class Engine:
         check(lib().sofia_profile_step(self.ctx, steps, 1 if full_rebuild else 0, ptr(out)))
         return dict(zip(self.PHASES, out.tolist()))
 
+    def greatest_cluster(self, denscut: float, densmin: int):
+        """get_greatestCluster per replica over the solute molecules: (total, core,
+        surface) arrays (Source/mod_cluster.f90:15-83)."""
+        n = len(self.reps)
+        tot, core, surf = np.empty(n, i32), np.empty(n, i32), np.empty(n, i32)
+        check(lib().sofia_cluster_greatest(self.ctx, float(denscut), int(densmin), ptr(tot), ptr(core),
+                                           ptr(surf)))
+        return tot, core, surf
+
     def largest_cluster(self, rcontact: float = 3.5) -> np.ndarray:
         o = _c([4, 5], i32)
         nt = _c([3], i32)

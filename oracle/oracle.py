@@ -203,6 +203,30 @@ def dsf_x64(xyz, types, q, nlist, numneigh, neigh_alloc, lj, prm: OrcParams):
     return ff64, pot.value, ec.value, vir
 
 
+def molecule_centers(x, gid, mol, nmol, box):
+    """Centres of molecules 0..nmol-1 (mol[i] < 0: not a molecule atom), made whole around
+    each molecule's lowest-gid atom, wrapped into [0, box]: (nmol, 3)."""
+    com = np.zeros((nmol, 3), _d)
+    lib().orc_molecule_centers(C.c_int32(len(x)), P(np.ascontiguousarray(x, _d)),
+                               P(np.ascontiguousarray(gid, _i)), P(np.ascontiguousarray(mol, _i)),
+                               C.c_int32(nmol), C.c_float(box), P(com))
+    return com
+
+
+def greatest_cluster(com, box, denscut, densmin):
+    """get_greatestCluster on centres: (core + surface, core, surface, denstrack,
+    belongsto)."""
+    nmol = len(com)
+    core, surf = C.c_int32(), C.c_int32()
+    dens = np.zeros(nmol, _i)
+    bel = np.zeros(nmol, _i)
+    lib().orc_greatest_cluster.restype = C.c_int32
+    tot = lib().orc_greatest_cluster(C.c_int32(nmol), P(np.ascontiguousarray(com, _d)), C.c_float(box),
+                                     C.c_double(denscut), C.c_int32(densmin), C.byref(core),
+                                     C.byref(surf), P(dens), P(bel))
+    return int(tot), core.value, surf.value, dens, bel
+
+
 def coul_long(pos, types, q, nlist, numneigh, neigh_alloc, lj, prm: OrcParams):
     n = len(pos)
     a = ainfo(pos, types)

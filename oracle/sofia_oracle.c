@@ -1134,3 +1134,132 @@ void orc_improper_cvff(int32_t ni, const int32_t *improlist, const orc_ainfo *po
     }
     *e_improper = e;
 }
+
+/* ------------------------------------------------------------------------------------
+ * Nucleation order parameter of get_greatestCluster (Source/mod_cluster.f90:15-83).
+ * ---------------------------------------------------------------------------------- */
+
+/* Molecule centres as xyzcenter_molecule forms them (Source/mod_cluster.f90:803-836: the
+ * unweighted mean of the atoms of the molecule made whole) and pbc_molcom wraps them
+ * (841-884).  A molecule is made whole around its atom of lowest global id: each atom's
+ * offset from it is taken with the minimum image, rounded to a multiple of 2^-32 A and
+ * summed in 64-bit integers, so the sum is exact and independent of the atom order; the
+ * centre is ref + sum 2^-32 / count.  mol[i] (0-based, < 0 = not a solute molecule) and
+ * gid[i] label the atoms; x is FP64 x, y, z per atom; com receives x, y, z per molecule. */
+void orc_molecule_centers(int32_t n, const double *x, const int32_t *gid, const int32_t *mol,
+                          int32_t nmol, float box, double *com) {
+    int32_t *ref = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nmol > 0 ? nmol : 1));
+    long long *sum = (long long *)calloc((size_t)3 * (nmol > 0 ? nmol : 1), sizeof(long long));
+    int32_t *cnt = (int32_t *)calloc((size_t)(nmol > 0 ? nmol : 1), sizeof(int32_t));
+    const double bx = (double)box, ib = 1.0 / bx;
+    for (int m = 0; m < nmol; ++m) ref[m] = -1;
+    for (int i = 0; i < n; ++i) {
+        const int m = mol[i];
+        if (m < 0 || m >= nmol) continue;
+        if (ref[m] < 0 || gid[i] < gid[ref[m]]) ref[m] = i;
+    }
+    for (int i = 0; i < n; ++i) {
+        const int m = mol[i];
+        if (m < 0 || m >= nmol) continue;
+        for (int d = 0; d < 3; ++d) {
+            double off = x[3 * (size_t)i + d] - x[3 * (size_t)ref[m] + d];
+            off = off - bx * rint(off * ib);
+            sum[3 * (size_t)m + d] += llrint(off * 4294967296.0);
+        }
+        cnt[m] += 1;
+    }
+    for (int m = 0; m < nmol; ++m) {
+        for (int d = 0; d < 3; ++d) {
+            double c = cnt[m] ? x[3 * (size_t)ref[m] + d] +
+                                    ((double)sum[3 * (size_t)m + d] * (1.0 / 4294967296.0)) / (double)cnt[m]
+                              : 0.0;
+            while (c > bx) c = c - bx;
+            while (c < 0.0) c = c + bx;
+            com[3 * (size_t)m + d] = c;
+        }
+    }
+    free(ref); free(sum); free(cnt);
+}
+
+/* Squared centre distance as mol_build_neighbor_nosort forms it
+ * (Source/mod_neighbor_cluster.f90:91-107): dx = x1 - x2, shift = -box*nint(dx*ibox) with
+ * the real*4 product box*nint, dr2 = dx*dx + dy*dy + dz*dz. */
+static double com_dr2(const double *a, const double *b, float box, float ibox) {
+    double s[3];
+    for (int d = 0; d < 3; ++d) {
+        double dx = a[d] - b[d];
+        const int nn = nint_d(dx * (double)ibox);
+        const float shift = -(box * (float)nn);
+        s[d] = dx + (double)shift;
+    }
+    return (s[0] * s[0] + s[1] * s[1]) + s[2] * s[2];
+}
+
+/* get_greatestCluster on given centres: denstrack(i) = number of other centres closer than
+ * denscut (mol_build_neighbor_nosort); molecules with denstrack >= densmin are dense
+ * (clusterCriterium_surface, Source/mod_cluster.f90:138-152); buildClusters numbers the
+ * connected components of dense molecules, walking molecules in index order and flooding
+ * each new one through its dense neighbours (harvestCluster, recursive in the reference,
+ * an explicit stack here: the members do not depend on the walk order); the greatest
+ * cluster is the first largest in that numbering (68-74); append_surface_molecules adds
+ * the non-dense neighbours of its members (156-182).  Returns core + surface and sets
+ * *core, *surface and, when given, denstrack[nmol] and belongsto[nmol] (cluster number,
+ * 0 = none, surface molecules set to the greatest cluster as the reference does). */
+int32_t orc_greatest_cluster(int32_t nmol, const double *com, float box, double denscut,
+                             int32_t densmin, int32_t *core, int32_t *surface,
+                             int32_t *denstrack_out, int32_t *belongsto_out) {
+    const size_t nm = (size_t)(nmol > 0 ? nmol : 1);
+    const float ibox = 1.0f / box;
+    const double cut2 = denscut * denscut;
+    int32_t *dens = (int32_t *)calloc(nm, sizeof(int32_t));
+    int32_t *belong = (int32_t *)calloc(nm, sizeof(int32_t));
+    int32_t *stack = (int32_t *)malloc(sizeof(int32_t) * nm);
+    int32_t *histo = (int32_t *)calloc(nm + 1, sizeof(int32_t));
+    char *surf = (char *)calloc(nm, 1);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < nmol; ++i) {
+        int cnt = 0;
+        for (int j = 0; j < nmol; ++j)
+            if (j != i && com_dr2(com + 3 * (size_t)i, com + 3 * (size_t)j, box, ibox) < cut2) ++cnt;
+        dens[i] = cnt;
+    }
+    int ncl = 0;
+    for (int ti = 0; ti < nmol; ++ti) {
+        if (belong[ti] != 0 || dens[ti] < densmin) continue;
+        belong[ti] = ++ncl;
+        int top = 0;
+        stack[top++] = ti;
+        while (top > 0) {
+            const int a = stack[--top];
+            for (int j = 0; j < nmol; ++j) {
+                if (j == a || belong[j] != 0 || dens[j] < densmin) continue;
+                if (com_dr2(com + 3 * (size_t)a, com + 3 * (size_t)j, box, ibox) < cut2) {
+                    belong[j] = ncl;
+                    stack[top++] = j;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < nmol; ++i)
+        if (belong[i]) histo[belong[i]] += 1;
+    int best = 0, gb = 0;
+    for (int k = 1; k <= ncl; ++k)
+        if (histo[k] > best) { best = histo[k]; gb = k; }
+    int ns = 0;
+    if (gb)
+        for (int i = 0; i < nmol; ++i) {
+            if (belong[i] != gb) continue;
+            for (int j = 0; j < nmol; ++j)
+                if (j != i && dens[j] < densmin &&
+                    com_dr2(com + 3 * (size_t)i, com + 3 * (size_t)j, box, ibox) < cut2)
+                    surf[j] = 1;
+        }
+    for (int j = 0; j < nmol; ++j)
+        if (surf[j]) { ++ns; belong[j] = gb; }
+    if (denstrack_out) memcpy(denstrack_out, dens, sizeof(int32_t) * (size_t)(nmol > 0 ? nmol : 0));
+    if (belongsto_out) memcpy(belongsto_out, belong, sizeof(int32_t) * (size_t)(nmol > 0 ? nmol : 0));
+    *core = best;
+    *surface = ns;
+    free(dens); free(belong); free(stack); free(histo); free(surf);
+    return best + ns;
+}

@@ -132,6 +132,13 @@ void orc_improper_cvff(int32_t ni, const int32_t *improlist, const orc_ainfo *po
                        const double *k, const int32_t *sign, const int32_t *mult, float box,
                        float ibox, double *f, double *e_improper, double *virial);
 
+/* --- nucleation order parameter: get_greatestCluster (Source/mod_cluster.f90:15-83) --- */
+void orc_molecule_centers(int32_t n, const double *x, const int32_t *gid, const int32_t *mol,
+                          int32_t nmol, float box, double *com);
+int32_t orc_greatest_cluster(int32_t nmol, const double *com, float box, double denscut,
+                             int32_t densmin, int32_t *core, int32_t *surface,
+                             int32_t *denstrack, int32_t *belongsto);
+
 #ifdef __cplusplus
 }
 #endif

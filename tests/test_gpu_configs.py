@@ -62,12 +62,20 @@ def make_systems(specs):
     return [_make(s) for s in specs]
 
 
-def engine_vs_oracle(eng, systems, which):
-    """The engine's pair kernel at its current state against O.dsf_x64 on the engine's own
-    list and FP64 positions, for replicas `which`: [(replica, force rel RMS, LJ rel err,
-    Coulomb rel err, virial rel err, list entries)]."""
+def engine_vs_oracle(eng, systems, which, f32pos=False):
+    """The engine's pair kernel at its current state against the oracle's pair sum on the
+    engine's own list, for replicas `which`: on the engine's FP64 positions (O.dsf_x64), or
+    with f32pos on the real*4 copy of them that the FP64 kernel reads, as the reference
+    keeps positions in real*4.  Returns [(replica, force rel RMS, LJ rel err, Coulomb rel
+    err, virial rel err, list entries)]."""
+    import ctypes
+    from molecular_simulation_code_amd._lib import check, lib, ptr
     f, e = eng.pair_forces()
     x, gid = eng.device_view()
+    if f32pos:
+        a = np.empty((eng.N, 4), np.float32)
+        check(lib().sofia_get_state(eng.ctx, ptr(a), None, None))
+        x = a[:, :3].astype(np.float64)
     off = np.concatenate([[0], np.cumsum(eng.natoms)])
     lj = S.lj_tables()
     out = []
@@ -101,7 +109,10 @@ def test_engine_pair_kernels_match_oracle_on_the_64_replica_mix():
     """C4: the seeded 64-replica mix (5,000-60,000 atoms log-uniform, glycine mole fraction
     0.01-0.04) in one engine, minimised and advanced with Langevin steps; the smallest,
     median and largest replica's pair forces and energies against the oracle, for the
-    mixed engine (what the benchmark times) and for the FP64 engine."""
+    mixed engine (what the benchmark times; its pair kernel reads the FP64 positions through
+    32-bit box fractions, so it is checked against the FP64-position sum) and for the FP64
+    engine (its pair kernel, like the reference, reads the real*4 copy of the positions, so
+    it is checked against the oracle on that copy, to FP64 rounding)."""
     host = _host()
     comps = S.replica_sizes(64)
     say("[C4] generating 64 replicas")
@@ -123,7 +134,8 @@ def test_engine_pair_kernels_match_oracle_on_the_64_replica_mix():
             e.minimize(50)
             e.run(30)
             assert e.rebuilds().min() >= 1
-            _check(engine_vs_oracle(e, systems, which), f"C4 {name}", frtol, ertol)
+            _check(engine_vs_oracle(e, systems, which, f32pos=prec == host.PREC_FP64),
+                   f"C4 {name}", frtol, ertol)
         finally:
             e.close()
 
@@ -133,7 +145,8 @@ def test_engine_pair_kernels_match_oracle_on_the_64_replica_mix():
 def test_engine_pair_kernels_match_oracle_on_the_million_atom_system():
     """C3: 6,000 glycine + 320,000 water (1,020,000 atoms) in one engine, minimised and
     advanced; the whole system's pair forces and energies against the oracle on the
-    engine's own list, mixed precision and FP64."""
+    engine's own list, mixed precision (FP64 positions) and FP64 (the real*4 copy of the
+    positions, as in the 64-replica test)."""
     host = _host()
     say("[C3] generating 6,000 glycine + 320,000 water (about 100 s)")
     s = S.make_system(6000, 320000, seed=20261204)
@@ -147,7 +160,8 @@ def test_engine_pair_kernels_match_oracle_on_the_million_atom_system():
             e.init()
             e.minimize(30)
             e.run(20)
-            _check(engine_vs_oracle(e, [s], [0]), f"C3 {name}", frtol, ertol)
+            _check(engine_vs_oracle(e, [s], [0], f32pos=prec == host.PREC_FP64),
+                   f"C3 {name}", frtol, ertol)
         finally:
             e.close()
 

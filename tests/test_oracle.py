@@ -673,3 +673,86 @@ def test_settle_rejects_a_water_beyond_reach():
     # the same water, moved slightly, is solvable
     x1[1, 1, 2] = x0[1, 1, 2] + 0.05
     assert np.isfinite(O.settle(x0, x1, mo, mh, doh, dhh)).all()
+
+
+# ------------------------------------------------------------------ get_greatestCluster
+def _bfs_greatest(com, box, cut, densmin):
+    """Independent statement of get_greatestCluster: numpy minimum-image distances,
+    scipy connected components of the dense molecules, first-largest by lowest member,
+    surface = non-dense molecules within the cut of a member."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    d = com[:, None, :] - com[None, :, :]
+    d -= box * np.round(d / box)
+    r2 = (d ** 2).sum(-1)
+    near = (r2 < cut * cut) & ~np.eye(len(com), dtype=bool)
+    dens = near.sum(1)
+    dense = dens >= densmin
+    sub = np.flatnonzero(dense)
+    if len(sub) == 0:
+        return 0, 0, 0, dens
+    _, lab = connected_components(csr_matrix(near[np.ix_(sub, sub)]), directed=False)
+    sizes = np.bincount(lab)
+    first = np.array([sub[lab == k].min() for k in range(len(sizes))])
+    best = max(range(len(sizes)), key=lambda k: (sizes[k], -first[k]))
+    members = sub[lab == best]
+    surf = np.flatnonzero(~dense & near[members].any(0))
+    return int(sizes[best] + len(surf)), int(sizes[best]), int(len(surf)), dens
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_greatest_cluster_matches_bruteforce_bfs(seed):
+    """O.greatest_cluster (the restatement of buildClusters / harvestCluster /
+    append_surface_molecules) against an independent connected-components statement on
+    random centres with planted dense clumps, across cutoffs and density thresholds."""
+    rng = np.random.default_rng(seed)
+    box = np.float32(60.0)
+    bg = rng.uniform(0, 60, (300, 3))
+    clumps = [c + rng.normal(0, 2.0, (n, 3)) for c, n in
+              ((rng.uniform(0, 60, 3), 40), (rng.uniform(0, 60, 3), 25), (np.array([0.5, 30, 59.5]), 30))]
+    com = np.mod(np.concatenate([bg] + clumps), 60.0)
+    for cut, dmin in ((4.0, 3), (5.0, 4), (6.5, 6), (3.0, 1)):
+        tot, core, surf, dens, bel = O.greatest_cluster(com, box, cut, dmin)
+        t2, c2, s2, d2 = _bfs_greatest(com, float(box), cut, dmin)
+        assert np.array_equal(dens, d2)
+        assert (tot, core, surf) == (t2, c2, s2), (cut, dmin)
+        assert core > 0
+
+
+def test_greatest_cluster_tie_goes_to_the_lowest_numbered_cluster():
+    """Two identical dense triangles far apart: the greatest cluster is the one holding the
+    lower-indexed molecule (clusters are numbered in molecule order and the first largest
+    wins), and only its own non-dense neighbour joins as surface."""
+    tri = np.array([[0, 0, 0], [2, 0, 0], [1, 1.7, 0]], float)
+    com = np.concatenate([tri + 40.0, tri + 10.0, [[44.5, 40.0, 40.0]], [[14.5, 10.0, 10.0]]])
+    tot, core, surf, dens, bel = O.greatest_cluster(com, np.float32(80.0), 2.5, 2)
+    assert (core, surf, tot) == (3, 0, 3)
+    assert bel[0] == 1 and bel[3] == 2
+    tot, core, surf, dens, bel = O.greatest_cluster(com, np.float32(80.0), 3.0, 2)
+    # now molecule 6 (4.5 - 2 = 2.5 from molecule 1) joins cluster 1 as surface
+    assert (core, surf, tot) == (3, 1, 4) and bel[6] == 1 and bel[7] == 0
+
+
+def test_molecule_centers_are_the_unwrapped_means():
+    """O.molecule_centers: each molecule made whole around its lowest-id atom and averaged,
+    wrapped into [0, box], equals the plain mean of the unwrapped coordinates (to 1e-9 A),
+    also for molecules split by the box edge and whatever the atom order."""
+    rng = np.random.default_rng(9)
+    box = np.float32(30.0)
+    nmol, per = 50, 10
+    centres = rng.uniform(0, 30, (nmol, 3))
+    centres[:5, 0] = 0.3                     # straddling the edge
+    local = rng.normal(0, 1.2, (nmol, per, 3))
+    xu = centres[:, None, :] + local
+    want = np.mod(xu.mean(1), 30.0)
+    x = np.mod(xu, 30.0).reshape(-1, 3)
+    mol = np.repeat(np.arange(nmol), per).astype(np.int32)
+    gid = np.arange(1, nmol * per + 1, dtype=np.int32)
+    order = rng.permutation(len(x))
+    com = O.molecule_centers(x[order], gid[order], mol[order], nmol, box)
+    d = com - want
+    d -= 30.0 * np.round(d / 30.0)
+    assert np.abs(d).max() < 1e-9
+    assert (com >= 0).all() and (com <= 30.0).all()
+    com2 = O.molecule_centers(x, gid, mol, nmol, box)
+    assert np.array_equal(com, com2)
