@@ -106,66 +106,86 @@ __global__ void k_uf_largest(int32_t nmol, const int32_t *__restrict__ parent,
 
 constexpr double COM_Q = 4294967296.0;   // molecule offsets in 2^-32 A, summed exactly
 
-__global__ void k_com_init(int32_t nmol, int32_t *__restrict__ ref_gid, long long *__restrict__ sum,
-                           int32_t *__restrict__ cnt) {
+// Inputs of the greatest-cluster kernels: atom positions x[i*xs + d*ds] (FP64), global ids,
+// molecule per atom (-1 = none), replica per atom and per molecule (nullptr: all in
+// replica 0), first molecule of each replica (nrep + 1) and the replica table.
+struct GcInput {
+    int32_t n, nmol, nrep;
+    const double *x;
+    int32_t xs, ds;
+    const int32_t *gid, *mol, *rep, *mol_rep, *mol_first;
+    const RepInfo *reps;
+};
+// Their working arrays: per molecule (refgid, cnt, dens, parent, size, sum x3, refx x3,
+// com x3), per replica (key, surface) and the give-up flag of the union-find.
+struct GcWork {
+    int32_t *refgid, *cnt, *dens, *parent, *size;
+    long long *sum;
+    double *refx, *com;
+    unsigned long long *key;
+    int32_t *surface, *err;
+};
+
+__device__ inline int rep_of(const int32_t *r, int i) { return r ? r[i] : 0; }
+
+__global__ void k_gc_clear(GcInput in, GcWork w) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nmol) return;
-    ref_gid[m] = 0x7fffffff;
-    sum[3 * (size_t)m] = 0; sum[3 * (size_t)m + 1] = 0; sum[3 * (size_t)m + 2] = 0;
-    cnt[m] = 0;
+    if (m < in.nmol) {
+        w.refgid[m] = 0x7fffffff;
+        w.sum[3 * (size_t)m] = 0; w.sum[3 * (size_t)m + 1] = 0; w.sum[3 * (size_t)m + 2] = 0;
+        w.cnt[m] = 0;
+        w.parent[m] = m;
+        w.size[m] = 0;
+    }
+    if (m < in.nrep) {
+        w.key[m] = 0;
+        w.surface[m] = 0;
+    }
+    if (m == 0) *w.err = 0;
 }
 
-__global__ void k_com_ref(int32_t n, const int32_t *__restrict__ mol, const int32_t *__restrict__ gid,
-                          int32_t *__restrict__ ref_gid) {
+__global__ void k_com_ref(GcInput in, GcWork w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || mol[i] < 0) return;
-    atomicMin(&ref_gid[mol[i]], gid[i]);
+    if (i >= in.n || in.mol[i] < 0) return;
+    atomicMin(&w.refgid[in.mol[i]], in.gid[i]);
 }
 
-__global__ void k_com_refpos(int32_t n, const int32_t *__restrict__ mol, const int32_t *__restrict__ gid,
-                             const int32_t *__restrict__ ref_gid, const double *__restrict__ xd,
-                             double *__restrict__ refx) {
+__global__ void k_com_refpos(GcInput in, GcWork w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || mol[i] < 0) return;
-    const int m = mol[i];
-    if (gid[i] != ref_gid[m]) return;
+    if (i >= in.n || in.mol[i] < 0) return;
+    const int m = in.mol[i];
+    if (in.gid[i] != w.refgid[m]) return;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) refx[3 * (size_t)m + d] = xd[(size_t)d * n + i];
+    for (int d = 0; d < 3; ++d) w.refx[3 * (size_t)m + d] = in.x[(size_t)i * in.xs + (size_t)d * in.ds];
 }
 
-__global__ void k_com_sum(int32_t n, const int32_t *__restrict__ mol, const int32_t *__restrict__ rep,
-                          const RepInfo *__restrict__ reps, const double *__restrict__ xd,
-                          const double *__restrict__ refx, long long *__restrict__ sum,
-                          int32_t *__restrict__ cnt) {
+__global__ void k_com_sum(GcInput in, GcWork w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || mol[i] < 0) return;
-    const int m = mol[i];
-    const double bx = (double)reps[rep[i]].box, ib = 1.0 / bx;
+    if (i >= in.n || in.mol[i] < 0) return;
+    const int m = in.mol[i];
+    const double bx = (double)in.reps[rep_of(in.rep, i)].box, ib = 1.0 / bx;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        double off = xd[(size_t)d * n + i] - refx[3 * (size_t)m + d];
+        double off = in.x[(size_t)i * in.xs + (size_t)d * in.ds] - w.refx[3 * (size_t)m + d];
         off = off - bx * rint(off * ib);
-        atomicAdd((unsigned long long *)&sum[3 * (size_t)m + d],
+        atomicAdd((unsigned long long *)&w.sum[3 * (size_t)m + d],
                   (unsigned long long)__double2ll_rn(off * COM_Q));
     }
-    atomicAdd(&cnt[m], 1);
+    atomicAdd(&w.cnt[m], 1);
 }
 
-__global__ void k_com_final(int32_t nmol, const int32_t *__restrict__ mol_rep,
-                            const RepInfo *__restrict__ reps, const double *__restrict__ refx,
-                            const long long *__restrict__ sum, const int32_t *__restrict__ cnt,
-                            double *__restrict__ com) {
+__global__ void k_com_final(GcInput in, GcWork w) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nmol) return;
-    const double bx = (double)reps[mol_rep[m]].box;
+    if (m >= in.nmol) return;
+    const double bx = (double)in.reps[rep_of(in.mol_rep, m)].box;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        double c = cnt[m] ? refx[3 * (size_t)m + d] +
-                                ((double)sum[3 * (size_t)m + d] * (1.0 / COM_Q)) / (double)cnt[m]
-                          : 0.0;
+        double c = w.cnt[m] ? w.refx[3 * (size_t)m + d] +
+                                  ((double)w.sum[3 * (size_t)m + d] * (1.0 / COM_Q)) / (double)w.cnt[m]
+                            : 0.0;
         while (c > bx) c = c - bx;      // pbc_molcom
         while (c < 0.0) c = c + bx;
-        com[3 * (size_t)m + d] = c;
+        w.com[3 * (size_t)m + d] = c;
     }
 }
 
@@ -182,119 +202,100 @@ __device__ inline double com_dr2(const double *a, const double *b, float box, fl
 }
 
 // denstrack: other centres of the same replica closer than the cut
-__global__ void k_dens_count(int32_t nmol, const int32_t *__restrict__ mol_rep,
-                             const int32_t *__restrict__ mol_first, const RepInfo *__restrict__ reps,
-                             const double *__restrict__ com, double cut2, int32_t *__restrict__ dens) {
+__global__ void k_dens_count(GcInput in, GcWork w, double cut2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nmol) return;
-    const int r = mol_rep[i];
-    const float box = reps[r].box, ibox = reps[r].ibox;
-    const double ci[3] = {com[3 * (size_t)i], com[3 * (size_t)i + 1], com[3 * (size_t)i + 2]};
+    if (i >= in.nmol) return;
+    const int r = rep_of(in.mol_rep, i);
+    const float box = in.reps[r].box, ibox = in.reps[r].ibox;
+    const double ci[3] = {w.com[3 * (size_t)i], w.com[3 * (size_t)i + 1], w.com[3 * (size_t)i + 2]};
     int cnt = 0;
-    for (int j = mol_first[r]; j < mol_first[r + 1]; ++j)
-        if (j != i && com_dr2(ci, com + 3 * (size_t)j, box, ibox) < cut2) ++cnt;
-    dens[i] = cnt;
+    for (int j = in.mol_first[r]; j < in.mol_first[r + 1]; ++j)
+        if (j != i && com_dr2(ci, w.com + 3 * (size_t)j, box, ibox) < cut2) ++cnt;
+    w.dens[i] = cnt;
 }
 
 // components of the dense molecules; each pair is linked once (from its lower index)
-__global__ void k_dens_union(int32_t nmol, const int32_t *__restrict__ mol_rep,
-                             const int32_t *__restrict__ mol_first, const RepInfo *__restrict__ reps,
-                             const double *__restrict__ com, double cut2, int32_t densmin,
-                             const int32_t *__restrict__ dens, int32_t *__restrict__ parent,
-                             int32_t *__restrict__ err) {
+__global__ void k_dens_union(GcInput in, GcWork w, double cut2, int32_t densmin) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nmol || dens[i] < densmin) return;
-    const int r = mol_rep[i];
-    const float box = reps[r].box, ibox = reps[r].ibox;
-    const double ci[3] = {com[3 * (size_t)i], com[3 * (size_t)i + 1], com[3 * (size_t)i + 2]};
-    for (int j = i + 1; j < mol_first[r + 1]; ++j)
-        if (dens[j] >= densmin && com_dr2(ci, com + 3 * (size_t)j, box, ibox) < cut2)
-            uf_union(parent, i, j, err);
+    if (i >= in.nmol || w.dens[i] < densmin) return;
+    const int r = rep_of(in.mol_rep, i);
+    const float box = in.reps[r].box, ibox = in.reps[r].ibox;
+    const double ci[3] = {w.com[3 * (size_t)i], w.com[3 * (size_t)i + 1], w.com[3 * (size_t)i + 2]};
+    for (int j = i + 1; j < in.mol_first[r + 1]; ++j)
+        if (w.dens[j] >= densmin && com_dr2(ci, w.com + 3 * (size_t)j, box, ibox) < cut2)
+            uf_union(w.parent, i, j, w.err);
 }
 
-__global__ void k_dens_size(int32_t nmol, int32_t densmin, const int32_t *__restrict__ dens,
-                            int32_t *__restrict__ parent, int32_t *__restrict__ size) {
+__global__ void k_dens_size(GcInput in, GcWork w, int32_t densmin) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nmol || dens[m] < densmin) return;
-    atomicAdd(&size[uf_find(parent, m)], 1);
+    if (m >= in.nmol || w.dens[m] < densmin) return;
+    atomicAdd(&w.size[uf_find(w.parent, m)], 1);
 }
 
 // the first largest component: largest size, then lowest root (lowest member)
-__global__ void k_dens_greatest(int32_t nmol, int32_t densmin, const int32_t *__restrict__ dens,
-                                const int32_t *__restrict__ parent, const int32_t *__restrict__ size,
-                                const int32_t *__restrict__ mol_rep,
-                                unsigned long long *__restrict__ key) {
+__global__ void k_dens_greatest(GcInput in, GcWork w, int32_t densmin) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nmol || dens[m] < densmin || parent[m] != m) return;
-    const unsigned long long k = ((unsigned long long)(uint32_t)size[m] << 32) |
+    if (m >= in.nmol || w.dens[m] < densmin || w.parent[m] != m) return;
+    const unsigned long long k = ((unsigned long long)(uint32_t)w.size[m] << 32) |
                                  (unsigned long long)(0xffffffffu - (uint32_t)m);
-    atomicMax(&key[mol_rep[m]], k);
+    atomicMax(&w.key[rep_of(in.mol_rep, m)], k);
 }
 
 // non-dense molecules closer than the cut to a member of the greatest component
-__global__ void k_dens_surface(int32_t nmol, const int32_t *__restrict__ mol_rep,
-                               const int32_t *__restrict__ mol_first, const RepInfo *__restrict__ reps,
-                               const double *__restrict__ com, double cut2, int32_t densmin,
-                               const int32_t *__restrict__ dens, int32_t *__restrict__ parent,
-                               const unsigned long long *__restrict__ key,
-                               int32_t *__restrict__ surface) {
+__global__ void k_dens_surface(GcInput in, GcWork w, double cut2, int32_t densmin) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nmol || dens[j] >= densmin) return;
-    const int r = mol_rep[j];
-    const unsigned long long k = key[r];
+    if (j >= in.nmol || w.dens[j] >= densmin) return;
+    const int r = rep_of(in.mol_rep, j);
+    const unsigned long long k = w.key[r];
     if (k == 0) return;
     const int root = (int)(0xffffffffu - (uint32_t)(k & 0xffffffffull));
-    const float box = reps[r].box, ibox = reps[r].ibox;
-    const double cj[3] = {com[3 * (size_t)j], com[3 * (size_t)j + 1], com[3 * (size_t)j + 2]};
-    for (int i = mol_first[r]; i < mol_first[r + 1]; ++i) {
-        if (i == j || dens[i] < densmin) continue;
-        if (com_dr2(com + 3 * (size_t)i, cj, box, ibox) < cut2 && uf_find(parent, i) == root) {
-            atomicAdd(&surface[r], 1);
+    const float box = in.reps[r].box, ibox = in.reps[r].ibox;
+    const double cj[3] = {w.com[3 * (size_t)j], w.com[3 * (size_t)j + 1], w.com[3 * (size_t)j + 2]};
+    for (int i = in.mol_first[r]; i < in.mol_first[r + 1]; ++i) {
+        if (i == j || w.dens[i] < densmin) continue;
+        if (com_dr2(w.com + 3 * (size_t)i, cj, box, ibox) < cut2 && uf_find(w.parent, i) == root) {
+            atomicAdd(&w.surface[r], 1);
             return;
         }
     }
+}
+
+// The whole analysis on stream s: every working array is set by the kernels themselves.
+void launch_greatest(const GcInput &in, const GcWork &w, double denscut, int32_t densmin,
+                     hipStream_t s) {
+    const double cut2 = denscut * denscut;
+    const int nc = in.nmol > in.nrep ? in.nmol : in.nrep;
+    const dim3 gc(nblocks(nc > 0 ? nc : 1, 256)), gm(nblocks(in.nmol > 0 ? in.nmol : 1, 256));
+    const dim3 ga(nblocks(in.n > 0 ? in.n : 1, 256)), b(256);
+    hipLaunchKernelGGL(k_gc_clear, gc, b, 0, s, in, w);
+    if (in.nmol <= 0) return;
+    hipLaunchKernelGGL(k_com_ref, ga, b, 0, s, in, w);
+    hipLaunchKernelGGL(k_com_refpos, ga, b, 0, s, in, w);
+    hipLaunchKernelGGL(k_com_sum, ga, b, 0, s, in, w);
+    hipLaunchKernelGGL(k_com_final, gm, b, 0, s, in, w);
+    hipLaunchKernelGGL(k_dens_count, gm, b, 0, s, in, w, cut2);
+    hipLaunchKernelGGL(k_dens_union, gm, b, 0, s, in, w, cut2, densmin);
+    hipLaunchKernelGGL(k_dens_size, gm, b, 0, s, in, w, densmin);
+    hipLaunchKernelGGL(k_dens_greatest, gm, b, 0, s, in, w, densmin);
+    hipLaunchKernelGGL(k_dens_surface, gm, b, 0, s, in, w, cut2, densmin);
 }
 
 }  // namespace
 
 int cluster_greatest(sofia_ctx *c, double denscut, int32_t densmin, int32_t *total, int32_t *core,
                      int32_t *surface) {
-    const int nmol = c->nmol, n = c->N, nrep = c->nrep;
-    int32_t *err = c->uf_largest + nrep;
-    // scratch of set_molecules: per-replica keys and surface counts after the molecule arrays
-    unsigned long long *key = c->gc_key;
-    int32_t *surf = c->gc_surface;
-    SOFIA_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), c->stream));
-    SOFIA_HIP(hipMemsetAsync(key, 0, sizeof(unsigned long long) * nrep, c->stream));
-    SOFIA_HIP(hipMemsetAsync(surf, 0, sizeof(int32_t) * nrep, c->stream));
-    if (nmol > 0) {
-        const double cut2 = denscut * denscut;
-        const dim3 gm(nblocks(nmol, 256)), ga(nblocks(n, 256)), b(256);
-        hipLaunchKernelGGL(k_com_init, gm, b, 0, c->stream, nmol, c->gc_refgid, c->gc_sum, c->gc_cnt);
-        hipLaunchKernelGGL(k_com_ref, ga, b, 0, c->stream, n, c->a.mol, c->a.gid, c->gc_refgid);
-        hipLaunchKernelGGL(k_com_refpos, ga, b, 0, c->stream, n, c->a.mol, c->a.gid, c->gc_refgid,
-                           c->a.xd, c->gc_refx);
-        hipLaunchKernelGGL(k_com_sum, ga, b, 0, c->stream, n, c->a.mol, c->rep, c->reps_d, c->a.xd,
-                           c->gc_refx, c->gc_sum, c->gc_cnt);
-        hipLaunchKernelGGL(k_com_final, gm, b, 0, c->stream, nmol, c->mol_rep, c->reps_d, c->gc_refx,
-                           c->gc_sum, c->gc_cnt, c->gc_com);
-        hipLaunchKernelGGL(k_dens_count, gm, b, 0, c->stream, nmol, c->mol_rep, c->gc_first, c->reps_d,
-                           c->gc_com, cut2, c->gc_dens);
-        hipLaunchKernelGGL(k_uf_init, gm, b, 0, c->stream, nmol, c->uf_parent, c->uf_size);
-        hipLaunchKernelGGL(k_dens_union, gm, b, 0, c->stream, nmol, c->mol_rep, c->gc_first, c->reps_d,
-                           c->gc_com, cut2, densmin, c->gc_dens, c->uf_parent, err);
-        hipLaunchKernelGGL(k_dens_size, gm, b, 0, c->stream, nmol, densmin, c->gc_dens, c->uf_parent,
-                           c->uf_size);
-        hipLaunchKernelGGL(k_dens_greatest, gm, b, 0, c->stream, nmol, densmin, c->gc_dens,
-                           c->uf_parent, c->uf_size, c->mol_rep, key);
-        hipLaunchKernelGGL(k_dens_surface, gm, b, 0, c->stream, nmol, c->mol_rep, c->gc_first,
-                           c->reps_d, c->gc_com, cut2, densmin, c->gc_dens, c->uf_parent, key, surf);
-    }
+    const int nrep = c->nrep;
+    GcInput in{c->N, c->nmol, nrep, c->a.xd, 1, c->N, c->a.gid, c->a.mol, c->rep, c->mol_rep,
+               c->gc_first, c->reps_d};
+    // uf_largest holds nrep maxima of the contact analysis and, after them, the error flag
+    GcWork w{c->gc_refgid, c->gc_cnt, c->gc_dens, c->uf_parent, c->uf_size, c->gc_sum,
+             c->gc_refx, c->gc_com, c->gc_key, c->gc_surface, c->uf_largest + nrep};
+    launch_greatest(in, w, denscut, densmin, c->stream);
     std::vector<unsigned long long> k(nrep);
     std::vector<int32_t> s(nrep), e(1);
-    SOFIA_HIP(hipMemcpyAsync(k.data(), key, sizeof(unsigned long long) * nrep, hipMemcpyDeviceToHost, c->stream));
-    SOFIA_HIP(hipMemcpyAsync(s.data(), surf, sizeof(int32_t) * nrep, hipMemcpyDeviceToHost, c->stream));
-    SOFIA_HIP(hipMemcpyAsync(e.data(), err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    SOFIA_HIP(hipMemcpyAsync(k.data(), w.key, sizeof(unsigned long long) * nrep, hipMemcpyDeviceToHost, c->stream));
+    SOFIA_HIP(hipMemcpyAsync(s.data(), w.surface, sizeof(int32_t) * nrep, hipMemcpyDeviceToHost, c->stream));
+    SOFIA_HIP(hipMemcpyAsync(e.data(), w.err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     SOFIA_HIP(hipStreamSynchronize(c->stream));
     if (e[0]) return fail(SOFIA_E_STATE, "union-find gave up on a union (corrupted forest)");
     for (int r = 0; r < nrep; ++r) {
