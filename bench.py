@@ -92,11 +92,22 @@ def _make(args):
 
 
 def make_systems(specs, workers):
+    """Builds the systems (in a process pool when workers > 1), reporting progress on
+    stderr every 8 systems so that a long generation is seen to advance."""
+    out = []
     if workers > 1 and len(specs) > 1:
         import multiprocessing as mp
         with mp.get_context("spawn").Pool(workers) as pool:
-            return pool.map(_make, specs)
-    return [_make(s) for s in specs]
+            for s in pool.imap(_make, specs):
+                out.append(s)
+                if len(out) % 8 == 0:
+                    log(f"[bench] generated {len(out)}/{len(specs)} systems")
+        return out
+    for s in specs:
+        out.append(_make(s))
+        if len(out) % 8 == 0:
+            log(f"[bench] generated {len(out)}/{len(specs)} systems")
+    return out
 
 
 def prepare_engine(eng, minimize, equil, tune=True):

@@ -144,15 +144,20 @@ __global__ void k_gc_clear(GcInput in, GcWork w) {
     if (m == 0) *w.err = 0;
 }
 
+// A molecule label past nmol is reported (flag 2) and its atom left out, never indexed.
 __global__ void k_com_ref(GcInput in, GcWork w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= in.n || in.mol[i] < 0) return;
+    if (in.mol[i] >= in.nmol) {
+        atomicOr(w.err, 2);
+        return;
+    }
     atomicMin(&w.refgid[in.mol[i]], in.gid[i]);
 }
 
 __global__ void k_com_refpos(GcInput in, GcWork w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= in.n || in.mol[i] < 0) return;
+    if (i >= in.n || in.mol[i] < 0 || in.mol[i] >= in.nmol) return;
     const int m = in.mol[i];
     if (in.gid[i] != w.refgid[m]) return;
 #pragma unroll
@@ -161,7 +166,7 @@ __global__ void k_com_refpos(GcInput in, GcWork w) {
 
 __global__ void k_com_sum(GcInput in, GcWork w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= in.n || in.mol[i] < 0) return;
+    if (i >= in.n || in.mol[i] < 0 || in.mol[i] >= in.nmol) return;
     const int m = in.mol[i];
     const double bx = (double)in.reps[rep_of(in.rep, i)].box, ib = 1.0 / bx;
 #pragma unroll
@@ -297,6 +302,7 @@ int cluster_greatest(sofia_ctx *c, double denscut, int32_t densmin, int32_t *tot
     SOFIA_HIP(hipMemcpyAsync(s.data(), w.surface, sizeof(int32_t) * nrep, hipMemcpyDeviceToHost, c->stream));
     SOFIA_HIP(hipMemcpyAsync(e.data(), w.err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     SOFIA_HIP(hipStreamSynchronize(c->stream));
+    if (e[0] & 2) return fail(SOFIA_E_STATE, "an atom's molecule label is nmol or more");
     if (e[0]) return fail(SOFIA_E_STATE, "union-find gave up on a union (corrupted forest)");
     for (int r = 0; r < nrep; ++r) {
         const int32_t cs = (int32_t)(k[r] >> 32);

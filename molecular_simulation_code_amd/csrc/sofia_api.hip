@@ -58,6 +58,10 @@ void dfree(T *&p) {
 // record layout, thermostat constants) as they were at capture; anything that changes one
 // of them drops the graph, and the next graph launch captures it again.
 void drop_graph(sofia_ctx *c) {
+    if (!c->graph && !c->graphk) return;
+    // an executable graph is destroyed only once the stream has drained: launches of it
+    // may still be queued (md_launch without md_finish)
+    (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graphk) (void)hipGraphExecDestroy(c->graphk);
     c->graph = nullptr;
