@@ -346,6 +346,19 @@ int sofia_cluster_largest(sofia_ctx *ctx, float rcontact, int32_t n_o, const int
  * reference's return value; core and surface may be NULL. */
 int sofia_cluster_greatest(sofia_ctx *ctx, double denscut, int32_t densmin, int32_t *total,
                            int32_t *core, int32_t *surface);
+/* No counterpart in the reference.  sofia_cluster_greatest for one system held in device
+ * memory by the caller, allocating nothing, for callers that cannot allocate (code under
+ * hipGraph capture, tight serving loops).  Device inputs: x (FP64 x, y, z per atom, 3n),
+ * gid (global ids, 1-based), mol (solute molecule per atom, 0..nmol-1, or -1); box in A.
+ * Device output out[4]: total, core, surface, and flags (1 = the union-find gave up, 2 = a
+ * molecule label of nmol or more, whose atom was left out); the
+ * results are ready once `stream` (a hipStream_t, NULL = the default stream) has run the
+ * call's kernels.  Working memory is a fixed device array of the library (about 2.9 MB per
+ * GPU, kept until the process ends) that every call on a GPU shares: at most 32768
+ * molecules per call (more is SOFIA_E_ARG), and calls that overlap on one GPU race. */
+int sofia_cluster_greatest_noalloc(int32_t n, const double *x, const int32_t *gid,
+                                   const int32_t *mol, int32_t nmol, float box, double denscut,
+                                   int32_t densmin, int32_t *out, void *stream);
 
 /* ---------------------------------------------------------------- files (host only) */
 /* Restart file as dump_global_velocity_id_mpi writes it (Source/mod_datadump.f90:113-151):

@@ -119,6 +119,7 @@ SIGNATURES = {
     "sofia_set_molecules": [_vp, _vp],
     "sofia_cluster_largest": [_vp, _f, _i32, _vp, _i32, _vp, _vp],
     "sofia_cluster_greatest": [_vp, _d, _i32, _vp, _vp, _vp],
+    "sofia_cluster_greatest_noalloc": [_i32, _vp, _vp, _vp, _i32, _f, _d, _i32, _vp, _vp],
     "sofia_write_restart": [C.c_char_p, _f, _f, _i32, _f, _i32, _vp, _vp, _vp],
     "sofia_read_restart": [C.c_char_p, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
     "sofia_read_gro": [C.c_char_p, _i32, _vp, _vp, _vp],

@@ -7,6 +7,9 @@
 // within the cutoff, so each N atom scans its list for O atoms of other solute molecules.
 // Roots are linked lowest-index-wins with compare-and-swap, so the final partition is
 // unique whatever the thread order; sizes and the per-replica maximum are integer atomics.
+#include <mutex>
+#include <string>
+
 #include "sofia_device.h"
 
 namespace sofia {
@@ -346,6 +349,103 @@ int cluster_largest(sofia_ctx *c, float rcontact, const std::vector<int> &o_type
     SOFIA_HIP(hipStreamSynchronize(c->stream));
     if (out[c->nrep]) return fail(SOFIA_E_STATE, "union-find gave up on a union (corrupted forest)");
     for (int r = 0; r < c->nrep; ++r) largest[r] = out[r];
+    return SOFIA_OK;
+}
+
+// ------------------------------------------------------------------ allocation-free variant
+// get_greatestCluster for one system whose positions, ids and molecule labels the caller
+// holds on the device, for callers that cannot allocate (graph capture, serving loops).
+// Every working array lives in `scratch`, a fixed file-scope device array; each GPU holds
+// its own copy until the process ends, and host code reaches it only through the address
+// hipGetSymbolAddress returns on that device.  Calls that overlap on one GPU share it and
+// race.
+namespace {
+
+constexpr int GC_MAX_MOL = 32768;
+constexpr size_t GC_ALIGN = 256;
+constexpr size_t gc_round(size_t b) { return (b + GC_ALIGN - 1) / GC_ALIGN * GC_ALIGN; }
+// layout: RepInfo, molecule range, per-replica key / surface / flag, then per molecule
+constexpr size_t GC_OFF_REP = 0;
+constexpr size_t GC_OFF_FIRST = GC_OFF_REP + gc_round(sizeof(RepInfo));
+constexpr size_t GC_OFF_KEY = GC_OFF_FIRST + gc_round(2 * sizeof(int32_t));
+constexpr size_t GC_OFF_SURF = GC_OFF_KEY + gc_round(sizeof(unsigned long long));
+constexpr size_t GC_OFF_ERR = GC_OFF_SURF + gc_round(sizeof(int32_t));
+constexpr size_t GC_OFF_I32 = GC_OFF_ERR + gc_round(sizeof(int32_t));     // 5 int32 arrays
+constexpr size_t GC_OFF_SUM = GC_OFF_I32 + gc_round(5 * sizeof(int32_t) * GC_MAX_MOL);
+constexpr size_t GC_OFF_REFX = GC_OFF_SUM + gc_round(3 * sizeof(long long) * GC_MAX_MOL);
+constexpr size_t GC_OFF_COM = GC_OFF_REFX + gc_round(3 * sizeof(double) * GC_MAX_MOL);
+constexpr size_t GC_SCRATCH_BYTES = GC_OFF_COM + gc_round(3 * sizeof(double) * GC_MAX_MOL);
+
+__device__ __align__(256) unsigned char scratch[GC_SCRATCH_BYTES];
+
+constexpr int GC_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[GC_MAX_DEV] = {};
+
+// scratch's address on the current device, looked up once per device
+int scratch_on_device(unsigned char **out) {
+    int dev = 0;
+    SOFIA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= GC_MAX_DEV) return fail(SOFIA_E_NODEV, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        SOFIA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(SOFIA_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SOFIA_OK;
+}
+
+// the one-replica table and molecule range, written on the device (no host copies)
+__global__ void k_gc_setup(RepInfo *rep, int32_t *first, float box, int32_t nmol) {
+    if (threadIdx.x != 0) return;
+    RepInfo R{};
+    R.box = box;
+    R.ibox = (float)(1.0 / (double)box);
+    *rep = R;
+    first[0] = 0;
+    first[1] = nmol;
+}
+
+// out = total, core, surface, union-find give-up flag
+__global__ void k_gc_out(const unsigned long long *key, const int32_t *surface, const int32_t *err,
+                         int32_t *out) {
+    if (threadIdx.x != 0) return;
+    const int32_t core = (int32_t)(*key >> 32);
+    out[0] = core + *surface;
+    out[1] = core;
+    out[2] = *surface;
+    out[3] = *err;
+}
+
+}  // namespace
+
+int cluster_greatest_noalloc(int32_t n, const double *x, const int32_t *gid, const int32_t *mol,
+                             int32_t nmol, float box, double denscut, int32_t densmin,
+                             int32_t *out, hipStream_t s) {
+    if (n < 0 || nmol < 0 || !out || !(denscut > 0.0) || densmin < 0 || !(box > 0.0f))
+        return fail(SOFIA_E_ARG, "bad argument");
+    if (n > 0 && (!x || !gid || !mol)) return fail(SOFIA_E_ARG, "null device array");
+    if (nmol > GC_MAX_MOL)
+        return fail(SOFIA_E_ARG, "more molecules than the allocation-free scratch holds (" +
+                                     std::to_string(GC_MAX_MOL) + ")");
+    unsigned char *p = nullptr;
+    const int rc = scratch_on_device(&p);
+    if (rc != SOFIA_OK) return rc;
+    RepInfo *rep = (RepInfo *)(p + GC_OFF_REP);
+    int32_t *first = (int32_t *)(p + GC_OFF_FIRST);
+    int32_t *i32 = (int32_t *)(p + GC_OFF_I32);
+    GcInput in{n, nmol, 1, x, 3, 1, gid, mol, nullptr, nullptr, first, rep};
+    GcWork w{i32, i32 + GC_MAX_MOL, i32 + 2 * GC_MAX_MOL, i32 + 3 * GC_MAX_MOL, i32 + 4 * GC_MAX_MOL,
+             (long long *)(p + GC_OFF_SUM), (double *)(p + GC_OFF_REFX), (double *)(p + GC_OFF_COM),
+             (unsigned long long *)(p + GC_OFF_KEY), (int32_t *)(p + GC_OFF_SURF),
+             (int32_t *)(p + GC_OFF_ERR)};
+    hipLaunchKernelGGL(k_gc_setup, dim3(1), dim3(64), 0, s, rep, first, box, nmol);
+    launch_greatest(in, w, denscut, densmin, s);
+    hipLaunchKernelGGL(k_gc_out, dim3(1), dim3(64), 0, s, w.key, w.surface, w.err, out);
+    SOFIA_HIP(hipGetLastError());
     return SOFIA_OK;
 }
 

@@ -2266,4 +2266,11 @@ int sofia_cluster_greatest(sofia_ctx *c, double denscut, int32_t densmin, int32_
     return cluster_greatest(c, denscut, densmin, total, core, surface);
 }
 
+int sofia_cluster_greatest_noalloc(int32_t n, const double *x, const int32_t *gid,
+                                   const int32_t *mol, int32_t nmol, float box, double denscut,
+                                   int32_t densmin, int32_t *out, void *stream) {
+    return cluster_greatest_noalloc(n, x, gid, mol, nmol, box, denscut, densmin, out,
+                                    (hipStream_t)stream);
+}
+
 }  // extern "C"

@@ -274,6 +274,9 @@ int cluster_largest(sofia_ctx *c, float rcontact, const std::vector<int> &o_type
                     const std::vector<int> &n_types, int32_t *largest);
 int cluster_greatest(sofia_ctx *c, double denscut, int32_t densmin, int32_t *total, int32_t *core,
                      int32_t *surface);
+int cluster_greatest_noalloc(int32_t n, const double *x, const int32_t *gid, const int32_t *mol,
+                             int32_t nmol, float box, double denscut, int32_t densmin,
+                             int32_t *out, hipStream_t s);
 
 // error helpers (sofia_api.cpp)
 int fail(int code, const std::string &msg);
