@@ -384,9 +384,12 @@ def profile_run(args, workers):
             # profiled process captures one graph only
             eng.init()
             eng.minimize(args.minimize)
+            log(f"[profile-run] minimised ({args.minimize} steps)")
             if args.equil > 0:
                 eng.set_thermostat(gammas=[20.0] * len(eng.reps))
-                eng.run(args.equil, use_graph=False)
+                for k in range(0, args.equil, 100):
+                    eng.run(min(100, args.equil - k), use_graph=False)
+                    log(f"[profile-run] equilibration {min(k + 100, args.equil)}/{args.equil}")
                 eng.set_thermostat()
             tuning = args.tuning
         else:

@@ -32,6 +32,8 @@
 // Loads.  The list row, the position records and the charges are read with raw buffer
 // loads (32-bit offsets; an offset past the end of a buffer reads zero), so the software
 // pipeline can prefetch past the end of a row without clamping its index.
+#include <type_traits>
+
 #include "sofia_device.h"
 
 namespace sofia {
@@ -55,6 +57,13 @@ __device__ __forceinline__ int4 ld96(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)off, 0, 0);
     return make_int4((int)v[0], (int)v[1], (int)v[2], 0);
 }
+// byte offset of record j: 12 or 16 bytes per record.  The 12-byte form uses the 24-bit
+// multiply (full VALU rate; a 32-bit multiply issues at a quarter of it): the host keeps
+// 12-byte records to systems of fewer than 2^24 atoms.
+template <bool QT>
+__device__ __forceinline__ uint32_t rec_off(uint32_t j) {
+    return QT ? __umul24(j, 12u) : (j << 4);
+}
 __device__ __forceinline__ uint32_t clamp_bytes(size_t b) {
     return b > 0xffffffffull ? 0xffffffffu : (uint32_t)b;
 }
@@ -77,26 +86,47 @@ struct PairOut {
     float t;       // position inside the table bin
     int kb;        // table bin (tn = the zero bin)
     float ebelow;  // Coulomb energy per C qi qj below the table start (0 elsewhere)
+    bool below;    // r below the table start
 };
+
+// Constants of the direct DSF evaluation, rounded to float once per kernel (uniform, so they
+// stay in scalar registers).
+struct DsfDirect {
+    float al, al2, pa, a1, a2, a3, a4, a5, c2, fs, es;
+};
+__device__ __forceinline__ DsfDirect dsf_direct_consts(const DevParams &P) {
+    DsfDirect D;
+    D.al = (float)P.alpha;
+    D.al2 = D.al * D.al;
+    D.pa = (float)(P.ewald_p * P.alpha);
+    D.a1 = (float)P.a1; D.a2 = (float)P.a2; D.a3 = (float)P.a3; D.a4 = (float)P.a4; D.a5 = (float)P.a5;
+    D.c2 = (float)(2.0 * P.alpha * P.my_pis_inv);
+    D.fs = (float)P.f_shift;
+    D.es = (float)P.e_shift;
+    return D;
+}
 
 // DSF Coulomb terms evaluated directly with the A&S 7.1.26 erfc (Source/mod_force_nonbond.f90:
 // 160-183), for separations below the table start: (force over r, energy) per C qi qj.
-__device__ __noinline__ float2 dsf_direct(float r, float dri, const DevParams &P) {
-    const float al = (float)P.alpha;
-    const float erfcd = expf(-(al * al) * (r * r));
-    const float t = 1.0f / (1.0f + (float)(P.ewald_p * P.alpha) * r);
-    const float erfcc =
-        t * ((float)P.a1 + t * ((float)P.a2 + t * ((float)P.a3 + t * ((float)P.a4 + t * (float)P.a5)))) *
-        erfcd;
-    const float fs = (float)P.f_shift;
-    const float g = (erfcc * dri + (float)(2.0 * P.alpha * P.my_pis_inv) * erfcd + r * fs) * (dri * dri);
-    const float e = erfcc * dri - (float)P.e_shift - r * fs;
+// Inlined (a call would make the kernel keep its parameters on the stack).
+__device__ __forceinline__ float2 dsf_direct(float r, float dri, const DsfDirect &D) {
+    const float erfcd = expf(-D.al2 * (r * r));
+    const float t = 1.0f / (1.0f + D.pa * r);
+    const float erfcc = t * (D.a1 + t * (D.a2 + t * (D.a3 + t * (D.a4 + t * D.a5)))) * erfcd;
+    const float g = (erfcc * dri + D.c2 * erfcd + r * D.fs) * (dri * dri);
+    const float e = erfcc * dri - D.es - r * D.fs;
     return make_float2(g, e);
 }
 
+// Separations below the table start (u < 0) read the zero bin when ZB (their Coulomb
+// energy then comes from the direct form alone); force-only kernels leave them on whatever
+// in-range bin u gives.  DIRECT: such pairs take the direct form here; otherwise the caller
+// sees o.below and evaluates them in a separate pass (keeping the direct form out of its hot
+// loop, where its registers would cost occupancy).
+template <bool ZB, bool DIRECT>
 __device__ __forceinline__ PairOut pair_dsf(float dr2, float qq, const float4 &lj,
                                             const float4 *__restrict__ s_tab, float tinv,
-                                            float u0, int tn, float rcut2, const DevParams &P) {
+                                            float u0, int tn, float rcut2, const DsfDirect &D) {
     PairOut o;
     const float dri = __builtin_amdgcn_rsqf(dr2);
     const float r = dr2 * dri;
@@ -105,16 +135,16 @@ __device__ __forceinline__ PairOut pair_dsf(float dr2, float qq, const float4 &l
     if (dr2 > rcut2) dr6i = 0.0f;
     const float flj = dr6i * __builtin_fmaf(lj.x, dr6i, -lj.y);
     const float u = __builtin_fmaf(r, tinv, u0);
-    // r beyond the table (u >= tn) reads the zero bin; r below its start (u < 0) reads the
-    // zero bin too and takes the direct form instead (the reference evaluates DSF at every
-    // r < cut_coul)
-    const bool below = u < 0.0f;
-    o.kb = below ? tn : (int)min((uint32_t)(int)u, (uint32_t)tn);
+    // r beyond the table (u >= tn) reads the zero bin; r below its start (u < 0) takes the
+    // direct form (the reference evaluates DSF at every r < cut_coul)
+    o.below = u < 0.0f;
+    const int kb = (int)min((uint32_t)(int)u, (uint32_t)tn);
+    o.kb = (ZB && o.below) ? tn : kb;
     o.t = __builtin_amdgcn_fractf(u);
     float gc = cubic_f(s_tab[o.kb], o.t);
     o.ebelow = 0.0f;
-    if (below) {
-        const float2 d = dsf_direct(r, dri, P);
+    if (DIRECT && o.below) {
+        const float2 d = dsf_direct(r, dri, D);
         gc = d.x;
         o.ebelow = d.y;
     }
@@ -164,12 +194,15 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             double *__restrict__ fnb, float *__restrict__ ff32, long long *__restrict__ eacc) {
     // QT: charges by atom type; 12-byte records (x, y, z) and a pair table (lj1, lj2,
     // C q_i q_j) in place of the charge in the record
+    // s_dyn[0] is a zero pad ahead of the type-pair tables: a slot whose record lies past
+    // the end of the position buffer reads type 0, i.e. element -1 of its row
     extern __shared__ float4 s_dyn[];
     const int nt2 = P.ntypes * P.ntypes;
     const int tn = P.tab_n;
-    float4 *s_lj = s_dyn;
-    float4 *s_pt = s_dyn + nt2;
-    float4 *s_tab = s_dyn + (QT ? 2 : 1) * nt2;
+    float4 *s_lj = s_dyn + 1;
+    float4 *s_pt = s_lj + nt2;
+    float4 *s_tab = s_lj + (QT ? 2 : 1) * nt2;
+    if (threadIdx.x == 0) s_dyn[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int t = threadIdx.x; t < nt2; t += blockDim.x) {
         s_lj[t] = ljtab[t];
         if (QT) s_pt[t] = ptab[t];
@@ -208,6 +241,7 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     const float rcut2 = (float)P.rcut2;
     const float qc = P.coulpre * __int_as_float(pi.w);
     const float tinv = P.tab_inv_h, u0 = -P.tab_r0 * P.tab_inv_h;
+    const DsfDirect D = dsf_direct_consts(P);
     const float4 *s_etab = s_tab + tn + 1;
 
     double fxq = 0.0, fyq = 0.0, fzq = 0.0;
@@ -218,7 +252,9 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     // consumed, so its next gather (4 entries ahead) and list index (8 entries ahead) are
     // in flight for the other three slots and no register is moved while a load is
     // outstanding; scheduling barriers keep the compiler from hoisting the reloads.
-    // Entries past the end of the row are computed on stale slot data and then dropped.
+    // Entries past the end of the row are computed on whatever the list buffer holds there
+    // (stale indices of an earlier build, or a record past the end of the position buffer,
+    // which reads zero) and then dropped: no index is clamped in the loop.
     const uint32_t step = 4u * L;
     const uint32_t row0 = (uint32_t)(ii - iw) * (uint32_t)na * 4u;
     uint32_t off = row0 + (uint32_t)sub * 4u;
@@ -231,49 +267,61 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        PJ[u] = QT ? ld96(prs, JJ[u] * RB) : ld128(prs, JJ[u] * RB);
+        PJ[u] = QT ? ld96(prs, rec_off<QT>(JJ[u])) : ld128(prs, rec_off<QT>(JJ[u]));
         JJ[u] = ld32(lrs, off + (u + 4) * step);
         __builtin_amdgcn_sched_barrier(0);
     }
+    // One list entry on record pj.  First pass (direct = false): pairs below the table start
+    // are left out and flagged in lowq; second pass (direct = true): only they count, with
+    // the direct form.  Either way a pair's quanta are those of a single-pass evaluation.
+    bool lowq = false;
+    auto entry = [&](const int4 &pj, bool valid, auto direct) {
+        constexpr bool DIR = decltype(direct)::value;
+        const uint32_t tj = (uint32_t)pj.x & TYPE_BITS;
+        const float dx = sep_f(xi, (int)((uint32_t)pj.x & ~TYPE_BITS));
+        const float dy = sep_f(pi.y, pj.y), dz = sep_f(pi.z, pj.z);
+        const float dr2 = norm2_f(dx, dy, dz) * s2;
+        const float4 lj = ljrow[tj];
+        const float qq = QT ? lj.z : qc * __int_as_float(pj.w);
+        const PairOut o = pair_dsf<ENERGY, DIR>(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2, D);
+        const bool take = valid && (DIR ? o.below : !o.below);
+        if (!DIR) lowq = lowq || (valid && o.below);
+        const float fs = take ? o.force * fq : 0.0f;
+        fxq += (double)__builtin_truncf(dx * fs);
+        fyq += (double)__builtin_truncf(dy * fs);
+        fzq += (double)__builtin_truncf(dz * fs);
+        if (ENERGY) {
+            if (take) {
+                const float4 le = erow[tj];
+                pot += (double)(o.dr6i * __builtin_fmaf(o.dr6i, le.z, -le.w));
+                ecl += (double)(qq * (cubic_f(s_etab[o.kb], o.t) + o.ebelow));
+                const float fv = o.force * s2;
+                vx += (double)(dx * dx * fv);
+                vy += (double)(dy * dy * fv);
+                vz += (double)(dz * dz * fv);
+            }
+        }
+    };
+
     off += 8u * step;
     for (int k = sub; k < nn; k += 4 * L) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const bool valid = k + u * L < nn;
-            const int4 pj = PJ[u];
-            const uint32_t tj = (uint32_t)pj.x & TYPE_BITS;
-            const float dx = sep_f(xi, (int)((uint32_t)pj.x & ~TYPE_BITS));
-            const float dy = sep_f(pi.y, pj.y), dz = sep_f(pi.z, pj.z);
-            const float dr2 = norm2_f(dx, dy, dz) * s2;
-            const float4 lj = ljrow[valid ? tj : 1u];
-            const float qq = QT ? lj.z : qc * __int_as_float(pj.w);
-            const PairOut o = pair_dsf(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2, P);
-            const float fs = valid ? o.force * fq : 0.0f;
-            fxq += (double)__builtin_truncf(dx * fs);
-            fyq += (double)__builtin_truncf(dy * fs);
-            fzq += (double)__builtin_truncf(dz * fs);
-            if (ENERGY) {
-                if (valid) {
-                    const float4 le = erow[tj];
-                    pot += (double)(o.dr6i * __builtin_fmaf(o.dr6i, le.z, -le.w));
-                    ecl += (double)(qq * (cubic_f(s_etab[o.kb], o.t) + o.ebelow));
-                    const float fv = o.force * s2;
-                    vx += (double)(dx * dx * fv);
-                    vy += (double)(dy * dy * fv);
-                    vz += (double)(dz * dz * fv);
-                }
-            }
-            // entries past the end of the row (their index is whatever follows in the
-            // list buffer) gather the atom's own record instead, a cache hit
-            const uint32_t jg = (k + (u + 4) * L < nn) ? JJ[u] : (uint32_t)ii;
-            PJ[u] = QT ? ld96(prs, jg * RB) : ld128(prs, jg * RB);
-            // likewise list reads past the row re-read its first entry instead of pulling the
-            // unused tail of the row from memory
-            JJ[u] = ld32(lrs, (k + (u + 8) * L < nn) ? off + u * step : row0);
+            entry(PJ[u], k + u * L < nn, std::false_type{});
+            PJ[u] = QT ? ld96(prs, rec_off<QT>(JJ[u])) : ld128(prs, rec_off<QT>(JJ[u]));
+            JJ[u] = ld32(lrs, off + u * step);
             // VALU, SALU and LDS work may cross; memory loads stay in slot order
             __builtin_amdgcn_sched_barrier(0x0086);
         }
         off += 4u * step;
+    }
+    // pairs closer than the table start (none in a physical system): the lane walks its
+    // entries again and evaluates those directly
+    if (lowq) {
+        for (int k = sub; k < nn; k += L) {
+            const uint32_t j = ld32(lrs, row0 + 4u * (uint32_t)k);
+            entry(QT ? ld96(prs, rec_off<QT>(j)) : ld128(prs, rec_off<QT>(j)), true, std::true_type{});
+        }
     }
 
     const double qinv = 1.0 / (double)PAIR_QUANT;
@@ -356,6 +404,7 @@ k_dsf_compact(int32_t n, const int4 *__restrict__ pf,
     const float keep2 = fmaxf(rcut2, (float)P.cut_coulsq) * 1.0001f;
     const float qc = P.coulpre * __int_as_float(pi.w);
     const float tinv = P.tab_inv_h, u0 = -P.tab_r0 * P.tab_inv_h;
+    const DsfDirect D = dsf_direct_consts(P);
     const float4 *s_etab = s_tab + tn + 1;
 
     double fxq = 0.0, fyq = 0.0, fzq = 0.0;
@@ -370,7 +419,7 @@ k_dsf_compact(int32_t n, const int4 *__restrict__ pf,
             const float dx = e.x, dy = e.y, dz = e.z;
             const float dr2 = norm2_f(dx, dy, dz) * s2;
             const float qq = qc * e.w;
-            const PairOut o = pair_dsf(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2, P);
+            const PairOut o = pair_dsf<ENERGY, true>(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2, D);
             const float fs = o.force * fq;
             fxq += (double)__builtin_truncf(dx * fs);
             fyq += (double)__builtin_truncf(dy * fs);
@@ -452,7 +501,7 @@ void launch_lanes(sofia_ctx *c, bool energies, float *ff32, bool qt) {
     const long long threads = (long long)c->N * L;
     const int grid = (int)((threads + block - 1) / block);
     const int nt2 = c->dp.ntypes * c->dp.ntypes;
-    const size_t lds = sizeof(float4) * ((qt ? 2 : 1) * nt2 + (energies ? 2 : 1) * (c->dp.tab_n + 1));
+    const size_t lds = sizeof(float4) * (1 + (qt ? 2 : 1) * nt2 + (energies ? 2 : 1) * (c->dp.tab_n + 1));
 #define DSF_LAUNCH(E, Q)                                                                          \
     hipLaunchKernelGGL((k_dsf_mixed<L, E, Q>), dim3(grid), dim3(block), lds, c->stream, c->N,       \
                        c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh, c->neigh_alloc,          \

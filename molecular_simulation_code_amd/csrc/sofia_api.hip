@@ -1600,12 +1600,15 @@ int classify_shake(sofia_ctx *c) {
     return upload(c->a.wclus, wclus.data(), sizeof(int32_t) * wclus.size());
 }
 
-// Charges by atom type?  Then the mixed pair kernel reads 12-byte position records and takes
-// C q_i q_j from a per-type-pair table, formed in float32 exactly as the kernel forms it
-// from per-atom charges ((C q_i) q_j), so the forces do not change.
+// Charges by atom type (and fewer than 2^24 atoms)?  Then the mixed pair kernel reads
+// 12-byte position records and takes C q_i q_j from a per-type-pair table, formed in float32
+// exactly as the kernel forms it from per-atom charges ((C q_i) q_j), so the forces do not
+// change.
 int classify_charges(sofia_ctx *c) {
     c->q_by_type = false;
-    if (c->p.precision != SOFIA_PREC_MIXED || c->p.coul_flag == SOFIA_COUL_EWALD || c->lj_host.empty())
+    // 12-byte record offsets are formed with a 24-bit multiply: fewer than 2^24 atoms
+    if (c->p.precision != SOFIA_PREC_MIXED || c->p.coul_flag == SOFIA_COUL_EWALD || c->lj_host.empty() ||
+        c->N >= (1 << 24))
         return SOFIA_OK;
     TRY(sync_check(c));
     const int n = c->N, nt = c->p.ntypes;
