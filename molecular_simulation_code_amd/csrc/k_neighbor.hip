@@ -15,6 +15,8 @@
 // the staged candidates 64 at a time, compacting hits with a ballot + popcount prefix so
 // the output keeps the reference's order.  Several replicas share one launch: every
 // cell belongs to one replica and only replicas flagged for a rebuild do work.
+#include <algorithm>
+
 #include "sofia_internal.h"
 
 namespace sofia {
@@ -45,27 +47,35 @@ __device__ inline float mimg32(float dx, float box, float ibox) {
     return dx - s * box;
 }
 
+// The list-maintenance kernels below run every step and do work only for the replicas
+// flagged in `rebuild`; `any` (the word after the nrep flags) is set whenever one is, so a
+// step that rebuilds nothing ends each of them after one read.  They loop over their items
+// with a fixed grid (at most GRID_CAP workgroups), which keeps such a step's launches short.
+constexpr int GRID_CAP = 2048;
+
 __global__ void k_cell_reset(int32_t ncell, const int32_t *__restrict__ cell_rep,
-                             const int32_t *__restrict__ rebuild, int32_t *__restrict__ ccount) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncell && rebuild[cell_rep[c]]) ccount[c] = 0;
+                             const int32_t *__restrict__ rebuild, const int32_t *__restrict__ any,
+                             int32_t *__restrict__ ccount) {
+    if (!*any) return;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
+        if (rebuild[cell_rep[c]]) ccount[c] = 0;
 }
 
 __global__ void k_bin(int32_t n, const float4 *__restrict__ pos4, const int32_t *__restrict__ rep,
                       const RepInfo *__restrict__ reps, const int32_t *__restrict__ rebuild,
-                      int32_t *__restrict__ ccount, int32_t *__restrict__ acell,
-                      int32_t *__restrict__ aslot, int32_t *__restrict__ perm,
-                      int32_t *__restrict__ tt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int r = rep[i];
-    // perm and tt of a replica that rebuilds nothing are stale: every reader (scatter, cell
-    // sort, gather, copy-back, topology remap) skips such replicas
-    if (!rebuild[r]) return;
-    const RepInfo R = reps[r];
-    const int c = R.cell0 + xyz2bin_dev(pos4[i], R);
-    acell[i] = c;
-    aslot[i] = atomicAdd(&ccount[c], 1);
+                      const int32_t *__restrict__ any, int32_t *__restrict__ ccount,
+                      int32_t *__restrict__ acell, int32_t *__restrict__ aslot) {
+    if (!*any) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int r = rep[i];
+        // perm and tt of a replica that rebuilds nothing are stale: every reader (scatter,
+        // cell sort, gather, copy-back, topology remap) skips such replicas
+        if (!rebuild[r]) continue;
+        const RepInfo R = reps[r];
+        const int c = R.cell0 + xyz2bin_dev(pos4[i], R);
+        acell[i] = c;
+        aslot[i] = atomicAdd(&ccount[c], 1);
+    }
 }
 
 // One workgroup per replica: exclusive scan of its cell counts into absolute starts.
@@ -97,71 +107,77 @@ __global__ void k_cell_scan(const RepInfo *__restrict__ reps, const int32_t *__r
 }
 
 __global__ void k_scatter(int32_t n, const int32_t *__restrict__ rep,
-                          const int32_t *__restrict__ rebuild, const int32_t *__restrict__ acell,
-                          const int32_t *__restrict__ aslot, const int32_t *__restrict__ cstart,
-                          int32_t *__restrict__ sorted_ids) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !rebuild[rep[i]]) return;
-    sorted_ids[cstart[acell[i]] + aslot[i]] = i;
+                          const int32_t *__restrict__ rebuild, const int32_t *__restrict__ any,
+                          const int32_t *__restrict__ acell, const int32_t *__restrict__ aslot,
+                          const int32_t *__restrict__ cstart, int32_t *__restrict__ sorted_ids) {
+    if (!*any) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (rebuild[rep[i]]) sorted_ids[cstart[acell[i]] + aslot[i]] = i;
 }
 
-// One workgroup per cell: order the cell's atoms by descending old index (the chain
-// order of the reference's linked cells) by ranking, which is deterministic.
-__global__ void k_cell_sort(const int32_t *__restrict__ cell_rep, const int32_t *__restrict__ rebuild,
+// One workgroup per cell at a time: order the cell's atoms by descending old index (the
+// chain order of the reference's linked cells) by ranking, which is deterministic.
+__global__ void k_cell_sort(int32_t ncell, const int32_t *__restrict__ cell_rep,
+                            const int32_t *__restrict__ rebuild, const int32_t *__restrict__ any,
                             const int32_t *__restrict__ ccount, const int32_t *__restrict__ cstart,
                             const int32_t *__restrict__ sorted_ids, int32_t *__restrict__ perm,
                             int32_t *__restrict__ tt, int32_t *__restrict__ atombin) {
-    const int c = blockIdx.x;
-    if (!rebuild[cell_rep[c]]) return;
-    const int n = ccount[c], base = cstart[c];
+    if (!*any) return;
     constexpr int CAP = 2048;
     __shared__ int32_t ids[CAP];
-    const bool staged = n <= CAP;
-    if (staged)
-        for (int e = threadIdx.x; e < n; e += blockDim.x) ids[e] = sorted_ids[base + e];
-    __syncthreads();
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const int me = staged ? ids[e] : sorted_ids[base + e];
-        int rank = 0;
-        for (int f = 0; f < n; ++f) {
-            const int other = staged ? ids[f] : sorted_ids[base + f];
-            rank += other > me;
+    for (int c = blockIdx.x; c < ncell; c += gridDim.x) {   // uniform over the workgroup
+        if (!rebuild[cell_rep[c]]) continue;
+        const int n = ccount[c], base = cstart[c];
+        const bool staged = n <= CAP;
+        if (staged)
+            for (int e = threadIdx.x; e < n; e += blockDim.x) ids[e] = sorted_ids[base + e];
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int me = staged ? ids[e] : sorted_ids[base + e];
+            int rank = 0;
+            for (int f = 0; f < n; ++f) {
+                const int other = staged ? ids[f] : sorted_ids[base + f];
+                rank += other > me;
+            }
+            perm[base + rank] = me;
+            tt[me] = base + rank;
+            atombin[base + rank] = c;
         }
-        perm[base + rank] = me;
-        tt[me] = base + rank;
-        atombin[base + rank] = c;
+        __syncthreads();   // ids is refilled for the next cell
     }
 }
 
 // Gather every permuted per-atom array from `src` into `dst` (flagged replicas only),
 // remapping specbond entries through tt; FP64 engine state too when `engine`.
 __global__ void k_gather(int32_t n, const int32_t *__restrict__ rep,
-                         const int32_t *__restrict__ rebuild, const int32_t *__restrict__ perm,
-                         const int32_t *__restrict__ tt, AtomArrays src, AtomArrays dst,
-                         int engine) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || !rebuild[rep[k]]) return;
-    const int o = perm[k];
-    dst.pos4[k] = src.pos4[o];
-    dst.q[k] = src.q[o];
-    dst.gid[k] = src.gid[o];
-    dst.num1[k] = src.num1[o];
-    dst.num2[k] = src.num2[o];
-    dst.num3[k] = src.num3[o];
-    dst.mol[k] = src.mol[o];
-    dst.wclus[k] = src.wclus[o];
+                         const int32_t *__restrict__ rebuild, const int32_t *__restrict__ any,
+                         const int32_t *__restrict__ perm, const int32_t *__restrict__ tt,
+                         AtomArrays src, AtomArrays dst, int engine) {
+    if (!*any) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        if (!rebuild[rep[k]]) continue;
+        const int o = perm[k];
+        dst.pos4[k] = src.pos4[o];
+        dst.q[k] = src.q[o];
+        dst.gid[k] = src.gid[o];
+        dst.num1[k] = src.num1[o];
+        dst.num2[k] = src.num2[o];
+        dst.num3[k] = src.num3[o];
+        dst.mol[k] = src.mol[o];
+        dst.wclus[k] = src.wclus[o];
 #pragma unroll
-    for (int l = 0; l < SPEC; ++l) {
-        const int s = src.spec[(size_t)o * SPEC + l];
-        dst.spec[(size_t)k * SPEC + l] = (s >= 0) ? tt[s] : -1;
-    }
+        for (int l = 0; l < SPEC; ++l) {
+            const int s = src.spec[(size_t)o * SPEC + l];
+            dst.spec[(size_t)k * SPEC + l] = (s >= 0) ? tt[s] : -1;
+        }
 #pragma unroll
-    for (int d = 0; d < 3; ++d) dst.v32[3 * (size_t)k + d] = src.v32[3 * (size_t)o + d];
-    if (engine) {
+        for (int d = 0; d < 3; ++d) dst.v32[3 * (size_t)k + d] = src.v32[3 * (size_t)o + d];
+        if (engine) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            dst.xd[(size_t)d * n + k] = src.xd[(size_t)d * n + o];
-            dst.vd[(size_t)d * n + k] = src.vd[(size_t)d * n + o];
+            for (int d = 0; d < 3; ++d) {
+                dst.xd[(size_t)d * n + k] = src.xd[(size_t)d * n + o];
+                dst.vd[(size_t)d * n + k] = src.vd[(size_t)d * n + o];
+            }
         }
     }
 }
@@ -169,31 +185,34 @@ __global__ void k_gather(int32_t n, const int32_t *__restrict__ rep,
 // Copy the gathered arrays back (flagged replicas only) so device pointers stay fixed,
 // which keeps captured graphs valid; zero the displacement accumulators.
 __global__ void k_copyback(int32_t n, const int32_t *__restrict__ rep,
-                           const int32_t *__restrict__ rebuild, AtomArrays src, AtomArrays dst,
-                           float *__restrict__ dr32, double *__restrict__ drd, int engine) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || !rebuild[rep[k]]) return;
-    dst.pos4[k] = src.pos4[k];
-    dst.q[k] = src.q[k];
-    dst.gid[k] = src.gid[k];
-    dst.num1[k] = src.num1[k];
-    dst.num2[k] = src.num2[k];
-    dst.num3[k] = src.num3[k];
-    dst.mol[k] = src.mol[k];
-    dst.wclus[k] = src.wclus[k];
+                           const int32_t *__restrict__ rebuild, const int32_t *__restrict__ any,
+                           AtomArrays src, AtomArrays dst, float *__restrict__ dr32,
+                           double *__restrict__ drd, int engine) {
+    if (!*any) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        if (!rebuild[rep[k]]) continue;
+        dst.pos4[k] = src.pos4[k];
+        dst.q[k] = src.q[k];
+        dst.gid[k] = src.gid[k];
+        dst.num1[k] = src.num1[k];
+        dst.num2[k] = src.num2[k];
+        dst.num3[k] = src.num3[k];
+        dst.mol[k] = src.mol[k];
+        dst.wclus[k] = src.wclus[k];
 #pragma unroll
-    for (int l = 0; l < SPEC; ++l) dst.spec[(size_t)k * SPEC + l] = src.spec[(size_t)k * SPEC + l];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        dst.v32[3 * (size_t)k + d] = src.v32[3 * (size_t)k + d];
-        dr32[(size_t)d * n + k] = 0.0f;
-    }
-    if (engine) {
+        for (int l = 0; l < SPEC; ++l) dst.spec[(size_t)k * SPEC + l] = src.spec[(size_t)k * SPEC + l];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            dst.xd[(size_t)d * n + k] = src.xd[(size_t)d * n + k];
-            dst.vd[(size_t)d * n + k] = src.vd[(size_t)d * n + k];
-            drd[(size_t)d * n + k] = 0.0;
+            dst.v32[3 * (size_t)k + d] = src.v32[3 * (size_t)k + d];
+            dr32[(size_t)d * n + k] = 0.0f;
+        }
+        if (engine) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                dst.xd[(size_t)d * n + k] = src.xd[(size_t)d * n + k];
+                dst.vd[(size_t)d * n + k] = src.vd[(size_t)d * n + k];
+                drd[(size_t)d * n + k] = 0.0;
+            }
         }
     }
 }
@@ -209,18 +228,20 @@ struct RemapSet {
 };
 
 __global__ void k_remap_all(RemapSet S, const int32_t *__restrict__ tt, const int32_t *__restrict__ rep,
-                            const int32_t *__restrict__ rebuild) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (S.nlists == 0 || t >= S.end[S.nlists - 1]) return;
-    int L = 0;
-    while (t >= S.end[L]) ++L;
-    const int m = t - (L ? S.end[L - 1] : 0);
-    int32_t *item = S.lst[L] + (size_t)m * S.width[L];
-    const int a0 = item[0];
-    if (a0 < 0 || !rebuild[rep[a0]]) return;
-    for (int k = 0; k < S.nidx[L]; ++k) {
-        const int v = item[k];
-        if (v >= 0) item[k] = tt[v];
+                            const int32_t *__restrict__ rebuild, const int32_t *__restrict__ any) {
+    if (!*any || S.nlists == 0) return;
+    const int total = S.end[S.nlists - 1];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        int L = 0;
+        while (t >= S.end[L]) ++L;
+        const int m = t - (L ? S.end[L - 1] : 0);
+        int32_t *item = S.lst[L] + (size_t)m * S.width[L];
+        const int a0 = item[0];
+        if (a0 < 0 || !rebuild[rep[a0]]) continue;
+        for (int k = 0; k < S.nidx[L]; ++k) {
+            const int v = item[k];
+            if (v >= 0) item[k] = tt[v];
+        }
     }
 }
 
@@ -334,28 +355,31 @@ k_build_list(const float4 *__restrict__ pos4, const int32_t *__restrict__ num3,
 }
 
 __global__ void k_check32(int32_t n, const int32_t *__restrict__ rep, const float *__restrict__ dr,
-                          double rvrc2, int32_t *__restrict__ rebuild) {
+                          double rvrc2, int32_t *__restrict__ rebuild, int32_t *__restrict__ any) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float x = dr[i], y = dr[(size_t)n + i], z = dr[2 * (size_t)n + i];
     const double d2 = (double)((x * x + y * y) + z * z);
-    if (d2 >= rvrc2) atomicOr(&rebuild[rep[i]], 1);
+    // the first atom to flag its replica also raises `any`
+    if (d2 >= rvrc2 && atomicOr(&rebuild[rep[i]], 1) == 0) atomicOr(any, 1);
 }
 
 // Engine criterion: per-replica threshold that keeps every pair inside the force cutoff in
 // the list (half the usable skin), so forces do not depend on when the list was built.
 __global__ void k_check64(int32_t n, const int32_t *__restrict__ rep, const double *__restrict__ dr,
-                          const RepInfo *__restrict__ reps, int32_t *__restrict__ rebuild) {
+                          const RepInfo *__restrict__ reps, int32_t *__restrict__ rebuild,
+                          int32_t *__restrict__ any) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int r = rep[i];
     const double x = dr[i], y = dr[(size_t)n + i], z = dr[2 * (size_t)n + i];
-    if ((x * x + y * y) + z * z >= reps[r].chk2) atomicOr(&rebuild[r], 1);
+    if ((x * x + y * y) + z * z >= reps[r].chk2 && atomicOr(&rebuild[r], 1) == 0) atomicOr(any, 1);
 }
 
+// rebuild[0..nrep) and `any` = rebuild[nrep] to value
 __global__ void k_set_flags(int32_t nrep, int32_t *__restrict__ rebuild, int32_t value) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nrep) rebuild[r] = value;
+    if (r <= nrep) rebuild[r] = value;
 }
 
 __global__ void k_finish(int32_t nrep, int32_t *__restrict__ rebuild, int32_t *__restrict__ nbuilds) {
@@ -364,20 +388,25 @@ __global__ void k_finish(int32_t nrep, int32_t *__restrict__ rebuild, int32_t *_
         nbuilds[r] += 1;
         rebuild[r] = 0;
     }
+    if (r == nrep) rebuild[r] = 0;   // any
 }
 
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 }  // namespace
 
+// grid of the looping list-maintenance kernels
+inline int capped(int n, int b) { return std::max(1, std::min(nblk(n, b), GRID_CAP)); }
+inline const int32_t *any_flag(const sofia_ctx *c) { return c->rebuild + c->nrep; }
+
 void launch_cell_reset(sofia_ctx *c, bool) {
-    hipLaunchKernelGGL(k_cell_reset, dim3(nblk(c->ncell_total, 256)), dim3(256), 0, c->stream,
-                       c->ncell_total, c->cell_rep, c->rebuild, c->ccount);
+    hipLaunchKernelGGL(k_cell_reset, dim3(capped(c->ncell_total, 256)), dim3(256), 0, c->stream,
+                       c->ncell_total, c->cell_rep, c->rebuild, any_flag(c), c->ccount);
 }
 
 void launch_bin(sofia_ctx *c) {
-    hipLaunchKernelGGL(k_bin, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->a.pos4,
-                       c->rep, c->reps_d, c->rebuild, c->ccount, c->acell, c->aslot, c->perm, c->tt);
+    hipLaunchKernelGGL(k_bin, dim3(capped(c->N, 256)), dim3(256), 0, c->stream, c->N, c->a.pos4,
+                       c->rep, c->reps_d, c->rebuild, any_flag(c), c->ccount, c->acell, c->aslot);
 }
 
 void launch_cell_scan(sofia_ctx *c) {
@@ -386,17 +415,18 @@ void launch_cell_scan(sofia_ctx *c) {
 }
 
 void launch_scatter_sort(sofia_ctx *c) {
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
-                       c->rebuild, c->acell, c->aslot, c->cstart, c->sorted_ids);
-    hipLaunchKernelGGL(k_cell_sort, dim3(c->ncell_total), dim3(256), 0, c->stream, c->cell_rep,
-                       c->rebuild, c->ccount, c->cstart, c->sorted_ids, c->perm, c->tt, c->atombin);
+    hipLaunchKernelGGL(k_scatter, dim3(capped(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
+                       c->rebuild, any_flag(c), c->acell, c->aslot, c->cstart, c->sorted_ids);
+    hipLaunchKernelGGL(k_cell_sort, dim3(capped(c->ncell_total, 1)), dim3(256), 0, c->stream,
+                       c->ncell_total, c->cell_rep, c->rebuild, any_flag(c), c->ccount, c->cstart,
+                       c->sorted_ids, c->perm, c->tt, c->atombin);
 }
 
 void launch_gather(sofia_ctx *c, bool engine) {
-    hipLaunchKernelGGL(k_gather, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
-                       c->rebuild, c->perm, c->tt, c->a, c->b, engine ? 1 : 0);
-    hipLaunchKernelGGL(k_copyback, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
-                       c->rebuild, c->b, c->a, c->dr32, c->drd, engine ? 1 : 0);
+    hipLaunchKernelGGL(k_gather, dim3(capped(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
+                       c->rebuild, any_flag(c), c->perm, c->tt, c->a, c->b, engine ? 1 : 0);
+    hipLaunchKernelGGL(k_copyback, dim3(capped(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
+                       c->rebuild, any_flag(c), c->b, c->a, c->dr32, c->drd, engine ? 1 : 0);
 }
 
 void launch_remap_topology(sofia_ctx *c) {
@@ -420,8 +450,8 @@ void launch_remap_topology(sofia_ctx *c) {
     add(c->ebond, c->nbe, 3, 2);
     add(c->eangle, c->nae, 4, 3);
     if (total)
-        hipLaunchKernelGGL(k_remap_all, dim3(nblk(total, 256)), dim3(256), 0, c->stream, S, c->tt,
-                           c->rep, c->rebuild);
+        hipLaunchKernelGGL(k_remap_all, dim3(capped(total, 256)), dim3(256), 0, c->stream, S, c->tt,
+                           c->rep, c->rebuild, any_flag(c));
 }
 
 void launch_build_list(sofia_ctx *c) {
@@ -443,19 +473,19 @@ void launch_build_list(sofia_ctx *c) {
 void launch_check_neighbor(sofia_ctx *c, bool engine) {
     if (engine)
         hipLaunchKernelGGL(k_check64, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
-                           c->drd, c->reps_d, c->rebuild);
+                           c->drd, c->reps_d, c->rebuild, c->rebuild + c->nrep);
     else
         hipLaunchKernelGGL(k_check32, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->rep,
-                           c->dr32, c->dp.rvrc2, c->rebuild);
+                           c->dr32, c->dp.rvrc2, c->rebuild, c->rebuild + c->nrep);
 }
 
 void launch_set_rebuild(sofia_ctx *c, int32_t value) {
-    hipLaunchKernelGGL(k_set_flags, dim3(nblk(c->nrep, 256)), dim3(256), 0, c->stream, c->nrep,
+    hipLaunchKernelGGL(k_set_flags, dim3(nblk(c->nrep + 1, 256)), dim3(256), 0, c->stream, c->nrep,
                        c->rebuild, value);
 }
 
 void launch_finish_rebuild(sofia_ctx *c, bool) {
-    hipLaunchKernelGGL(k_finish, dim3(nblk(c->nrep, 256)), dim3(256), 0, c->stream, c->nrep,
+    hipLaunchKernelGGL(k_finish, dim3(nblk(c->nrep + 1, 256)), dim3(256), 0, c->stream, c->nrep,
                        c->rebuild, c->nbuilds);
 }
 

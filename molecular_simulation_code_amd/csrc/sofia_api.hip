@@ -437,7 +437,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     TRY(dalloc(&c->sorted_ids, n));
     TRY(dalloc(&c->numneigh, n));
     TRY(dalloc(&c->maxcnt, 1));
-    TRY(dalloc(&c->rebuild, nrep));
+    TRY(dalloc(&c->rebuild, nrep + 1));   // nrep flags, then `any`
     TRY(dalloc(&c->nbuilds, nrep));
     TRY(dalloc(&c->shake_err, 1));
     TRY(dalloc(&c->shake_iters, 1));
@@ -448,7 +448,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     SOFIA_HIP(hipMemset(c->drd, 0, sizeof(double) * 3 * (size_t)n));
     SOFIA_HIP(hipMemset(c->fix, 0, sizeof(long long) * 3 * (size_t)n));
     SOFIA_HIP(hipMemset(c->fnb, 0, sizeof(double) * 3 * (size_t)n));
-    SOFIA_HIP(hipMemset(c->rebuild, 0, sizeof(int32_t) * nrep));
+    SOFIA_HIP(hipMemset(c->rebuild, 0, sizeof(int32_t) * (nrep + 1)));
     SOFIA_HIP(hipMemset(c->nbuilds, 0, sizeof(int32_t) * nrep));
     SOFIA_HIP(hipMemset(c->numneigh, 0, sizeof(int32_t) * n));
     SOFIA_HIP(hipMemset(c->step_d, 0, sizeof(uint64_t) * nrep));
