@@ -399,6 +399,10 @@ def profile_run(args, workers):
         ms_copy = eng.profile_copy(1 << 30, 5)
         log(f"[profile-run] {sum(r.np for r in reps)} atoms, tuning {tuning}; pair kernel "
             f"{ms:.4f} ms/launch, {pairs} entries in cutoff; copy 1 GiB {ms_copy:.4f} ms/launch")
+        # the last 5 list-build launches of the run rebuild every replica
+        full = eng.profile_step(5, full_rebuild=True)
+        log(f"[profile-run] full rebuild of every replica: list build {full['list build']:.4f} ms, "
+            f"sort+gather {full['sort+gather']:.4f} ms")
     finally:
         eng.close()
 

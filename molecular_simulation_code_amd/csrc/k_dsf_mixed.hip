@@ -30,8 +30,8 @@
 // the fusing to the compiler and a pair rounds the same way in both kernels of this file.
 //
 // Loads.  The list row, the position records and the charges are read with raw buffer
-// loads (32-bit offsets; an offset past the end of a buffer reads zero), so the software
-// pipeline can prefetch past the end of a row without clamping its index.
+// loads (32-bit offsets from a wave-uniform descriptor; an offset past the end of a buffer
+// reads zero).
 #include <type_traits>
 
 #include "sofia_device.h"
@@ -194,15 +194,12 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             double *__restrict__ fnb, float *__restrict__ ff32, long long *__restrict__ eacc) {
     // QT: charges by atom type; 12-byte records (x, y, z) and a pair table (lj1, lj2,
     // C q_i q_j) in place of the charge in the record
-    // s_dyn[0] is a zero pad ahead of the type-pair tables: a slot whose record lies past
-    // the end of the position buffer reads type 0, i.e. element -1 of its row
     extern __shared__ float4 s_dyn[];
     const int nt2 = P.ntypes * P.ntypes;
     const int tn = P.tab_n;
-    float4 *s_lj = s_dyn + 1;
-    float4 *s_pt = s_lj + nt2;
-    float4 *s_tab = s_lj + (QT ? 2 : 1) * nt2;
-    if (threadIdx.x == 0) s_dyn[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 *s_lj = s_dyn;
+    float4 *s_pt = s_dyn + nt2;
+    float4 *s_tab = s_dyn + (QT ? 2 : 1) * nt2;
     for (int t = threadIdx.x; t < nt2; t += blockDim.x) {
         s_lj[t] = ljtab[t];
         if (QT) s_pt[t] = ptab[t];
@@ -252,9 +249,9 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     // consumed, so its next gather (4 entries ahead) and list index (8 entries ahead) are
     // in flight for the other three slots and no register is moved while a load is
     // outstanding; scheduling barriers keep the compiler from hoisting the reloads.
-    // Entries past the end of the row are computed on whatever the list buffer holds there
-    // (stale indices of an earlier build, or a record past the end of the position buffer,
-    // which reads zero) and then dropped: no index is clamped in the loop.
+    // Entries past the end of the row gather the atom's own record (a cache hit, and a real
+    // atom type) and are then dropped; list reads past the row re-read its first entry
+    // instead of pulling the unused tail of the row from memory.
     const uint32_t step = 4u * L;
     const uint32_t row0 = (uint32_t)(ii - iw) * (uint32_t)na * 4u;
     uint32_t off = row0 + (uint32_t)sub * 4u;
@@ -267,7 +264,8 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        PJ[u] = QT ? ld96(prs, rec_off<QT>(JJ[u])) : ld128(prs, rec_off<QT>(JJ[u]));
+        const uint32_t j0 = (sub + u * L < nn) ? JJ[u] : (uint32_t)ii;
+        PJ[u] = QT ? ld96(prs, rec_off<QT>(j0)) : ld128(prs, rec_off<QT>(j0));
         JJ[u] = ld32(lrs, off + (u + 4) * step);
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -286,7 +284,11 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
         const PairOut o = pair_dsf<ENERGY, DIR>(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2, D);
         const bool take = valid && (DIR ? o.below : !o.below);
         if (!DIR) lowq = lowq || (valid && o.below);
-        const float fs = take ? o.force * fq : 0.0f;
+        // an empty asm pins the force as computed on every lane, so the compiler selects
+        // instead of branching around the pair math (a branch per entry costs more)
+        float force = o.force;
+        __asm__("" : "+v"(force));
+        const float fs = take ? force * fq : 0.0f;
         fxq += (double)__builtin_truncf(dx * fs);
         fyq += (double)__builtin_truncf(dy * fs);
         fzq += (double)__builtin_truncf(dz * fs);
@@ -308,8 +310,9 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             entry(PJ[u], k + u * L < nn, std::false_type{});
-            PJ[u] = QT ? ld96(prs, rec_off<QT>(JJ[u])) : ld128(prs, rec_off<QT>(JJ[u]));
-            JJ[u] = ld32(lrs, off + u * step);
+            const uint32_t jg = (k + (u + 4) * L < nn) ? JJ[u] : (uint32_t)ii;
+            PJ[u] = QT ? ld96(prs, rec_off<QT>(jg)) : ld128(prs, rec_off<QT>(jg));
+            JJ[u] = ld32(lrs, (k + (u + 8) * L < nn) ? off + u * step : row0);
             // VALU, SALU and LDS work may cross; memory loads stay in slot order
             __builtin_amdgcn_sched_barrier(0x0086);
         }
@@ -501,7 +504,7 @@ void launch_lanes(sofia_ctx *c, bool energies, float *ff32, bool qt) {
     const long long threads = (long long)c->N * L;
     const int grid = (int)((threads + block - 1) / block);
     const int nt2 = c->dp.ntypes * c->dp.ntypes;
-    const size_t lds = sizeof(float4) * (1 + (qt ? 2 : 1) * nt2 + (energies ? 2 : 1) * (c->dp.tab_n + 1));
+    const size_t lds = sizeof(float4) * ((qt ? 2 : 1) * nt2 + (energies ? 2 : 1) * (c->dp.tab_n + 1));
 #define DSF_LAUNCH(E, Q)                                                                          \
     hipLaunchKernelGGL((k_dsf_mixed<L, E, Q>), dim3(grid), dim3(block), lds, c->stream, c->N,       \
                        c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh, c->neigh_alloc,          \

@@ -27,8 +27,10 @@ namespace {
 constexpr int ECAP = 4096;            // staged candidates per pass (64 KiB of LDS)
 constexpr float CULL_MARGIN = 0.01f;  // A
 
+// Two workgroups fit a CU's LDS; the second bound (waves per SIMD) keeps the register
+// budget that lets both run (64 VGPRs at 16 waves per workgroup).
 template <int WAVES>
-__global__ void __launch_bounds__(WAVES * 64)
+__global__ void __launch_bounds__(WAVES * 64, WAVES * 2 / 4)
 k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__restrict__ num3,
                const int32_t *__restrict__ spec, const int32_t *__restrict__ cstart,
                const int32_t *__restrict__ ccount, const int32_t *__restrict__ cell_rep,
@@ -73,7 +75,7 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
         if (tid < 64) {   // exclusive scan of the 128 counts, two per lane
             const int a = s_cnt[2 * tid], b = s_cnt[2 * tid + 1];
             int v = a + b;
-    #pragma unroll
+#pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const int u = __shfl_up(v, o, 64);
                 if (tid >= o) v += u;
@@ -86,8 +88,8 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
         __syncthreads();
         const int total = s_off[125];
 
-        const int wave = tid >> 6, lane = tid & 63;
-        const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        // the wave index is uniform: the atoms a wave takes, and their exclusions, stay scalar
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
         const int i0 = cstart[c], ni = ccount[c];
         const float rvm = rv + CULL_MARGIN, rvm2 = rvm * rvm;
         const float irn = 1.0f / rn;
@@ -118,15 +120,15 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
                 const int i = i0 + ii;
                 const float4 xi = pos4[i];
                 const int n3 = num3[i];
+                // the atom's exclusions (wave-uniform: scalar loads, held in scalar registers)
                 int ex[SPEC];
                 int exmin = 0x7fffffff, exmax = -1;
-    #pragma unroll
+#pragma unroll
                 for (int l = 0; l < SPEC; ++l) {
-                    ex[l] = (l < n3) ? spec[(size_t)i * SPEC + l] : -1;
-                    if (l < n3) {
-                        exmin = min(exmin, ex[l]);
-                        exmax = max(exmax, ex[l]);
-                    }
+                    const int v = spec[(size_t)i * SPEC + l];
+                    ex[l] = (l < n3) ? v : -1;
+                    exmin = (l < n3) ? min(exmin, v) : exmin;
+                    exmax = (l < n3) ? max(exmax, v) : exmax;
                 }
                 // row bounds: lane k < 25 takes row (dy, dz) = (k % 5 - 2, k / 5 - 2)
                 int rlo = 0, rhi = 0;
@@ -146,22 +148,22 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
                 uint64_t rows = __ballot(rhi > rlo);
                 int cnt = (g0 == 0) ? 0 : numneigh[i];
                 int32_t *out = nlist + (size_t)i * na;
-                // test one staged candidate per lane; hits appended in staged order
+                // test one staged candidate per lane; hits appended in staged order.  The tests
+                // combine without short-circuit (no branch per condition); the exclusion
+                // compares run only for a chunk where some lane falls in [exmin, exmax].
                 auto sweep = [&](const float4 pj, const bool valid) {
                     const int z = __float_as_int(pj.w);
                     const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
-                    bool pred = valid && (dx * dx + dy * dy + dz * dz < rv2) && (z != i);
-                    const bool maybe = pred && z >= exmin && z <= exmax;
-                    if (__any(maybe)) {
-                        if (maybe) {
-    #pragma unroll
-                            for (int l = 0; l < SPEC; ++l)
-                                if (ex[l] == z) pred = false;
-                        }
+                    bool pred = valid & (dx * dx + dy * dy + dz * dz < rv2) & (z != i);
+                    if (__ballot(pred & (z >= exmin) & (z <= exmax))) {
+#pragma unroll
+                        for (int l = 0; l < SPEC; ++l) pred = pred & (ex[l] != z);
                     }
                     const uint64_t mask = __ballot(pred);
-                    const int pos = cnt + __popcll(mask & lt_mask);
-                    if (pred && pos < na) out[pos] = z;
+                    const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi(
+                                              (uint32_t)(mask >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                    if (pred & (pos < na)) out[pos] = z;
                     cnt += __popcll(mask);
                 };
                 while (rows) {

@@ -539,9 +539,9 @@ class Engine:
     def autotune(self, steps: int = 80, skins=(1.0, 1.3, 1.6, 2.0, 2.5), lanes=(0, 8, 16, 32),
                  waves=(4, 8, 16), graph_steps=(1, 20), cache: bool = True):
         """Pick the Verlet skin and the pair kernel's lanes per atom on the current state.
-        For each skin the lanes are chosen by timing the pair kernel alone; skins are
-        compared on whole MD steps (list length against rebuild rate); then the list build's
-        waves per workgroup on full rebuilds, the bonded evaluation (per-kind kernels or
+        First the list build's waves per workgroup on full rebuilds; then, for each skin, the
+        lanes by timing the pair kernel alone, and the skins compared on whole MD steps (list
+        length against rebuild rate); then the bonded evaluation (per-kind kernels or
         molecule bundles) on eager steps, and the MD steps per captured graph on whole
         steps.  The knobs change speed only.  Results are cached per bucket (tune_key: size
         and density on a coarse scale) for the process and, when SOFIA_TUNE_CACHE names a
@@ -560,15 +560,24 @@ class Engine:
         import time
         best = None
         self.tune_log = {"pair_ms": {}, "step_ms": {}, "list_ms": {}}
+        # the list build's waves first (full rebuilds at the middle skin), so that the skins
+        # are compared with the rebuild cost they will run with
+        self.set_skin(skins[len(skins) // 2])
+        t_w = {}
+        for w in waves:
+            self.set_tuning(16, 256, w)
+            t_w[w] = self.profile_step(3, full_rebuild=True)["list build"]
+        self.tune_log["list_ms"] = t_w
+        w = min(t_w, key=t_w.get)
         for sk in skins:
             self.set_skin(sk)
             t_l = {}
             for L in lanes:
-                self.set_tuning(L, 256, 4)
+                self.set_tuning(L, 256, w)
                 t_l[L] = self.profile_nonbond(8)[0]
             self.tune_log["pair_ms"][sk] = t_l
             L = min(t_l, key=t_l.get)
-            self.set_tuning(L, 256, 4)
+            self.set_tuning(L, 256, w)
             self.run(5)
             self.synchronize()
             t0 = time.perf_counter()
@@ -579,13 +588,6 @@ class Engine:
             if best is None or ms < best[2]:
                 best = (sk, L, ms)
         self.set_skin(best[0])
-        self.set_tuning(best[1], 256, 4)
-        t_w = {}
-        for w in waves:
-            self.set_tuning(best[1], 256, w)
-            t_w[w] = self.profile_step(3, full_rebuild=True)["list build"]
-        self.tune_log["list_ms"] = t_w
-        w = min(t_w, key=t_w.get)
         self.set_tuning(best[1], 256, w)
         t_b = {}
         for mode in (0, 1):
