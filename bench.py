@@ -175,44 +175,34 @@ def dropin_rate(system, state, steps, warmup):
     replaced by its C-ABI drop-in on host-owned real*4 arrays, as a Fortran host would call
     them through the ISO_C_BINDING shim: nve_v, nve_x, pbc, check_neighbor (and
     neighbor_wrapper when triggered), the FP64 DSF pair kernel, the 1-4 pair, the bonded
-    terms, shake, nve_v.  Each call copies its arrays over PCIe, as the reference's MIC
-    offload did.  Returns (atom-steps/s, list builds during the timed steps)."""
-    from molecular_simulation_code_amd.host import SofiaGPU
+    terms, shake, nve_v.  Each routine copies the arrays it reads in and the arrays it writes
+    out over PCIe (host.DropinStep, page-locked arrays), as the reference's MIC offload
+    copied position in and ff out per call.  Returns (atom-steps/s, list builds during the
+    timed steps, MB moved over the host link per step)."""
+    from molecular_simulation_code_amd.host import DropinStep, SofiaGPU
     s = system.copy()
     s.pos = wrapped_f32(state[0], s.box)
     s.v = np.ascontiguousarray(state[1], np.float32)
     g = SofiaGPU(s)
-    dthalf = float(s.derived()["dthalf"])
+    d = None
     try:
-        def forces():
-            g.potential = g.e_coul = g.e_coul_14 = g.pot_14 = 0.0
-            g.e_bond = g.e_angle = g.e_dihed = g.e_improper = 0.0
-            f = g.lj_cut_coul_dsf_nonewton(0).astype(np.float64)
-            f += g.pair14_dsf()
-            f += g.bonded_forces()
-            g.ff = f.astype(np.float32)          # the ff + ffh merge
-            g.shake(1)
-
-        def step():
-            g.nve_v(dthalf)
-            g.nve_x()
-            g.pbc()
-            if g.check_neighbor():
-                g.neighbor_wrapper()
-            forces()
-            g.nve_v(dthalf)
-
         g.neighbor_wrapper(0)
-        forces()
+        d = DropinStep(g)
+        d.forces()
         for _ in range(max(1, warmup)):
-            step()
-        b0 = g.nn_builds
+            d.step()
+        builds = 0
         t0 = time.perf_counter()
         for _ in range(steps):
-            step()
+            builds += d.step()
         el = time.perf_counter() - t0
-        return s.np * steps / el, g.nn_builds - b0
+        # per step without a rebuild: nve_v 7 arrays-of-3 (v, ff in; v out) twice, nve_x, pbc,
+        # check, three force routines and shake, in units of np floats
+        floats = 2 * 9 + (4 + 3 + 3 + 4 + 3) + (4 + 4) + 3 + 3 * (4 + 3) + (4 + 3 + 3 + 3)
+        return s.np * steps / el, builds, floats * 4 * s.np / 1e6
     finally:
+        if d is not None:
+            d.close()
         g.close()
 
 
@@ -502,9 +492,9 @@ def main():
 
     # ---------------------------------------------------------------- plain drop-in: C2 through the C ABI
     kd = max(10, min(K, 100))
-    dropin, dropin_builds = dropin_rate(plain_sys, state_c2, kd, max(1, W // 2))
+    dropin, dropin_builds, dropin_mb = dropin_rate(plain_sys, state_c2, kd, max(1, W // 2))
     log(f"[plain drop-in] {dropin / 1e6:.1f} Matom-steps/s ({kd} steps, {dropin_builds} list builds, "
-        "host arrays, PCIe copies per call)")
+        f"host arrays, {dropin_mb:.0f} MB over the host link per step)")
 
     # ---------------------------------------------------------------- batched: 64 replicas mixed
     natoms = sum(r.np for r in reps)
@@ -648,9 +638,11 @@ def main():
         # the plain GPU drop-in (C2 through the C ABI, host arrays) over the CPU port on C2
         "plain_dropin_c2": {"value": round(dropin / 1e6, 3), "unit": "Matom-steps/s",
                             "steps": kd, "list_builds": int(dropin_builds),
+                            "host_link_MB_per_step": round(dropin_mb, 1),
                             "workload": "204,000-atom glycine/water, FP64: the reference's DSF step "
                                         "with each routine replaced by its C-ABI drop-in on host "
-                                        "real*4 arrays (PCIe copies per call included)"},
+                                        "real*4 arrays in page-locked memory, each routine copying "
+                                        "in what it reads and out what it writes (PCIe included)"},
         "plain_over_cpu": None if cpu is None else round(dropin / 1e6 / cpu["value"], 1),
         "plain_engine_c2_over_cpu": None if cpu is None else round(plain_rate / 1e6 / cpu["value"], 1),
         "cpu_baseline_c1": cpu_c1,

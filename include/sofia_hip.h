@@ -139,6 +139,13 @@ int sofia_put_state_f64(sofia_ctx *ctx, const double *x, const double *v);
 /* drx/dry/drz displacement accumulators since the last list build */
 int sofia_get_displacement(sofia_ctx *ctx, float *drx, float *dry, float *drz);
 int sofia_put_displacement(sofia_ctx *ctx, const float *drx, const float *dry, const float *drz);
+/* Page-locked host memory for the arrays a host hands to the drop-ins (position, v, ff,
+ * drx/dry/drz): transfers from and to it run at the full host-link rate instead of through
+ * the runtime's staging of pageable memory.  No counterpart in the reference, whose MIC
+ * offload managed its own transfer buffers (Source/mod_force.f90:446-461).  *out is NULL
+ * on failure; free with sofia_host_free. */
+int sofia_host_alloc(size_t bytes, void **out);
+int sofia_host_free(void *p);
 
 /* ---------------------------------------------------------------- (1) drop-ins */
 /* neighbor_wrapper(step), Source/mod_neighbor.f90:8-86: check_allocation, new_nlist,

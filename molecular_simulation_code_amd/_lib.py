@@ -73,6 +73,8 @@ SIGNATURES = {
     "sofia_put_state_f64": [_vp] * 3,
     "sofia_get_displacement": [_vp] * 4,
     "sofia_put_displacement": [_vp] * 4,
+    "sofia_host_alloc": [C.c_size_t, _vp],
+    "sofia_host_free": [_vp],
     "sofia_neighbor_wrapper": [_vp, _i32, _vp],
     "sofia_get_cells": [_vp] * 7,
     "sofia_get_nlist": [_vp] * 4,

@@ -97,22 +97,32 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
         for (int g0 = 0; g0 < total || g0 == 0; g0 += ECAP) {
             const int g1 = min(total, g0 + ECAP);
             if (g0 > 0) __syncthreads();   // the previous pass is done with s_j
-            for (int t = tid; t < g1 - g0; t += WAVES * 64) {
-                const int f = g0 + t;
-                int lo = 0, hi = 125;      // last stencil cell k with s_off[k] <= f
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_off[mid] <= f) lo = mid; else hi = mid;
+            // a wave stages whole (dy, dz) rows: the row's 5 cells are found with 4 compares
+            // against its (uniform) cell offsets, and the y / z image shifts are uniform
+            for (int row = wave; row < 25; row += WAVES) {
+                const int k0 = 5 * row;
+                const int r0 = max(s_off[k0], g0), r1 = min(s_off[k0 + 5], g1);
+                if (r0 >= r1) continue;
+                const int ay = cy + row % 5 - 2, az = cz + row / 5 - 2;
+                const float sy = (ay < 0) ? -box : ((ay >= n) ? box : 0.0f);
+                const float sz = (az < 0) ? -box : ((az >= n) ? box : 0.0f);
+                int b[4], d[5];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    if (q > 0) b[q - 1] = s_off[k0 + q];
+                    d[q] = s_src[k0 + q] - s_off[k0 + q];   // staged index -> atom id
                 }
-                const int k = lo;
-                const int ax = cx + k % 5 - 2, ay = cy + (k / 5) % 5 - 2, az = cz + k / 25 - 2;
-                const int id = s_src[k] + (f - s_off[k]);
-                float4 p = pos4[id];
-                p.x += (ax < 0) ? -box : ((ax >= n) ? box : 0.0f);
-                p.y += (ay < 0) ? -box : ((ay >= n) ? box : 0.0f);
-                p.z += (az < 0) ? -box : ((az >= n) ? box : 0.0f);
-                p.w = __int_as_float(id);
-                s_j[t] = p;
+                for (int f = r0 + lane; f < r1; f += 64) {
+                    const int dk = (int)(f >= b[0]) + (int)(f >= b[1]) + (int)(f >= b[2]) + (int)(f >= b[3]);
+                    const int id = f + (dk == 0 ? d[0] : dk == 1 ? d[1] : dk == 2 ? d[2] : dk == 3 ? d[3] : d[4]);
+                    const int ax = cx + dk - 2;
+                    float4 p = pos4[id];
+                    p.x += (ax < 0) ? -box : ((ax >= n) ? box : 0.0f);
+                    p.y += sy;
+                    p.z += sz;
+                    p.w = __int_as_float(id);
+                    s_j[f - g0] = p;
+                }
             }
             __syncthreads();
 
@@ -150,7 +160,8 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
                 int32_t *out = nlist + (size_t)i * na;
                 // test one staged candidate per lane; hits appended in staged order.  The tests
                 // combine without short-circuit (no branch per condition); the exclusion
-                // compares run only for a chunk where some lane falls in [exmin, exmax].
+                // compares run only for a chunk where some lane falls in [exmin, exmax]
+                // (measured cheaper than compares bounded by the atom's exclusion count).
                 auto sweep = [&](const float4 pj, const bool valid) {
                     const int z = __float_as_int(pj.w);
                     const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;

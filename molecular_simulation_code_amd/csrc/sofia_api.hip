@@ -1046,6 +1046,21 @@ int sofia_put_displacement(sofia_ctx *c, const float *dx, const float *dy, const
     return SOFIA_OK;
 }
 
+int sofia_host_alloc(size_t bytes, void **out) {
+    if (!out) return fail(SOFIA_E_ARG, "null argument");
+    *out = nullptr;
+    if (bytes == 0) return fail(SOFIA_E_ARG, "zero bytes");
+    void *p = nullptr;
+    SOFIA_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    *out = p;
+    return SOFIA_OK;
+}
+
+int sofia_host_free(void *p) {
+    if (p) SOFIA_HIP(hipHostFree(p));
+    return SOFIA_OK;
+}
+
 // ------------------------------------------------------------------ drop-ins
 int sofia_neighbor_wrapper(sofia_ctx *c, int32_t, int32_t *tt) {
     if (!c) return fail(SOFIA_E_ARG, "null context");

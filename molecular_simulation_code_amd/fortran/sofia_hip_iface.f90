@@ -93,6 +93,49 @@ module sofia_hip_iface
        type(c_ptr), value :: ctx
        real(c_float), intent(in) :: dx(*), dy(*), dz(*)
      end function
+     integer(c_int) function sofia_get_displacement(ctx, dx, dy, dz) &
+          bind(c, name="sofia_get_displacement")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(inout) :: dx(*), dy(*), dz(*)
+     end function
+     ! The same two C functions with only some arrays (the others NULL): a wrapper moves
+     ! just the arrays its routine reads (put) or writes (get).
+     integer(c_int) function sofia_put_pos(ctx, pos, q, v, gid) bind(c, name="sofia_put_state")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, q, v, gid
+       type(*), intent(in) :: pos(*)
+     end function
+     integer(c_int) function sofia_put_pos_v(ctx, pos, q, v, gid) bind(c, name="sofia_put_state")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, q, gid
+       type(*), intent(in) :: pos(*), v(*)
+     end function
+     integer(c_int) function sofia_put_v(ctx, pos, q, v, gid) bind(c, name="sofia_put_state")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, pos, q, gid
+       type(*), intent(in) :: v(*)
+     end function
+     integer(c_int) function sofia_get_pos(ctx, pos, v, gid) bind(c, name="sofia_get_state")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, v, gid
+       type(*), intent(inout) :: pos(*)
+     end function
+     integer(c_int) function sofia_get_v(ctx, pos, v, gid) bind(c, name="sofia_get_state")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, pos, gid
+       type(*), intent(inout) :: v(*)
+     end function
+     ! page-locked host memory for position / v / ff (bind with c_f_pointer)
+     integer(c_int) function sofia_host_alloc(bytes, p) bind(c, name="sofia_host_alloc")
+       import :: c_int, c_ptr, c_size_t
+       integer(c_size_t), value :: bytes
+       type(c_ptr), intent(out) :: p
+     end function
+     integer(c_int) function sofia_host_free(p) bind(c, name="sofia_host_free")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: p
+     end function
      integer(c_int) function sofia_neighbor_wrapper(ctx, step, tt) bind(c, name="sofia_neighbor_wrapper")
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx
@@ -330,10 +373,12 @@ contains
     nflag = f
   end subroutine check_neighbor_hip
 
+  ! Each wrapper copies in the arrays its routine reads and copies out the arrays it
+  ! writes; q, global_id and the topology change only in neighbor_wrapper_hip.
   subroutine lj_cut_coul_dsf_nonewton_NOMIC_hip(step)
     integer :: step
     real(c_double) :: vir(3)
-    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_put_pos(ctx, position, c_null_ptr, c_null_ptr, c_null_ptr), 'put_state')
     call sofia_check(sofia_lj_cut_coul_dsf(ctx, ff, c_null_ptr, potential, e_coul, vir), 'dsf')
   end subroutine lj_cut_coul_dsf_nonewton_NOMIC_hip
 
@@ -341,7 +386,7 @@ contains
   ! then wait (offload ... signal(sig) / offload_wait).
   subroutine lj_cut_coul_dsf_nonewton_MIC_signal_hip(step)
     integer :: step
-    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_put_pos(ctx, position, c_null_ptr, c_null_ptr, c_null_ptr), 'put_state')
     call sofia_check(sofia_lj_cut_coul_dsf_async(ctx, 0_c_int32_t), 'dsf_async')
   end subroutine lj_cut_coul_dsf_nonewton_MIC_signal_hip
 
@@ -352,7 +397,7 @@ contains
 
   subroutine lj_cut_coul_long_nonewton_NOMIC_hip
     real(c_double) :: vir(3)
-    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_put_pos(ctx, position, c_null_ptr, c_null_ptr, c_null_ptr), 'put_state')
     call sofia_check(sofia_lj_cut_coul_long(ctx, ff, potential, e_coul, vir), 'coul_long')
     virialx = vir(1); virialy = vir(2); virialz = vir(3)
   end subroutine lj_cut_coul_long_nonewton_NOMIC_hip
@@ -380,23 +425,24 @@ contains
 
   subroutine nve_v_hip(dt_partial)
     real*4 :: dt_partial
-    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_put_v(ctx, c_null_ptr, c_null_ptr, v, c_null_ptr), 'put_state')
     call sofia_check(sofia_nve_v(ctx, dt_partial, ff, ke), 'nve_v')
-    call sofia_check(sofia_get_state(ctx, position, v, global_id), 'get_state')
+    call sofia_check(sofia_get_v(ctx, c_null_ptr, v, c_null_ptr), 'get_state')
   end subroutine nve_v_hip
 
   subroutine nve_x_hip
-    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_put_pos_v(ctx, position, c_null_ptr, v, c_null_ptr), 'put_state')
     call sofia_check(sofia_put_displacement(ctx, drx, dry, drz), 'put_displacement')
     call sofia_check(sofia_nve_x(ctx), 'nve_x')
-    call sofia_check(sofia_get_state(ctx, position, v, global_id), 'get_state')
+    call sofia_check(sofia_get_pos(ctx, position, c_null_ptr, c_null_ptr), 'get_state')
+    call sofia_check(sofia_get_displacement(ctx, drx, dry, drz), 'get_displacement')
   end subroutine nve_x_hip
 
   subroutine pbc_hip(step)
     integer :: step
-    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_put_pos(ctx, position, c_null_ptr, c_null_ptr, c_null_ptr), 'put_state')
     call sofia_check(sofia_pbc(ctx), 'pbc')
-    call sofia_check(sofia_get_state(ctx, position, v, global_id), 'get_state')
+    call sofia_check(sofia_get_pos(ctx, position, c_null_ptr, c_null_ptr), 'get_state')
   end subroutine pbc_hip
 
   subroutine shake_hip(iflag)
@@ -404,7 +450,7 @@ contains
     real(c_double) :: vir(3)
     integer(c_int32_t) :: its
     vir = (/ virialx, virialy, virialz /)
-    call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
+    call sofia_check(sofia_put_pos_v(ctx, position, c_null_ptr, v, c_null_ptr), 'put_state')
     call sofia_check(sofia_shake(ctx, int(iflag, c_int32_t), ff, vir, its), 'shake')
     virialx = vir(1); virialy = vir(2); virialz = vir(3)
   end subroutine shake_hip

@@ -404,6 +404,136 @@ class SofiaGPU:
         return out
 
 
+class PinnedArrays:
+    """numpy arrays in page-locked host memory (sofia_host_alloc); release() frees them,
+    after which the arrays must not be used."""
+
+    def __init__(self):
+        self._bufs = []
+
+    def empty(self, shape, dtype) -> np.ndarray:
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = C.c_void_p()
+        check(lib().sofia_host_alloc(nbytes, C.byref(p)))
+        self._bufs.append(p)
+        return np.frombuffer((C.c_char * nbytes).from_address(p.value), dtype=dtype).reshape(shape)
+
+    def release(self):
+        for p in self._bufs:
+            lib().sofia_host_free(p)
+        self._bufs = []
+
+
+class DropinStep:
+    """The reference's DSF time step (Source/mod_integrate.f90:270-308 with
+    force_wrapper_DSF_NOMIC, Source/mod_force.f90:510-554) driven through the drop-in entry
+    points the way a Fortran host with page-locked arrays would call them: each routine
+    uploads the arrays it reads and downloads the arrays it writes, and nothing else.
+
+    * nve_v: v, ff -> v;  nve_x: position, v, drx/dry/drz -> position, drx/dry/drz;
+    * pbc: position -> position;  check_neighbor: drx/dry/drz;
+    * the DSF pair kernel, the 1-4 pair and the bonded terms: position -> their forces,
+      summed on the host as the reference sums ff, ffh and the bonded forces;
+    * shake(1): position, v, ff -> ff.
+
+    neighbor_wrapper, when check_neighbor asks for it, goes through SofiaGPU (full state in
+    and out, host arrays re-indexed).  The trajectory equals SofiaGPU's step for step."""
+
+    def __init__(self, g: "SofiaGPU"):
+        self.g = g
+        self.mem = PinnedArrays()
+        n = g.np
+        self.pos = self.mem.empty((n, 4), f32)       # type(ainfo): x, y, z, type
+        self.v = self.mem.empty((n, 3), f32)
+        self.ff = self.mem.empty((n, 3), f32)
+        self.ffh = self.mem.empty((n, 3), f32)
+        self.ffb = self.mem.empty((n, 3), f32)
+        self.dr = self.mem.empty((3, n), f32)
+        self.dthalf = float(g.sys.derived()["dthalf"])
+        self._from_g()
+
+    def close(self):
+        self.pos = self.v = self.ff = self.ffh = self.ffb = self.dr = None
+        self.mem.release()
+
+    def _from_g(self):
+        g = self.g
+        self.pos[:, :3] = g.position
+        self.pos[:, 3] = g.type.view(f32)
+        self.v[:] = g.v
+        self.ff[:] = g.ff
+        self.dr[0], self.dr[1], self.dr[2] = g.drx, g.dry, g.drz
+
+    def _to_g(self):
+        g = self.g
+        g.position = self.pos[:, :3].copy()
+        g.type = self.pos[:, 3].copy().view(i32)
+        g.v = self.v.copy()
+        g.ff = self.ff.copy()
+        g.drx, g.dry, g.drz = self.dr[0].copy(), self.dr[1].copy(), self.dr[2].copy()
+
+    def _put(self, pos=False, v=False):
+        check(lib().sofia_put_state(self.g.ctx, ptr(self.pos) if pos else None, None,
+                                    ptr(self.v) if v else None, None))
+
+    def _get(self, pos=False, v=False):
+        check(lib().sofia_get_state(self.g.ctx, ptr(self.pos) if pos else None,
+                                    ptr(self.v) if v else None, None))
+
+    def _dr(self, put: bool):
+        fn = lib().sofia_put_displacement if put else lib().sofia_get_displacement
+        check(fn(self.g.ctx, ptr(self.dr[0]), ptr(self.dr[1]), ptr(self.dr[2])))
+
+    def nve_v(self):
+        ke = C.c_double()
+        self._put(v=True)
+        check(lib().sofia_nve_v(self.g.ctx, float(np.float32(self.dthalf)), ptr(self.ff), C.byref(ke)))
+        self._get(v=True)
+
+    def forces(self):
+        ctx = self.g.ctx
+        z = [C.c_double(0.0) for _ in range(6)]
+        vir = np.zeros(3, f64)
+        self._put(pos=True)
+        check(lib().sofia_lj_cut_coul_dsf(ctx, ptr(self.ff), None, C.byref(z[0]), C.byref(z[1]), ptr(vir)))
+        self._put(pos=True)
+        check(lib().sofia_pair14_dsf(ctx, ptr(self.ffh), C.byref(z[2]), C.byref(z[3])))
+        self._put(pos=True)
+        e = [C.c_double(0.0) for _ in range(4)]
+        check(lib().sofia_bonded_forces(ctx, ptr(self.ffb), *[C.byref(x) for x in e]))
+        f = self.ff.astype(f64)              # the ff + ffh + bonded merge
+        f += self.ffh
+        f += self.ffb
+        self.ff[:] = f.astype(f32)
+        self._put(pos=True, v=True)
+        mit = C.c_int32()
+        check(lib().sofia_shake(ctx, 1, ptr(self.ff), ptr(vir), C.byref(mit)))
+
+    def step(self) -> bool:
+        """One time step; True when the list was rebuilt."""
+        ctx = self.g.ctx
+        self.nve_v()
+        self._put(pos=True, v=True)
+        self._dr(True)
+        check(lib().sofia_nve_x(ctx))
+        self._get(pos=True)
+        self._dr(False)
+        self._put(pos=True)
+        check(lib().sofia_pbc(ctx))
+        self._get(pos=True)
+        self._dr(True)
+        flag = C.c_int32()
+        check(lib().sofia_check_neighbor(ctx, C.byref(flag)))
+        rebuilt = bool(flag.value)
+        if rebuilt:
+            self._to_g()
+            self.g.neighbor_wrapper()
+            self._from_g()
+        self.forces()
+        self.nve_v()
+        return rebuilt
+
+
 _TUNE_CACHE: dict = {}
 
 
