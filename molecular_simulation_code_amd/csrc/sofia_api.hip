@@ -970,8 +970,13 @@ int sofia_put_state(sofia_ctx *c, const sofia_ainfo *pos, const float *q, const 
     const int n = c->N;
     TRY(sync_check(c));   // kernels in flight may still read the arrays replaced here
     if (pos) {
-        for (int i = 0; i < n; ++i)
-            if (pos[i].type < 1 || pos[i].type > c->p.ntypes) return fail(SOFIA_E_ARG, "atom type out of range");
+        // one pass without early exit (it vectorises): types within 1..ntypes
+        int32_t tmin = INT32_MAX, tmax = INT32_MIN;
+        for (int i = 0; i < n; ++i) {
+            tmin = std::min(tmin, pos[i].type);
+            tmax = std::max(tmax, pos[i].type);
+        }
+        if (n > 0 && (tmin < 1 || tmax > c->p.ntypes)) return fail(SOFIA_E_ARG, "atom type out of range");
         TRY(upload(c->a.pos4, pos, sizeof(float4) * n));
         c->q_by_type = false;   // types may have changed
         drop_graph(c);

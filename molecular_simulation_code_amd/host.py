@@ -449,6 +449,7 @@ class DropinStep:
         self.ffh = self.mem.empty((n, 3), f32)
         self.ffb = self.mem.empty((n, 3), f32)
         self.dr = self.mem.empty((3, n), f32)
+        self.f64 = np.empty((n, 3), f64)               # the host's force merge
         self.dthalf = float(g.sys.derived()["dthalf"])
         self._from_g()
 
@@ -501,10 +502,9 @@ class DropinStep:
         self._put(pos=True)
         e = [C.c_double(0.0) for _ in range(4)]
         check(lib().sofia_bonded_forces(ctx, ptr(self.ffb), *[C.byref(x) for x in e]))
-        f = self.ff.astype(f64)              # the ff + ffh + bonded merge
-        f += self.ffh
-        f += self.ffb
-        self.ff[:] = f.astype(f32)
+        np.add(self.ff, self.ffh, out=self.f64, dtype=f64)   # the ff + ffh + bonded merge
+        self.f64 += self.ffb
+        self.ff[:] = self.f64
         self._put(pos=True, v=True)
         mit = C.c_int32()
         check(lib().sofia_shake(ctx, 1, ptr(self.ff), ptr(vir), C.byref(mit)))

@@ -100,6 +100,43 @@ def test_engine_list_holds_every_pair_inside_the_cutoff():
         e.close()
 
 
+def test_engine_list_complete_when_the_stencil_exceeds_one_lds_stage():
+    """Cutoff 12.5 A with a 2 A skin: a cell's 125-cell stencil holds about 5,600 atoms,
+    more than the 4,096 candidates one LDS stage takes, so every cell is built in two passes
+    (the engine's answer to check_cache_size, Source/mod_neighbor.f90:313-388).  The lists
+    must still hold every pair closer than rv, and the mixed pair forces must match the
+    oracle's DSF on the same list and positions."""
+    host = _engine()
+    from oracle import oracle as O
+    s = _sys(50, 1700, 31, rcut=12.5, cut_coul=12.5)
+    d = s.derived()
+    e = host.Engine([host.ReplicaSpec(s, temp=300.0, gamma_ps=1.0, seed=5)], precision=host.PREC_MIXED)
+    try:
+        e.set_skin(2.0)
+        e.init()
+        rv = 12.5 + 2.0
+        (x, gid, pairs), = _engine_pairs(e)
+        L = float(s.box)
+        ex = _excluded(s, gid)
+        must = _pairs_within(x, L, rv - 1e-3) - ex
+        may = _pairs_within(x, L, rv + 1e-3) - ex
+        per_atom = len(must) / s.np
+        say(f"[wide stencil] np {s.np}, box {L:.2f}, {per_atom:.0f} listed pairs per atom within rv")
+        assert must <= pairs, f"{len(must - pairs)} pairs closer than rv missing"
+        assert pairs <= may, f"{len(pairs - may)} listed pairs beyond rv or excluded"
+        e.run(20)
+        f, _ = e.pair_forces()
+        xv, gidv = e.device_view()
+        nl, nn, na = e.replica_list(0)
+        prm = O.params_from(d, S.NTYPES)
+        ref, _, _, _ = O.dsf_x64(xv, s.types[gidv - 1], s.q[gidv - 1], nl, nn, na, S.lj_tables(), prm)
+        err = float(np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()))
+        say(f"[wide stencil] mixed pair forces vs oracle rel RMS {err:.2e}")
+        assert err < 1e-5
+    finally:
+        e.close()
+
+
 def test_engine_bonded_energies_match_oracle():
     """Bond, angle, torsion and improper energies of the engine (rigid waters evaluated inside the SHAKE
     kernel, the rest in the bonded kernels; torsions by the multiple-angle recurrence) against
