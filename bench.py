@@ -550,7 +550,7 @@ def main():
     log(f"[precision] mixed vs FP64 force rel RMS = {ferr:.2e}")
 
     # ---------------------------------------------------------------- served stream
-    p50 = p99 = served_rate = None
+    p50 = p99 = served_rate = serve_budget = None
     nreq = 0
     if not args.no_serve:
         log(f"[serve] relaxing a pool of {len(pool)} systems")
@@ -563,9 +563,11 @@ def main():
         lat = np.array([r.latency_ms for r in res])
         p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
         served_rate = atom_steps / wall
+        serve_budget = (srv.budget / 1e9, srv.bytes_per_atom)
         log(f"[serve] {nreq} requests, p50 {p50:.0f} ms, p99 {p99:.0f} ms, "
             f"{served_rate / 1e6:.1f} Matom-steps/s, {srv.cohorts_started} cohorts, "
-            f"{srv.rebatches} re-batches, {srv.segments_run} segments")
+            f"{srv.rebatches} re-batches, {srv.segments_run} segments; HBM budget "
+            f"{serve_budget[0]:.0f} GB read from the device, {serve_budget[1]:.0f} B/atom measured")
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = cpu_c1 = None
@@ -631,6 +633,8 @@ def main():
             "served": {"workload": f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g}s, "
                                    "500-step requests, 3,000-150,000 atoms",
                        "requests": nreq, "p50_ms": p50, "p99_ms": p99,
+                       "hbm_budget_GB": None if serve_budget is None else round(serve_budget[0], 1),
+                       "bytes_per_atom": None if serve_budget is None else round(serve_budget[1]),
                        "value": None if served_rate is None else round(served_rate / 1e6, 3)},
         },
         "batched_over_plain": round(batched_rate / per_launch_rate, 3),
@@ -658,6 +662,8 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "peak_unpacked": PEAK_FP32_UNPACKED / 1e12,
                      "frac_unpacked": achieved / PEAK_FP32_UNPACKED, "traffic": traffic_bytes(),
+                     "traffic_workload": (_profile_json("pair_kernel_batched64_traffic.json")
+                                          or {}).get("workload"),
                      "valu_counters": valu,
                      "kernel": PAIR_KERNEL, "ms_per_launch": ms_b,
                      "pairs_in_cutoff_per_launch": pairs_b, "flop_per_pair": FLOP_PER_PAIR},

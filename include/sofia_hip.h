@@ -78,6 +78,9 @@ typedef struct sofia_ctx sofia_ctx;
 /* ---------------------------------------------------------------- lifecycle */
 int sofia_last_error(char *buf, size_t len);
 int sofia_device_count(int *count);
+/* Free and total device memory of the current GPU (hipMemGetInfo), for callers that size a
+ * batch or an admission budget by HBM.  No counterpart in the reference. */
+int sofia_device_memory(size_t *free_bytes, size_t *total_bytes);
 /* One system of np atoms (a batch of one). */
 int sofia_ctx_create(const sofia_params *p, int32_t np, float box, sofia_ctx **out);
 /* nrep replicas; natoms[r] atoms and box[r] for replica r (atoms stored back to back). */

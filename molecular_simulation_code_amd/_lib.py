@@ -74,6 +74,7 @@ SIGNATURES = {
     "sofia_get_displacement": [_vp] * 4,
     "sofia_put_displacement": [_vp] * 4,
     "sofia_host_alloc": [C.c_size_t, _vp],
+    "sofia_device_memory": [_vp, _vp],
     "sofia_host_free": [_vp],
     "sofia_neighbor_wrapper": [_vp, _i32, _vp],
     "sofia_get_cells": [_vp] * 7,
@@ -171,3 +172,10 @@ def device_count() -> int:
     n = C.c_int(0)
     check(lib().sofia_device_count(C.byref(n)))
     return n.value
+
+
+def device_memory() -> tuple:
+    """(free, total) bytes of the current GPU's memory."""
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    check(lib().sofia_device_memory(C.byref(f), C.byref(t)))
+    return f.value, t.value

@@ -887,7 +887,7 @@ __global__ void k_step_add(int32_t nrep, uint64_t *step, long long delta) {
 }  // namespace
 
 void launch_zero_forces(sofia_ctx *c, long long *f, bool energies) {
-    const int n3 = 3 * c->N, ne = c->nrep * NE;
+    const int n3 = 3 * c->N, ne = (int)eacc_len(c->nrep);
     hipLaunchKernelGGL(k_zero, dim3(nblocks(n3 > ne ? n3 : ne, 256)), dim3(256), 0, c->stream, n3, f,
                        ne, energies ? c->eacc : nullptr);
 }

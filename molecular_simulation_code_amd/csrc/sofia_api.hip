@@ -441,7 +441,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     TRY(dalloc(&c->nbuilds, nrep));
     TRY(dalloc(&c->shake_err, 1));
     TRY(dalloc(&c->shake_iters, 1));
-    TRY(dalloc(&c->eacc, (size_t)nrep * NE));
+    TRY(dalloc(&c->eacc, eacc_len(nrep)));
     TRY(dalloc(&c->step_d, nrep));
     TRY(dalloc(&c->reps_d, nrep));
     SOFIA_HIP(hipMemset(c->dr32, 0, sizeof(float) * 3 * (size_t)n));
@@ -483,11 +483,20 @@ int sync_check(sofia_ctx *c) {
     return SOFIA_OK;
 }
 
+// The ESPREAD copies of every (replica, slot) accumulator summed into e[r * NE + slot].
+void sum_eacc(const long long *raw, int nrep, std::vector<long long> &e) {
+    e.assign((size_t)nrep * NE, 0);
+    for (size_t k = 0; k < e.size(); ++k)
+        for (int cp = 0; cp < ESPREAD; ++cp) e[k] += raw[(k * ESPREAD + cp) * ECOPY];
+}
+
 int read_energies(sofia_ctx *c, std::vector<long long> &e) {
-    e.resize((size_t)c->nrep * NE);
-    SOFIA_HIP(hipMemcpyAsync(e.data(), c->eacc, sizeof(long long) * e.size(), hipMemcpyDeviceToHost,
+    std::vector<long long> raw(eacc_len(c->nrep));
+    SOFIA_HIP(hipMemcpyAsync(raw.data(), c->eacc, sizeof(long long) * raw.size(), hipMemcpyDeviceToHost,
                              c->stream));
-    return sync_check(c);
+    TRY(sync_check(c));
+    sum_eacc(raw.data(), c->nrep, e);
+    return SOFIA_OK;
 }
 
 double E(const std::vector<long long> &e, int r, int slot) { return (double)e[(size_t)r * NE + slot] * EFIX_INV; }
@@ -702,6 +711,12 @@ int sofia_device_count(int *count) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
     *count = n;
+    return SOFIA_OK;
+}
+
+int sofia_device_memory(size_t *free_bytes, size_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(SOFIA_E_ARG, "null argument");
+    SOFIA_HIP(hipMemGetInfo(free_bytes, total_bytes));
     return SOFIA_OK;
 }
 
@@ -1177,7 +1192,7 @@ int sofia_lj_cut_coul_dsf_async(sofia_ctx *c, int32_t mixed) {
     if (!c->have_list) return fail(SOFIA_E_STATE, "no Verlet list: call sofia_neighbor_wrapper first");
     if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
     if (c->async_mode >= 0) return fail(SOFIA_E_STATE, "a pair-kernel call is already in flight");
-    const size_t fbytes = sizeof(float) * 3 * (size_t)c->N, ebytes = sizeof(long long) * c->nrep * NE;
+    const size_t fbytes = sizeof(float) * 3 * (size_t)c->N, ebytes = sizeof(long long) * eacc_len(c->nrep);
     if (c->pinned_bytes < fbytes + ebytes) {
         if (c->pinned) (void)hipHostFree(c->pinned);
         c->pinned = nullptr;
@@ -1201,10 +1216,11 @@ int sofia_nonbond_wait(sofia_ctx *c, sofia_coords *ff, double *pot, double *ec, 
     TRY(sync_check(c));
     const size_t fbytes = sizeof(float) * 3 * (size_t)c->N;
     if (ff) std::memcpy(ff, c->pinned, fbytes);
-    const long long *e = (const long long *)((const char *)c->pinned + fbytes);
+    std::vector<long long> es;
+    sum_eacc((const long long *)((const char *)c->pinned + fbytes), c->nrep, es);
     double sp = 0, se = 0, vx = 0, vy = 0, vz = 0;
     for (int r = 0; r < c->nrep; ++r) {
-        const long long *o = e + (size_t)r * NE;
+        const long long *o = es.data() + (size_t)r * NE;
         sp += (double)o[E_LJ] * EFIX_INV; se += (double)o[E_COUL] * EFIX_INV;
         vx += (double)o[V_X] * EFIX_INV; vy += (double)o[V_Y] * EFIX_INV; vz += (double)o[V_Z] * EFIX_INV;
     }
@@ -1292,7 +1308,7 @@ int sofia_nve_v(sofia_ctx *c, float dt_partial, const sofia_coords *ff, double *
     if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
     float *st = staging_f32(c);
     TRY(upload(st, ff, sizeof(float) * 3 * (size_t)c->N));
-    SOFIA_HIP(hipMemsetAsync(c->eacc, 0, sizeof(long long) * c->nrep * NE, c->stream));
+    SOFIA_HIP(hipMemsetAsync(c->eacc, 0, sizeof(long long) * eacc_len(c->nrep), c->stream));
     launch_nve_v32(c, dt_partial, st);
     std::vector<long long> e;
     TRY(read_energies(c, e));
@@ -1351,7 +1367,7 @@ int sofia_shake(sofia_ctx *c, int32_t iflag, sofia_coords *ff, double *vir, int3
     if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
     float *st = staging_f32(c);
     TRY(upload(st, ff, sizeof(float) * 3 * (size_t)c->N));
-    SOFIA_HIP(hipMemsetAsync(c->eacc, 0, sizeof(long long) * c->nrep * NE, c->stream));
+    SOFIA_HIP(hipMemsetAsync(c->eacc, 0, sizeof(long long) * eacc_len(c->nrep), c->stream));
     SOFIA_HIP(hipMemsetAsync(c->shake_err, 0, sizeof(int32_t), c->stream));
     SOFIA_HIP(hipMemsetAsync(c->shake_iters, 0, sizeof(int32_t), c->stream));
     launch_shake32(c, iflag, st, nullptr);
@@ -1741,10 +1757,9 @@ int sofia_md_pair_forces(sofia_ctx *c, double *f, double *e) {
     SOFIA_HIP(hipMemcpyAsync(c->fscratch, c->fnb, fb, hipMemcpyDeviceToDevice, c->stream));
     launch_nonbond(c, nb_mode(c), true);
     std::vector<double> nb(3 * n);
-    std::vector<long long> eq((size_t)c->nrep * NE);
+    std::vector<long long> eq;
     SOFIA_HIP(hipMemcpyAsync(nb.data(), c->fnb, fb, hipMemcpyDeviceToHost, c->stream));
-    SOFIA_HIP(hipMemcpyAsync(eq.data(), c->eacc, sizeof(long long) * eq.size(), hipMemcpyDeviceToHost,
-                             c->stream));
+    TRY(read_energies(c, eq));
     SOFIA_HIP(hipMemcpyAsync(c->fnb, c->fscratch, fb, hipMemcpyDeviceToDevice, c->stream));
     TRY(sync_check(c));
     for (size_t i = 0; i < n; ++i)

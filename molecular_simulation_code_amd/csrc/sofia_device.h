@@ -34,6 +34,12 @@ __device__ inline T group_sum(T v) {
 __device__ inline long long to_fix(double v) { return __double2ll_rn(v * FIX_SCALE); }
 __device__ inline long long to_efix(double v) { return __double2ll_rn(v * EFIX_SCALE); }
 
+// Address of the copy of accumulator (r, slot) that this workgroup adds to.
+__device__ inline unsigned long long *eacc_slot(long long *base, int r, int slot) {
+    const size_t k = ((size_t)r * NE + slot) * ESPREAD + (blockIdx.x % ESPREAD);
+    return (unsigned long long *)&base[k * ECOPY];
+}
+
 // Add an int64 contribution to a per-replica accumulator slot.  One atomic per wave when
 // every contributing lane belongs to the same replica, per-lane atomics otherwise;
 // integer addition makes the total independent of that choice.
@@ -46,10 +52,9 @@ __device__ inline void wave_add(long long v, bool active, int r, long long *slot
         long long s = active ? v : 0;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if ((threadIdx.x & 63) == 0)
-            atomicAdd((unsigned long long *)&slot_base[(size_t)r0 * NE + slot], (unsigned long long)s);
+        if ((threadIdx.x & 63) == 0) atomicAdd(eacc_slot(slot_base, r0, slot), (unsigned long long)s);
     } else if (active) {
-        atomicAdd((unsigned long long *)&slot_base[(size_t)r * NE + slot], (unsigned long long)v);
+        atomicAdd(eacc_slot(slot_base, r, slot), (unsigned long long)v);
     }
 }
 

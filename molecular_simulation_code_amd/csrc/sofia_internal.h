@@ -24,6 +24,14 @@ enum {
     E_LJ = 0, E_COUL, E_POT14, E_COUL14, E_BOND, E_ANGLE, E_DIHED, E_KE,
     V_X, V_Y, V_Z, E_CORR, E_IMPRO, NE
 };
+// Each (replica, slot) accumulator exists in ESPREAD copies, one 128-byte line apart, and a
+// workgroup adds to copy blockIdx % ESPREAD: tens of thousands of wavefronts adding to one
+// address serialise at the L2, spreading them over lines does not.  The host sums the copies
+// (integers: the total does not depend on which copy a wave used).
+constexpr int ESPREAD = 16;
+constexpr int ECOPY = 16;                       // int64 words between copies (128 B)
+constexpr int EACC_PER_SLOT = ESPREAD * ECOPY;
+inline size_t eacc_len(int nrep) { return (size_t)nrep * NE * EACC_PER_SLOT; }
 
 // Per-replica geometry and thermostat (device table).
 struct RepInfo {

@@ -77,14 +77,24 @@ class _Slot:
 
 
 class SegmentServer:
-    def __init__(self, hbm_budget_gb: float = 64.0, segment: int = 100,
+    """Admission budget: hbm_budget_gb, or (None) 85 % of the GPU memory free when the server
+    starts, read from the device.  A request's footprint is its atoms times bytes_per_atom;
+    the first engine the server builds measures the real figure (device memory before and
+    after) and replaces the initial estimate when it is larger."""
+
+    def __init__(self, hbm_budget_gb: float | None = None, segment: int = 100,
                  precision: int = PREC_MIXED, bytes_per_atom: int = 6000,
                  rcontact: float = 3.5, keep_state: bool = False, tuning=None,
                  policy: str = "cohort", max_cohorts: int = 12):
-        self.budget = hbm_budget_gb * 1e9
+        if hbm_budget_gb is None:
+            from ._lib import device_memory
+            self.budget = 0.85 * device_memory()[0]
+        else:
+            self.budget = hbm_budget_gb * 1e9
         self.segment = segment
         self.precision = precision
         self.bytes_per_atom = bytes_per_atom
+        self._calibrated = False
         self.rcontact = rcontact
         self.keep_state = keep_state
         self.tuning = tuning
@@ -102,6 +112,9 @@ class SegmentServer:
     def _engine(self, slots):
         reps = [ReplicaSpec(s.req.system, temp=s.req.temp, gamma_ps=s.req.gamma_ps, seed=s.req.seed)
                 for s in slots]
+        if not self._calibrated:
+            from ._lib import device_memory
+            before = device_memory()[0]
         eng = Engine(reps, precision=self.precision)
         if self.tuning:
             skin, lanes, waves = self.tuning[:3]
@@ -114,6 +127,12 @@ class SegmentServer:
         eng.put_state([(s.x, s.v) for s in slots])
         eng.set_steps([s.step for s in slots])
         eng.resume([s.f for s in slots])
+        if not self._calibrated:
+            used = before - device_memory()[0]
+            natoms = sum(s.req.system.np for s in slots)
+            if used > 0 and natoms > 0:
+                self.bytes_per_atom = max(self.bytes_per_atom, used / natoms)
+            self._calibrated = True
         return eng
 
     def serve(self, requests, realtime: bool = True):
