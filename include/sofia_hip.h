@@ -158,6 +158,21 @@ int sofia_host_free(void *p);
  * tt (optional, np) receives tt(old) = new (1-based) so the host can re-index its own
  * arrays.  Zeroes drx/dry/drz. */
 int sofia_neighbor_wrapper(sofia_ctx *ctx, int32_t step, int32_t *tt);
+/* The arrays the sort permutes, read back from the device already permuted, in the layout
+ * the host gave them (1-based ids within each replica, 0 = empty slot, items in their
+ * original order): q (np), num1bond/num2bond/num3bond (np), specbond (12 per atom),
+ * bondlist (3 per bond), anglelist (4), dihedlist (5), improlist (5), SHAKE atoms (4 per
+ * cluster).  Replaces, for every array the device holds, the host-side re-index after
+ * neighbor_wrapper: sort_posit_charge's q, sort_specbond_numbond, sort_bondlist,
+ * sort_anglelist, sort_dihedlist, sort_improlist and sort_shakeatom
+ * (Source/mod_neighbor.f90:946-1168).  NULL skips an array.  A list comes back with the
+ * item count the device holds (the one given to sofia_set_bonded, sofia_set_impropers or
+ * sofia_set_shake); a list the host never gave the device is not written, so the host
+ * re-indexes that one itself. */
+int sofia_get_topology(sofia_ctx *ctx, float *q, int32_t *num1bond, int32_t *num2bond,
+                       int32_t *num3bond, int32_t *specbond, int32_t *bondlist,
+                       int32_t *anglelist, int32_t *dihedlist, int32_t *improlist,
+                       int32_t *shake_atoms);
 /* cell ranges of the last build: start/endposit (1-based, per cell 0..ncellT-1) and
  * atombin (per atom); ncellT/ncellD/rn of replica 0 */
 int sofia_get_cells(sofia_ctx *ctx, int32_t *start, int32_t *endposit, int32_t *atombin,

@@ -823,6 +823,19 @@ __global__ void k_copy16(size_t n16, const float4 *__restrict__ src, float4 *__r
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride) dst[k] = src[k];
 }
 
+// Atom types (the int bits of pos.w) outside 1..ntypes set *bad; any lane that finds one
+// writes the same value, so a plain store suffices.
+__global__ void k_type_range(int32_t n, const float4 *__restrict__ pos, int32_t ntypes,
+                             int32_t *__restrict__ bad) {
+    const int stride = gridDim.x * blockDim.x;
+    bool out = false;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const int t = __float_as_int(pos[k].w);
+        out |= (t < 1) | (t > ntypes);
+    }
+    if (__ballot(out) && __lane_id() == 0) *bad = 1;
+}
+
 // The fused rigid-water bonded terms alone (steps that run no SHAKE: the minimiser).
 __global__ void k_water_bonded(int32_t ns, const int32_t *__restrict__ num, ShakeArgs A,
                                long long *__restrict__ eacc) {
@@ -984,6 +997,11 @@ void launch_shake64(sofia_ctx *c) {
     if (nb_it + nb_w == 0) return;
     hipLaunchKernelGGL(k_constraints, dim3(nb_it + nb_w), dim3(128), 0, c->stream, c->shake_kidx,
                        e1, e2, e3, e4, nb_it, c->a.wclus, shake_args(c), c->eacc);
+}
+
+void launch_type_range(sofia_ctx *c, const float4 *pos, int32_t *bad) {
+    hipLaunchKernelGGL(k_type_range, dim3(std::min(nblocks(c->N, 256), 1024)), dim3(256), 0,
+                       c->stream, c->N, pos, c->p.ntypes, bad);
 }
 
 void launch_copy16(sofia_ctx *c, size_t n16, const void *src, void *dst) {

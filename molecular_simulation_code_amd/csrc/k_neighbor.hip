@@ -245,6 +245,20 @@ __global__ void k_remap_all(RemapSet S, const int32_t *__restrict__ tt, const in
     }
 }
 
+// A device topology array in the host's layout: in the first nidx columns of each item a
+// global 0-based atom id (-1 = empty) becomes the 1-based id within its replica (0 = empty);
+// the other columns (types) are copied.
+__global__ void k_host_ids(size_t count, int32_t width, int32_t nidx, const int32_t *__restrict__ src,
+                           int32_t *__restrict__ dst, const int32_t *__restrict__ rep,
+                           const RepInfo *__restrict__ reps) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int v = src[e];
+        const bool is_id = (int)(e % (size_t)width) < nidx;
+        dst[e] = !is_id ? v : (v < 0 ? 0 : v - reps[rep[v]].atom0 + 1);
+    }
+}
+
 template <int WAVES>
 __global__ void __launch_bounds__(WAVES * 64)
 k_build_list(const float4 *__restrict__ pos4, const int32_t *__restrict__ num3,
@@ -452,6 +466,15 @@ void launch_remap_topology(sofia_ctx *c) {
     if (total)
         hipLaunchKernelGGL(k_remap_all, dim3(capped(total, 256)), dim3(256), 0, c->stream, S, c->tt,
                            c->rep, c->rebuild, any_flag(c));
+}
+
+void launch_host_ids(sofia_ctx *c, const int32_t *src, int32_t *dst, size_t items, int width,
+                     int nidx) {
+    const size_t count = items * (size_t)width;
+    if (!count) return;
+    const int grid = (int)std::min<size_t>((count + 255) / 256, GRID_CAP);
+    hipLaunchKernelGGL(k_host_ids, dim3(grid), dim3(256), 0, c->stream, count, width, nidx, src,
+                       dst, c->rep, c->reps_d);
 }
 
 void launch_build_list(sofia_ctx *c) {

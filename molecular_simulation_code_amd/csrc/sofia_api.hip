@@ -440,6 +440,7 @@ int create(const sofia_params *p, int nrep, const int32_t *natoms, const float *
     TRY(dalloc(&c->rebuild, nrep + 1));   // nrep flags, then `any`
     TRY(dalloc(&c->nbuilds, nrep));
     TRY(dalloc(&c->shake_err, 1));
+    TRY(dalloc(&c->type_err, 1));
     TRY(dalloc(&c->shake_iters, 1));
     TRY(dalloc(&c->eacc, eacc_len(nrep)));
     TRY(dalloc(&c->step_d, nrep));
@@ -753,6 +754,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
     dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->improlist); dfree(c->impro_k);
     dfree(c->impro_chi0); dfree(c->impro_dn); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
+    dfree(c->topo_stage); dfree(c->type_err);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -984,15 +986,18 @@ int sofia_put_state(sofia_ctx *c, const sofia_ainfo *pos, const float *q, const 
     if (!c) return fail(SOFIA_E_ARG, "null context");
     const int n = c->N;
     TRY(sync_check(c));   // kernels in flight may still read the arrays replaced here
-    if (pos) {
-        // one pass without early exit (it vectorises): types within 1..ntypes
-        int32_t tmin = INT32_MAX, tmax = INT32_MIN;
-        for (int i = 0; i < n; ++i) {
-            tmin = std::min(tmin, pos[i].type);
-            tmax = std::max(tmax, pos[i].type);
-        }
-        if (n > 0 && (tmin < 1 || tmax > c->p.ntypes)) return fail(SOFIA_E_ARG, "atom type out of range");
-        TRY(upload(c->a.pos4, pos, sizeof(float4) * n));
+    if (pos && n > 0) {
+        // Types within 1..ntypes, checked on the device (a host pass over the array costs
+        // more than the copy) on a staging copy, the sort's gather target, which holds nothing
+        // between list builds; a rejected upload leaves the state as it was.
+        TRY(upload(c->b.pos4, pos, sizeof(float4) * n));
+        SOFIA_HIP(hipMemsetAsync(c->type_err, 0, sizeof(int32_t), c->stream));
+        launch_type_range(c, c->b.pos4, c->type_err);
+        int32_t bad = 0;
+        SOFIA_HIP(hipMemcpyAsync(&bad, c->type_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        TRY(sync_check(c));
+        if (bad) return fail(SOFIA_E_ARG, "atom type out of range");
+        launch_copy16(c, (size_t)n, c->b.pos4, c->a.pos4);
         c->q_by_type = false;   // types may have changed
         drop_graph(c);
     }
@@ -1093,6 +1098,46 @@ int sofia_neighbor_wrapper(sofia_ctx *c, int32_t, int32_t *tt) {
             const auto &R = c->reps_h[r];
             for (int k = 0; k < R.natom; ++k) tt[R.atom0 + k] = t[R.atom0 + k] - R.atom0 + 1;
         }
+    }
+    return SOFIA_OK;
+}
+
+int sofia_get_topology(sofia_ctx *c, float *q, int32_t *num1bond, int32_t *num2bond,
+                       int32_t *num3bond, int32_t *specbond, int32_t *bondlist, int32_t *anglelist,
+                       int32_t *dihedlist, int32_t *improlist, int32_t *shake_atoms) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    TRY(sync_check(c));
+    const size_t n = c->N;
+    if (q) TRY(download(q, c->a.q, sizeof(float) * n));
+    if (num1bond) TRY(download(num1bond, c->a.num1, sizeof(int32_t) * n));
+    if (num2bond) TRY(download(num2bond, c->a.num2, sizeof(int32_t) * n));
+    if (num3bond) TRY(download(num3bond, c->a.num3, sizeof(int32_t) * n));
+    struct Arr {
+        int32_t *dst;
+        const int32_t *src;
+        size_t items;
+        int width, nidx;
+    };
+    const Arr arrs[] = {{specbond, c->a.spec, n, SPEC, SPEC},
+                        {bondlist, c->bondlist, (size_t)c->nb, 3, 2},
+                        {anglelist, c->anglelist, (size_t)c->na, 4, 3},
+                        {dihedlist, c->dihedlist, (size_t)c->nd, 5, 4},
+                        {improlist, c->improlist, (size_t)c->nimp, 5, 4},
+                        {shake_atoms, c->shake_atoms, (size_t)c->ns, 4, 4}};
+    size_t need = 0;
+    for (const Arr &x : arrs)
+        if (x.dst) need = std::max(need, x.items * x.width);
+    if (need > c->topo_stage_len) {
+        dfree(c->topo_stage);
+        c->topo_stage_len = 0;
+        TRY(dalloc(&c->topo_stage, need));
+        c->topo_stage_len = need;
+    }
+    for (const Arr &x : arrs) {
+        if (!x.dst || !x.items) continue;
+        launch_host_ids(c, x.src, c->topo_stage, x.items, x.width, x.nidx);
+        TRY(sync_check(c));
+        TRY(download(x.dst, c->topo_stage, sizeof(int32_t) * x.items * x.width));
     }
     return SOFIA_OK;
 }

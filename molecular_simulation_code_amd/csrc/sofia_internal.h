@@ -138,6 +138,7 @@ struct sofia_ctx {
     int32_t *rebuild = nullptr;     // per replica
     int32_t *nbuilds = nullptr;     // per replica
     int32_t *shake_err = nullptr;   // device scalar
+    int32_t *type_err = nullptr;    // device scalar: put_state's type check
     int32_t *shake_iters = nullptr; // device scalar
 
     // topology (global 0-based indices)
@@ -147,6 +148,8 @@ struct sofia_ctx {
     int32_t *torsions = nullptr, ntor = 0;
     int32_t *improlist = nullptr, nimp = 0;   // (i1, i2, i3 central, i4, type) per improper
     int32_t *shake_num = nullptr, *shake_atoms = nullptr, ns = 0;
+    int32_t *topo_stage = nullptr;   // topology in the host's id layout, on its way out
+    size_t topo_stage_len = 0;
     // engine constraints: indices of the iteratively solved clusters grouped by kind 1, 3,
     // 4, -3, and the count of symmetric rigid waters (SETTLE, found through AtomArrays::wclus);
     // built when the engine sets up its lists
@@ -235,6 +238,8 @@ void launch_cell_scan(sofia_ctx *c);
 void launch_scatter_sort(sofia_ctx *c);
 void launch_gather(sofia_ctx *c, bool engine);
 void launch_remap_topology(sofia_ctx *c);
+void launch_host_ids(sofia_ctx *c, const int32_t *src, int32_t *dst, size_t items, int width,
+                     int nidx);
 void launch_build_list(sofia_ctx *c);
 void launch_build_list_engine(sofia_ctx *c);
 void launch_check_neighbor(sofia_ctx *c, bool engine);
@@ -273,6 +278,7 @@ void launch_langevin64(sofia_ctx *c);
 void launch_shake64(sofia_ctx *c);
 void launch_water_bonded(sofia_ctx *c);
 void launch_copy16(sofia_ctx *c, size_t n16, const void *src, void *dst);
+void launch_type_range(sofia_ctx *c, const float4 *pos, int32_t *bad);
 void launch_step_advance(sofia_ctx *c);
 void launch_step_add(sofia_ctx *c, long long delta);
 void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp);

@@ -142,6 +142,14 @@ module sofia_hip_iface
        integer(c_int32_t), value :: step
        integer(c_int32_t), intent(inout) :: tt(*)
      end function
+     integer(c_int) function sofia_get_topology(ctx, q, n1, n2, n3, spec, bl, al, dl, il, sa) &
+          bind(c, name="sofia_get_topology")
+       import :: c_int, c_ptr, c_int32_t, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(inout) :: q(*)
+       integer(c_int32_t), intent(inout) :: n1(*), n2(*), n3(*), spec(*), bl(*), al(*), dl(*)
+       type(c_ptr), value :: il, sa
+     end function
      integer(c_int) function sofia_get_nlist(ctx, nlist, numneigh, na) bind(c, name="sofia_get_nlist")
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx
@@ -304,45 +312,26 @@ contains
     allocate(tt_buf(np))
   end subroutine sofia_hip_init
 
-  ! neighbor_wrapper(step): the device sorts and builds the list; the returned tt re-indexes
-  ! the host arrays the device does not hold (mol, integrate_flag), as sort_posit_charge does.
-  ! neighbor_wrapper, Source/mod_neighbor.f90:8-86.  The device sorts and lists; the host
-  ! arrays the device does not return are re-indexed through tt here, as sort_posit_charge
-  ! and the sort_* routines do (Source/mod_neighbor.f90:946-1168): q, mol, integrate_flag,
-  ! num1/2/3bond + specbond, and the bond, angle, dihedral, improper and SHAKE lists.
+  ! neighbor_wrapper, Source/mod_neighbor.f90:8-86.  The device sorts and lists, and returns
+  ! the arrays it permuted itself in the host's layout (sofia_get_topology: q, num1/2/3bond,
+  ! specbond, the bond, angle and dihedral lists that sofia_hip_init gave it).  The arrays
+  ! the device does not hold (mol, integrate_flag, improlist, the shakeatom records) are
+  ! re-indexed through tt here, as sort_posit_charge, sort_improlist and sort_shakeatom do
+  ! (Source/mod_neighbor.f90:946-1005, 1043-1061, 1126-1168).
   subroutine neighbor_wrapper_hip(step)
-    integer :: step, i, j
+    integer :: step, i
     nn_builds = nn_builds + 1
     call sofia_check(sofia_put_state(ctx, position, q, v, global_id), 'put_state')
     call sofia_check(sofia_neighbor_wrapper(ctx, int(step, c_int32_t), tt_buf), 'neighbor_wrapper')
     call sofia_check(sofia_get_state(ctx, position, v, global_id), 'get_state')
+    call sofia_check(sofia_get_topology(ctx, q, num1bond, num2bond, num3bond, specbond, bondlist, &
+         anglelist, dihedlist, c_null_ptr, c_null_ptr), 'get_topology')
     do i = 1, np
-       qs(tt_buf(i)) = q(i)
        molsort(:, tt_buf(i)) = mol(:, i)
        integrate_flag_ss(tt_buf(i)) = integrate_flag(i)
-       num1bondss(tt_buf(i)) = num1bond(i)
-       num2bondss(tt_buf(i)) = num2bond(i)
-       num3bondss(tt_buf(i)) = num3bond(i)
-       do j = 1, num3bond(i)
-          specbond_ss(j, tt_buf(i)) = tt_buf(specbond(j, i))
-       end do
     end do
-    q(1:np) = qs(1:np)
     mol(:,:) = molsort(:,:)
     integrate_flag(:) = integrate_flag_ss(:)
-    specbond(:,:) = specbond_ss(:,:)
-    num1bond(:) = num1bondss(:)
-    num2bond(:) = num2bondss(:)
-    num3bond(:) = num3bondss(:)
-    do i = 1, numbond
-       bondlist(1:2, i) = tt_buf(bondlist(1:2, i))
-    end do
-    do i = 1, numangle
-       anglelist(1:3, i) = tt_buf(anglelist(1:3, i))
-    end do
-    do i = 1, numdihed
-       dihedlist(1:4, i) = tt_buf(dihedlist(1:4, i))
-    end do
     do i = 1, numimpro
        improlist(1:4, i) = tt_buf(improlist(1:4, i))
     end do

@@ -199,27 +199,33 @@ class SofiaGPU:
         self._put_pos()
         tt = np.empty(self.np, i32)
         check(lib().sofia_neighbor_wrapper(self.ctx, step, ptr(tt)))
-        self.nn_builds += 1
-        perm = np.empty(self.np, i32)
-        perm[tt - 1] = np.arange(1, self.np + 1, dtype=i32)
-        inv = perm - 1
         self._get_pos()
+        self.permute_host(tt)
+        return tt
+
+    def permute_host(self, tt: np.ndarray):
+        """Apply the sort permutation tt (old -> new, 1-based) to the host's own per-atom arrays
+        and atom ids, as sort_posit_charge and the sort_* routines do
+        (Source/mod_neighbor.f90:946-1168); position, v and global_id come back from the device
+        already sorted.  Clears drx/dry/drz and reads the list stride."""
+        self.nn_builds += 1
+        inv = np.empty(self.np, np.intp)              # new position -> old index
+        inv[tt - 1] = np.arange(self.np)
+        t0 = np.zeros(self.np + 1, i32)               # 1-based id -> new id; 0 (empty) stays 0
+        t0[1:] = tt
         self.q = self.q[inv]
         self.mol = self.mol[inv]
         self.num1bond = self.num1bond[inv]; self.num2bond = self.num2bond[inv]
         self.num3bond = self.num3bond[inv]
-        sp = self.specbond[inv]
-        self.specbond = np.where(sp > 0, tt[np.maximum(sp, 1) - 1], 0).astype(i32)
-        self.bondlist[:, :2] = tt[self.bondlist[:, :2] - 1]
-        self.anglelist[:, :3] = tt[self.anglelist[:, :3] - 1]
-        self.dihedlist[:, :4] = tt[self.dihedlist[:, :4] - 1]
-        sa = self.shake_atoms
-        self.shake_atoms = np.where(sa > 0, tt[np.maximum(sa, 1) - 1], 0).astype(i32)
+        self.specbond = t0[self.specbond[inv]]
+        self.bondlist[:, :2] = t0[self.bondlist[:, :2]]
+        self.anglelist[:, :3] = t0[self.anglelist[:, :3]]
+        self.dihedlist[:, :4] = t0[self.dihedlist[:, :4]]
+        self.shake_atoms = t0[self.shake_atoms]
         self.drx[:] = 0; self.dry[:] = 0; self.drz[:] = 0
         na = C.c_int32()
         check(lib().sofia_get_nlist(self.ctx, None, None, C.byref(na)))
         self.neigh_alloc = na.value
-        return tt
 
     def get_nlist(self):
         na = C.c_int32()
@@ -436,8 +442,10 @@ class DropinStep:
       summed on the host as the reference sums ff, ffh and the bonded forces;
     * shake(1): position, v, ff -> ff.
 
-    neighbor_wrapper, when check_neighbor asks for it, goes through SofiaGPU (full state in
-    and out, host arrays re-indexed).  The trajectory equals SofiaGPU's step for step."""
+    * neighbor_wrapper, when check_neighbor asks for it: position, v -> position, v,
+      global_id, then the host permutes its own per-atom arrays and topology.
+
+    The trajectory equals SofiaGPU's step for step."""
 
     def __init__(self, g: "SofiaGPU"):
         self.g = g
@@ -450,6 +458,11 @@ class DropinStep:
         self.ffb = self.mem.empty((n, 3), f32)
         self.dr = self.mem.empty((3, n), f32)
         self.f64 = np.empty((n, 3), f64)               # the host's force merge
+        # host arrays that sofia_get_topology fills in place after each list build
+        g.q = np.ascontiguousarray(g.q, f32)
+        for k in ("num1bond", "num2bond", "num3bond", "specbond", "bondlist", "anglelist",
+                  "dihedlist", "shake_atoms"):
+            setattr(g, k, np.ascontiguousarray(getattr(g, k), i32))
         self.dthalf = float(g.sys.derived()["dthalf"])
         self._from_g()
 
@@ -484,6 +497,34 @@ class DropinStep:
     def _dr(self, put: bool):
         fn = lib().sofia_put_displacement if put else lib().sofia_get_displacement
         check(fn(self.g.ctx, ptr(self.dr[0]), ptr(self.dr[1]), ptr(self.dr[2])))
+
+    def neighbor_wrapper(self):
+        """neighbor_wrapper (Source/mod_neighbor.f90:8-86): position and v in (q and
+        global_id are already on the device, which permutes them itself); the device sorts
+        the atoms and builds the list; position, v and global_id out; then the host permutes
+        its own per-atom arrays and topology."""
+        g = self.g
+        tt = np.empty(g.np, i32)
+        check(lib().sofia_put_state(g.ctx, ptr(self.pos), None, ptr(self.v), None))
+        check(lib().sofia_neighbor_wrapper(g.ctx, 0, ptr(tt)))
+        check(lib().sofia_get_state(g.ctx, ptr(self.pos), ptr(self.v), ptr(g.global_id)))
+        # the arrays the device holds come back permuted; the host permutes only the rest
+        check(lib().sofia_get_topology(g.ctx, ptr(g.q), ptr(g.num1bond), ptr(g.num2bond),
+                                       ptr(g.num3bond), ptr(g.specbond), ptr(g.bondlist),
+                                       ptr(g.anglelist), ptr(g.dihedlist), None,
+                                       ptr(g.shake_atoms)))
+        inv = np.empty(g.np, np.intp)
+        inv[tt - 1] = np.arange(g.np)
+        g.mol = g.mol[inv]
+        g.nn_builds += 1
+        na = C.c_int32()
+        check(lib().sofia_get_nlist(g.ctx, None, None, C.byref(na)))
+        g.neigh_alloc = na.value
+        self.dr[:] = 0
+
+    def export(self):
+        """Copy the step's position, type, v, ff and drx/dry/drz into the SofiaGPU host arrays."""
+        self._to_g()
 
     def nve_v(self):
         ke = C.c_double()
@@ -526,9 +567,7 @@ class DropinStep:
         check(lib().sofia_check_neighbor(ctx, C.byref(flag)))
         rebuilt = bool(flag.value)
         if rebuilt:
-            self._to_g()
-            self.g.neighbor_wrapper()
-            self._from_g()
+            self.neighbor_wrapper()
         self.forces()
         self.nve_v()
         return rebuilt

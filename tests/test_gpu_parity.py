@@ -429,5 +429,23 @@ def test_error_paths_are_reported_not_faulted():
             check(lib().sofia_berendsen(g.ctx, ptr(np.array([3.0])), None))   # mu out of range
         with pytest.raises(SofiaError):
             check(lib().sofia_md_run(g.ctx, 10, 1))          # engine not initialised
+        say("[gpu errors] atom types out of range leave the state as it was")
+        a0 = np.empty((g.np, 4), np.float32)
+        check(lib().sofia_get_state(g.ctx, ptr(a0), None, None))
+        for k, t in ((7, 99), (g.np - 1, 0), (0, -3)):
+            bad = a0.copy()
+            bad[:, :3] += 0.25
+            bad[k, 3] = np.array([t], np.int32).view(np.float32)[0]
+            with pytest.raises(SofiaError, match="type out of range"):
+                check(lib().sofia_put_state(g.ctx, ptr(bad), None, None, None))
+            now = np.empty_like(a0)
+            check(lib().sofia_get_state(g.ctx, ptr(now), None, None))
+            assert np.array_equal(now, a0)
+        good = a0.copy()
+        good[:, :3] += 0.25
+        check(lib().sofia_put_state(g.ctx, ptr(good), None, None, None))
+        now = np.empty_like(a0)
+        check(lib().sofia_get_state(g.ctx, ptr(now), None, None))
+        assert np.array_equal(now, good)
     finally:
         g.close()
