@@ -171,11 +171,11 @@ def wrapped_f32(x, box):
 
 def dropin_rate(system, state, steps, warmup):
     """The plain GPU drop-in: the reference's DSF time loop (Source/mod_integrate.f90:270-308,
-    force_wrapper_DSF_NOMIC Source/mod_force.f90:510-554) with every routine of the step
+    force_wrapper_DSF_NOMIC Source/mod_force.f90:508-551) with every routine of the step
     replaced by its C-ABI drop-in on host-owned real*4 arrays, as a Fortran host would call
     them through the ISO_C_BINDING shim: nve_v, nve_x, pbc, check_neighbor (and
-    neighbor_wrapper when triggered), the FP64 DSF pair kernel, the 1-4 pair, the bonded
-    terms, shake, nve_v.  Each routine copies the arrays it reads in and the arrays it writes
+    neighbor_wrapper when triggered), force_wrapper_DSF_NOMIC (the FP64 DSF pair kernel, the
+    1-4 pairs and the bonded terms), shake, nve_v.  Each routine copies the arrays it reads in and the arrays it writes
     out over PCIe (host.DropinStep, page-locked arrays), as the reference's MIC offload
     copied position in and ff out per call.  Returns (atom-steps/s, list builds during the
     timed steps, MB moved over the host link per step)."""
@@ -196,9 +196,9 @@ def dropin_rate(system, state, steps, warmup):
         for _ in range(steps):
             builds += d.step()
         el = time.perf_counter() - t0
-        # per step without a rebuild: nve_v 7 arrays-of-3 (v, ff in; v out) twice, nve_x, pbc,
-        # check, three force routines and shake, in units of np floats
-        floats = 2 * 9 + (4 + 3 + 3 + 4 + 3) + (4 + 4) + 3 + 3 * (4 + 3) + (4 + 3 + 3 + 3)
+        # per step without a rebuild, in units of np floats: nve_v (v, ff in; v out) twice,
+        # nve_x, pbc, check, the force wrapper (position in, ff out) and shake
+        floats = 2 * 9 + (4 + 3 + 3 + 4 + 3) + (4 + 4) + 3 + (4 + 3) + (4 + 3 + 3 + 3)
         return s.np * steps / el, builds, floats * 4 * s.np / 1e6
     finally:
         if d is not None:

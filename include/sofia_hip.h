@@ -221,6 +221,14 @@ int sofia_pair14_dsf(sofia_ctx *ctx, sofia_coords *ffh, double *e_coul_14, doubl
  * ADDED to.  Not on a context running the engine, which evaluates these itself. */
 int sofia_bonded_forces(sofia_ctx *ctx, sofia_coords *ff, double *e_bond, double *e_angle,
                         double *e_dihed, double *e_improper);
+/* force_wrapper_DSF_NOMIC (Source/mod_force.f90:508-551): the DSF pair forces, the 1-4 pairs
+ * and the bonded terms on the current positions in one call.  ff (np, set) is the three
+ * routines' real*4 outputs merged the way the host merges them, ff = real*4(dble(pair) +
+ * dble(1-4) + dble(bonded)); bit for bit what sofia_lj_cut_coul_dsf, sofia_pair14_dsf and
+ * sofia_bonded_forces followed by that merge give, with one copy out instead of three.
+ * e (8, added to): potential, e_coul, e_coul_14, pot_14, e_bond, e_angle, e_dihed,
+ * e_improper.  vir (3, set): the pair kernel's virial. */
+int sofia_force_wrapper_dsf(sofia_ctx *ctx, sofia_coords *ff, double *e, double *vir);
 /* nve_v(dt_partial) / nve_x / pbc(step), Source/mod_ensemble.f90:11-87,
  * Source/mod_pbc.f90:6-39, on the real*4 state with force ff (np). */
 int sofia_nve_v(sofia_ctx *ctx, float dt_partial, const sofia_coords *ff, double *ke);

@@ -78,6 +78,7 @@ SIGNATURES = {
     "sofia_host_free": [_vp],
     "sofia_neighbor_wrapper": [_vp, _i32, _vp],
     "sofia_get_topology": [_vp] * 11,
+    "sofia_force_wrapper_dsf": [_vp] * 4,
     "sofia_get_cells": [_vp] * 7,
     "sofia_get_nlist": [_vp] * 4,
     "sofia_check_neighbor": [_vp, _vp],

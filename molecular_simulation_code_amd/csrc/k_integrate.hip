@@ -823,6 +823,21 @@ __global__ void k_copy16(size_t n16, const float4 *__restrict__ src, float4 *__r
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride) dst[k] = src[k];
 }
 
+// The force wrapper's merge: each term rounded to real*4 as its own routine returns it (the
+// fixed-point sums as k_fix_to_aos rounds them), then pair + 1-4 + bonded summed in FP64 and
+// stored real*4 into ff (which holds the pair term on entry).
+__global__ void k_merge_forces(int32_t n, float *__restrict__ ff, const long long *__restrict__ f14,
+                               const long long *__restrict__ fb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float h = (float)((double)f14[(size_t)c * n + i] * FIX_INV);
+        const float b = (float)((double)fb[(size_t)c * n + i] * FIX_INV);
+        ff[3 * (size_t)i + c] = (float)(((double)ff[3 * (size_t)i + c] + (double)h) + (double)b);
+    }
+}
+
 // Atom types (the int bits of pos.w) outside 1..ntypes set *bad; any lane that finds one
 // writes the same value, so a plain store suffices.
 __global__ void k_type_range(int32_t n, const float4 *__restrict__ pos, int32_t ntypes,
@@ -997,6 +1012,11 @@ void launch_shake64(sofia_ctx *c) {
     if (nb_it + nb_w == 0) return;
     hipLaunchKernelGGL(k_constraints, dim3(nb_it + nb_w), dim3(128), 0, c->stream, c->shake_kidx,
                        e1, e2, e3, e4, nb_it, c->a.wclus, shake_args(c), c->eacc);
+}
+
+void launch_merge_forces(sofia_ctx *c, float *ff32, const long long *f14, const long long *fb) {
+    hipLaunchKernelGGL(k_merge_forces, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N, ff32,
+                       f14, fb);
 }
 
 void launch_type_range(sofia_ctx *c, const float4 *pos, int32_t *bad) {

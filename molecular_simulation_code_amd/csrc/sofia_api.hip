@@ -754,7 +754,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
     dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->improlist); dfree(c->impro_k);
     dfree(c->impro_chi0); dfree(c->impro_dn); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
-    dfree(c->topo_stage); dfree(c->type_err);
+    dfree(c->topo_stage); dfree(c->type_err); dfree(c->fix14);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1345,6 +1345,38 @@ int sofia_bonded_forces(sofia_ctx *c, sofia_coords *ff, double *e_bond, double *
     if (e_angle) *e_angle += sa;
     if (e_dihed) *e_dihed += sd;
     if (e_improper) *e_improper += si;
+    return SOFIA_OK;
+}
+
+int sofia_force_wrapper_dsf(sofia_ctx *c, sofia_coords *ff, double *e, double *vir) {
+    if (!c) return fail(SOFIA_E_ARG, "null context");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no Verlet list: call sofia_neighbor_wrapper first");
+    if (!c->have_ff) return fail(SOFIA_E_STATE, "force field not set");
+    if ((c->nb || c->na || c->ntor || c->nimp) && !c->bond_k)
+        return fail(SOFIA_E_STATE, "bonded coefficients not set");
+    if (c->engine_ready) return fail(SOFIA_E_STATE, "a running engine evaluates its own forces");
+    const int n = c->N;
+    if (!c->fix14) TRY(dalloc(&c->fix14, (size_t)3 * n));
+    float *st = staging_f32(c);
+    launch_zero_forces(c, c->fscratch, true);   // energies zeroed once: the terms use their own slots
+    launch_zero_forces(c, c->fix14, false);
+    launch_nonbond_out(c, 0, true, st);
+    launch_pair14(c, c->fix14);
+    launch_bonded_dropin(c, c->fscratch);
+    launch_merge_forces(c, st, c->fix14, c->fscratch);
+    std::vector<long long> es;
+    TRY(read_energies(c, es));
+    if (ff) TRY(download(ff, st, sizeof(float) * 3 * (size_t)n));
+    // summed over replicas first, then added, as each routine of its own adds its sums
+    const int slot[8] = {E_LJ, E_COUL, E_COUL14, E_POT14, E_BOND, E_ANGLE, E_DIHED, E_IMPRO};
+    double sum[8] = {0, 0, 0, 0, 0, 0, 0, 0}, vx = 0, vy = 0, vz = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        for (int k = 0; k < 8; ++k) sum[k] += E(es, r, slot[k]);
+        vx += E(es, r, V_X); vy += E(es, r, V_Y); vz += E(es, r, V_Z);
+    }
+    if (e)
+        for (int k = 0; k < 8; ++k) e[k] += sum[k];
+    if (vir) { vir[0] = vx; vir[1] = vy; vir[2] = vz; }
     return SOFIA_OK;
 }
 

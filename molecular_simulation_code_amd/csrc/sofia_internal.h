@@ -110,6 +110,7 @@ struct sofia_ctx {
     long long *fix = nullptr;   // fixed-point accumulator of every other force, SoA (3N)
     long long *fscratch = nullptr;   // fixed-point scratch for drop-in outputs (3N)
     float *stage32 = nullptr;        // AoS real*4 staging for drop-in inputs/outputs (3N)
+    long long *fix14 = nullptr;      // force wrapper: the 1-4 term's fixed-point sums (3N, lazy)
 
     // cluster analysis: global solute-molecule index per atom lives in AtomArrays::mol
     int32_t nmol = 0;
@@ -279,6 +280,7 @@ void launch_shake64(sofia_ctx *c);
 void launch_water_bonded(sofia_ctx *c);
 void launch_copy16(sofia_ctx *c, size_t n16, const void *src, void *dst);
 void launch_type_range(sofia_ctx *c, const float4 *pos, int32_t *bad);
+void launch_merge_forces(sofia_ctx *c, float *ff32, const long long *f14, const long long *fb);
 void launch_step_advance(sofia_ctx *c);
 void launch_step_add(sofia_ctx *c, long long delta);
 void launch_minimize_step(sofia_ctx *c, double gamma, double max_disp);

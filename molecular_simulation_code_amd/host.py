@@ -438,8 +438,8 @@ class DropinStep:
 
     * nve_v: v, ff -> v;  nve_x: position, v, drx/dry/drz -> position, drx/dry/drz;
     * pbc: position -> position;  check_neighbor: drx/dry/drz;
-    * the DSF pair kernel, the 1-4 pair and the bonded terms: position -> their forces,
-      summed on the host as the reference sums ff, ffh and the bonded forces;
+    * force_wrapper_DSF_NOMIC (the DSF pair kernel, the 1-4 pairs and the bonded terms,
+      merged on the device as the host merges the three routines' outputs): position -> ff;
     * shake(1): position, v, ff -> ff.
 
     * neighbor_wrapper, when check_neighbor asks for it: position, v -> position, v,
@@ -454,10 +454,9 @@ class DropinStep:
         self.pos = self.mem.empty((n, 4), f32)       # type(ainfo): x, y, z, type
         self.v = self.mem.empty((n, 3), f32)
         self.ff = self.mem.empty((n, 3), f32)
-        self.ffh = self.mem.empty((n, 3), f32)
-        self.ffb = self.mem.empty((n, 3), f32)
         self.dr = self.mem.empty((3, n), f32)
-        self.f64 = np.empty((n, 3), f64)               # the host's force merge
+        # potential, e_coul, e_coul_14, pot_14, e_bond, e_angle, e_dihed, e_improper (added to)
+        self.energies = np.zeros(8, f64)
         # host arrays that sofia_get_topology fills in place after each list build
         g.q = np.ascontiguousarray(g.q, f32)
         for k in ("num1bond", "num2bond", "num3bond", "specbond", "bondlist", "anglelist",
@@ -467,7 +466,7 @@ class DropinStep:
         self._from_g()
 
     def close(self):
-        self.pos = self.v = self.ff = self.ffh = self.ffb = self.dr = None
+        self.pos = self.v = self.ff = self.dr = None
         self.mem.release()
 
     def _from_g(self):
@@ -533,19 +532,12 @@ class DropinStep:
         self._get(v=True)
 
     def forces(self):
+        """force_wrapper_DSF_NOMIC (pair, 1-4 and bonded forces, merged on the device) then
+        shake(1)."""
         ctx = self.g.ctx
-        z = [C.c_double(0.0) for _ in range(6)]
         vir = np.zeros(3, f64)
         self._put(pos=True)
-        check(lib().sofia_lj_cut_coul_dsf(ctx, ptr(self.ff), None, C.byref(z[0]), C.byref(z[1]), ptr(vir)))
-        self._put(pos=True)
-        check(lib().sofia_pair14_dsf(ctx, ptr(self.ffh), C.byref(z[2]), C.byref(z[3])))
-        self._put(pos=True)
-        e = [C.c_double(0.0) for _ in range(4)]
-        check(lib().sofia_bonded_forces(ctx, ptr(self.ffb), *[C.byref(x) for x in e]))
-        np.add(self.ff, self.ffh, out=self.f64, dtype=f64)   # the ff + ffh + bonded merge
-        self.f64 += self.ffb
-        self.ff[:] = self.f64
+        check(lib().sofia_force_wrapper_dsf(ctx, ptr(self.ff), ptr(self.energies), ptr(vir)))
         self._put(pos=True, v=True)
         mit = C.c_int32()
         check(lib().sofia_shake(ctx, 1, ptr(self.ff), ptr(vir), C.byref(mit)))
