@@ -397,6 +397,29 @@ def profile_run(args, workers):
         eng.close()
 
 
+def dropin_run(args):
+    """C2 (204,000 atoms) minimised and equilibrated on the FP64 engine as the full run does,
+    then args.steps drop-in steps after args.warmup; logs the rate."""
+    from molecular_simulation_code_amd import _lib, host
+    from molecular_simulation_code_amd import system as S
+    c2 = S.make_system(1200, 64000, seed=20261203)
+    if _lib.device_count() < 1:
+        log("no HIP device")
+        sys.exit(3)
+    eng = host.Engine([host.ReplicaSpec(c2, temp=300.0, gamma_ps=1.0, seed=1)],
+                      precision=host.PREC_FP64)
+    eng.init()                       # prepare_engine's steps, eager (no graph under a tracer)
+    eng.minimize(args.minimize)
+    eng.set_thermostat(gammas=[20.0])
+    eng.run(args.equil, use_graph=False)
+    state = eng.state()[0]
+    eng.close()
+    log(f"[plain drop-in] C2 prepared: minimise {args.minimize}, Langevin {args.equil} steps")
+    rate, builds, mb = dropin_rate(c2, state, args.steps, args.warmup)
+    log(f"[plain drop-in] {rate / 1e6:.1f} Matom-steps/s ({args.steps} steps, {builds} list "
+        f"builds, {mb:.0f} MB over the host link per step)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,6 +446,9 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="short run of the batched workload for rocprofv3: a few steps, the pair "
                          "kernel and the 1 GiB calibration copy; no JSON line")
+    ap.add_argument("--dropin-run", action="store_true",
+                    help="the plain drop-in leg alone for rocprofv3: C2 prepared as the full run "
+                         "prepares it, then --steps drop-in steps; no JSON line")
     ap.add_argument("--tuning", default=None,
                     help="profile run: skin,lanes,list_waves[,bonded_mode[,steps_per_graph]] to use "
                          "instead of the autotuner")
@@ -445,6 +471,9 @@ def main():
         return
     if args.drift:
         drift_run(args, workers)
+        return
+    if args.dropin_run:
+        dropin_run(args)
         return
 
     # ---------------------------------------------------------------- inputs (CPU, before any GPU use)
