@@ -173,6 +173,13 @@ int sofia_get_topology(sofia_ctx *ctx, float *q, int32_t *num1bond, int32_t *num
                        int32_t *num3bond, int32_t *specbond, int32_t *bondlist,
                        int32_t *anglelist, int32_t *dihedlist, int32_t *improlist,
                        int32_t *shake_atoms);
+/* RDF_calc, Source/mod_force_nonbond.f90:708-775, on the current list and positions: for
+ * every atom of type itype and list neighbour of type jtype with dr2 < rcut2 (FP64 distance,
+ * real*8 minimum image), rdf[node] += 1 with node = int(sqrt(dr2) / rdf_delta), node from 0;
+ * nodes >= nbins are not counted (the reference has no bound; its call is itype = jtype = 3).
+ * numob_rdf (optional) is incremented.  Counts are exact while a bin stays below 2^24. */
+int sofia_rdf_calc(sofia_ctx *ctx, int32_t itype, int32_t jtype, float rdf_delta, int32_t nbins,
+                   float *rdf, int32_t *numob_rdf);
 /* cell ranges of the last build: start/endposit (1-based, per cell 0..ncellT-1) and
  * atombin (per atom); ncellT/ncellD/rn of replica 0 */
 int sofia_get_cells(sofia_ctx *ctx, int32_t *start, int32_t *endposit, int32_t *atombin,

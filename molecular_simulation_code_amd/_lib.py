@@ -79,6 +79,7 @@ SIGNATURES = {
     "sofia_neighbor_wrapper": [_vp, _i32, _vp],
     "sofia_get_topology": [_vp] * 11,
     "sofia_force_wrapper_dsf": [_vp] * 4,
+    "sofia_rdf_calc": [_vp, _i32, _i32, C.c_float, _i32, _vp, _vp],
     "sofia_get_cells": [_vp] * 7,
     "sofia_get_nlist": [_vp] * 4,
     "sofia_check_neighbor": [_vp, _vp],

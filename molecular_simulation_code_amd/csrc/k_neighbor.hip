@@ -259,6 +259,39 @@ __global__ void k_host_ids(size_t count, int32_t width, int32_t nidx, const int3
     }
 }
 
+// RDF_calc (Source/mod_force_nonbond.f90:708-775): one wavefront per atom.  An atom of type
+// itype walks its list row; each neighbour of type jtype with dr2 < rcut2 (FP64 distance, the
+// real*8 minimum image, no contraction in this file) adds 1 to bin int(sqrt(dr2) / delta).
+// The counts are integers, so the histogram does not depend on the order of the additions.
+__global__ void k_rdf(int32_t n, const float4 *__restrict__ pos4, const int32_t *__restrict__ rep,
+                      const RepInfo *__restrict__ reps, const int32_t *__restrict__ nlist,
+                      const int32_t *__restrict__ numneigh, int32_t na, double rcut2,
+                      double delta, int32_t itype, int32_t jtype, int32_t nbins,
+                      uint32_t *__restrict__ hist) {
+    const int i = blockIdx.x * (blockDim.x / 64) + (int)(threadIdx.x / 64);
+    if (i >= n) return;
+    const float4 xi = pos4[i];
+    if (__float_as_int(xi.w) != itype) return;   // uniform across the wavefront
+    const RepInfo R = reps[rep[i]];
+    const double bx = R.box, ib = R.ibox;
+    const double x1 = xi.x, y1 = xi.y, z1 = xi.z;
+    const int nn = min(numneigh[i], na);
+    const int32_t *row = nlist + (size_t)i * na;
+    for (int k = (int)(threadIdx.x & 63); k < nn; k += 64) {
+        const float4 pj = pos4[row[k]];
+        if (__float_as_int(pj.w) != jtype) continue;
+        double dx = x1 - (double)pj.x, dy = y1 - (double)pj.y, dz = z1 - (double)pj.z;
+        dx = dx - bx * __builtin_rint(dx * ib);
+        dy = dy - bx * __builtin_rint(dy * ib);
+        dz = dz - bx * __builtin_rint(dz * ib);
+        const double dr2 = dx * dx + dy * dy + dz * dz;
+        if (dr2 < rcut2) {
+            const int node = (int)(sqrt(dr2) / delta);
+            if (node < nbins) atomicAdd(&hist[node], 1u);
+        }
+    }
+}
+
 template <int WAVES>
 __global__ void __launch_bounds__(WAVES * 64)
 k_build_list(const float4 *__restrict__ pos4, const int32_t *__restrict__ num3,
@@ -466,6 +499,13 @@ void launch_remap_topology(sofia_ctx *c) {
     if (total)
         hipLaunchKernelGGL(k_remap_all, dim3(capped(total, 256)), dim3(256), 0, c->stream, S, c->tt,
                            c->rep, c->rebuild, any_flag(c));
+}
+
+void launch_rdf(sofia_ctx *c, int32_t itype, int32_t jtype, double delta, int32_t nbins,
+                uint32_t *hist) {
+    hipLaunchKernelGGL(k_rdf, dim3((c->N + 3) / 4), dim3(256), 0, c->stream, c->N, c->a.pos4,
+                       c->rep, c->reps_d, c->nlist, c->numneigh, c->neigh_alloc, c->dp.rcut2,
+                       delta, itype, jtype, nbins, hist);
 }
 
 void launch_host_ids(sofia_ctx *c, const int32_t *src, int32_t *dst, size_t items, int width,

@@ -754,7 +754,7 @@ int sofia_ctx_destroy(sofia_ctx *c) {
     dfree(c->bond_r0); dfree(c->angle_k); dfree(c->angle_th0); dfree(c->tors_k); dfree(c->tors_n);
     dfree(c->tors_shift); dfree(c->tors_cs); dfree(c->improlist); dfree(c->impro_k);
     dfree(c->impro_chi0); dfree(c->impro_dn); dfree(c->eacc); dfree(c->step_d); dfree(c->reps_d);
-    dfree(c->topo_stage); dfree(c->type_err); dfree(c->fix14);
+    dfree(c->topo_stage); dfree(c->type_err); dfree(c->fix14); dfree(c->rdf_hist);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1139,6 +1139,29 @@ int sofia_get_topology(sofia_ctx *c, float *q, int32_t *num1bond, int32_t *num2b
         TRY(sync_check(c));
         TRY(download(x.dst, c->topo_stage, sizeof(int32_t) * x.items * x.width));
     }
+    return SOFIA_OK;
+}
+
+int sofia_rdf_calc(sofia_ctx *c, int32_t itype, int32_t jtype, float rdf_delta, int32_t nbins,
+                   float *rdf, int32_t *numob_rdf) {
+    if (!c || !rdf) return fail(SOFIA_E_ARG, "null argument");
+    if (nbins < 1 || !(rdf_delta > 0.0f)) return fail(SOFIA_E_ARG, "nbins and rdf_delta must be positive");
+    if (!c->have_list) return fail(SOFIA_E_STATE, "no Verlet list: call sofia_neighbor_wrapper first");
+    if (nbins > c->rdf_cap) {
+        dfree(c->rdf_hist);
+        c->rdf_cap = 0;
+        TRY(dalloc(&c->rdf_hist, (size_t)nbins));
+        c->rdf_cap = nbins;
+    }
+    SOFIA_HIP(hipMemsetAsync(c->rdf_hist, 0, sizeof(uint32_t) * nbins, c->stream));
+    launch_rdf(c, itype, jtype, (double)rdf_delta, nbins, c->rdf_hist);
+    std::vector<uint32_t> h(nbins);
+    TRY(sync_check(c));
+    TRY(download(h.data(), c->rdf_hist, sizeof(uint32_t) * nbins));
+    // the reference adds 1.0 per pair to a real*4 bin; adding the count once is the same
+    // while a bin stays below 2^24
+    for (int k = 0; k < nbins; ++k) rdf[k] += (float)h[k];
+    if (numob_rdf) *numob_rdf += 1;
     return SOFIA_OK;
 }
 

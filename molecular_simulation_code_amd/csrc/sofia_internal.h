@@ -150,6 +150,8 @@ struct sofia_ctx {
     int32_t *improlist = nullptr, nimp = 0;   // (i1, i2, i3 central, i4, type) per improper
     int32_t *shake_num = nullptr, *shake_atoms = nullptr, ns = 0;
     int32_t *topo_stage = nullptr;   // topology in the host's id layout, on its way out
+    uint32_t *rdf_hist = nullptr;    // sofia_rdf_calc's counts (rdf_cap bins, lazy)
+    int32_t rdf_cap = 0;
     size_t topo_stage_len = 0;
     // engine constraints: indices of the iteratively solved clusters grouped by kind 1, 3,
     // 4, -3, and the count of symmetric rigid waters (SETTLE, found through AtomArrays::wclus);
@@ -241,6 +243,8 @@ void launch_gather(sofia_ctx *c, bool engine);
 void launch_remap_topology(sofia_ctx *c);
 void launch_host_ids(sofia_ctx *c, const int32_t *src, int32_t *dst, size_t items, int width,
                      int nidx);
+void launch_rdf(sofia_ctx *c, int32_t itype, int32_t jtype, double delta, int32_t nbins,
+                uint32_t *hist);
 void launch_build_list(sofia_ctx *c);
 void launch_build_list_engine(sofia_ctx *c);
 void launch_check_neighbor(sofia_ctx *c, bool engine);

@@ -142,6 +142,15 @@ module sofia_hip_iface
        integer(c_int32_t), value :: step
        integer(c_int32_t), intent(inout) :: tt(*)
      end function
+     integer(c_int) function sofia_rdf_calc(ctx, itype, jtype, delta, nbins, rdf, numob) &
+          bind(c, name="sofia_rdf_calc")
+       import :: c_int, c_ptr, c_int32_t, c_float
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: itype, jtype, nbins
+       real(c_float), value :: delta
+       real(c_float), intent(inout) :: rdf(*)
+       integer(c_int32_t), intent(inout) :: numob
+     end function
      integer(c_int) function sofia_get_topology(ctx, q, n1, n2, n3, spec, bl, al, dl, il, sa) &
           bind(c, name="sofia_get_topology")
        import :: c_int, c_ptr, c_int32_t, c_float
@@ -353,6 +362,19 @@ contains
     end do
     drx(:) = 0.0; dry(:) = 0.0; drz(:) = 0.0
   end subroutine neighbor_wrapper_hip
+
+  ! RDF_calc, Source/mod_force_nonbond.f90:708-775 (type 3 with type 3, as the reference),
+  ! on the list of the last neighbor_wrapper_hip: the k-th element of rdf receives bin k
+  ! (node k, from 0); bins past the end of rdf are not counted (`size` is a variable of
+  ! module global, so the extent comes from the bounds).
+  subroutine RDF_calc_hip
+    integer(c_int32_t) :: nob
+    nob = int(numob_rdf, c_int32_t)
+    call sofia_check(sofia_put_pos(ctx, position, c_null_ptr, c_null_ptr, c_null_ptr), 'put_state')
+    call sofia_check(sofia_rdf_calc(ctx, 3_c_int32_t, 3_c_int32_t, rdf_delta, &
+         int(ubound(rdf, 1) - lbound(rdf, 1) + 1, c_int32_t), rdf, nob), 'rdf_calc')
+    numob_rdf = nob
+  end subroutine RDF_calc_hip
 
   subroutine check_neighbor_hip(nflag, step)
     integer :: nflag, step

@@ -185,6 +185,20 @@ def dsf(pos, types, q, nlist, numneigh, neigh_alloc, lj, prm: OrcParams):
     return ff, ff64, pot.value, ec.value, vir
 
 
+def rdf_calc(pos, types, nlist, numneigh, neigh_alloc, box, ibox, rcut2, delta, itype, jtype,
+             nbins, numob=0):
+    """orc_rdf_calc: returns (rdf float32 (nbins,), numob_rdf)."""
+    a = ainfo(pos, types)
+    rdf = np.zeros(nbins, _f)
+    nob = C.c_int32(numob)
+    lib().orc_rdf_calc(C.c_int32(len(a)), P(a), P(np.ascontiguousarray(nlist, _i)),
+                       P(np.ascontiguousarray(numneigh, _i)), C.c_int32(neigh_alloc),
+                       C.c_float(box), C.c_float(ibox), C.c_double(rcut2), C.c_float(delta),
+                       C.c_int32(itype), C.c_int32(jtype), C.c_int32(nbins), P(rdf),
+                       C.byref(nob))
+    return rdf, nob.value
+
+
 def dsf_x64(xyz, types, q, nlist, numneigh, neigh_alloc, lj, prm: OrcParams):
     """orc_dsf on FP64 positions (n, 3): returns (ff64, potential, e_coul, virial)."""
     n = len(xyz)
@@ -442,6 +456,14 @@ class Ref:
         self.L.ref_dsf(C.c_int32(neigh_alloc), P(np.ascontiguousarray(nlist, _i)),
                        P(np.ascontiguousarray(numneigh, _i)), P(f), C.byref(e1), C.byref(e2))
         return f, e1.value, e2.value
+
+    def rdf(self, nlist, numneigh, neigh_alloc, delta, nbins, numob=0):
+        out = np.zeros(nbins, _f)
+        nob = C.c_int32(numob)
+        self.L.ref_rdf(C.c_int32(neigh_alloc), P(np.ascontiguousarray(nlist, _i)),
+                       P(np.ascontiguousarray(numneigh, _i)), C.c_float(delta), C.c_int32(nbins),
+                       P(out), C.byref(nob))
+        return out, nob.value
 
     def coul_long(self, nlist, numneigh, neigh_alloc):
         f = np.zeros((self.n, 3), _f)

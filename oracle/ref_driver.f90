@@ -105,6 +105,28 @@ contains
     epot = potential; ecoul = e_coul
   end subroutine ref_dsf
 
+  ! RDF_calc (Source/mod_force_nonbond.f90:708-775) on a given Verlet list: rdf is
+  ! allocated as rdf(0:nb-1) and zeroed, so out(k+1) is the count of bin k; numob_rdf in and
+  ! out.  The caller sizes nb past sqrt(rcut2)/delta (the routine has no bound check).
+  subroutine ref_rdf(na, nl, nn, delta, nb, out, nob) bind(c, name="ref_rdf")
+    integer(c_int), value :: na, nb
+    integer(c_int), intent(in) :: nl(na*np), nn(np)
+    real(c_float), value :: delta
+    real(c_float), intent(out) :: out(nb)
+    integer(c_int), intent(inout) :: nob
+    if (allocated(nlist)) deallocate(nlist, numneigh)
+    allocate(nlist(na*np), numneigh(np))
+    nlist = nl; numneigh = nn; neigh_alloc = na
+    if (allocated(rdf)) deallocate(rdf)
+    allocate(rdf(0:nb-1))
+    rdf = 0.0
+    rdf_delta = delta
+    numob_rdf = nob
+    call RDF_calc
+    out(1:nb) = rdf(0:nb-1)
+    nob = numob_rdf
+  end subroutine ref_rdf
+
   ! lj_cut_coul_long_nonewton_NOMIC (Source/mod_force_nonbond.f90:300-400).
   subroutine ref_coul_long(na, nl, nn, fout, epot, ecoul, vir) bind(c, name="ref_coul_long")
     integer(c_int), value :: na

@@ -348,6 +348,35 @@ void orc_dsf_x64(int32_t np, const double *xyz, const int32_t *type, const float
 /* lj_cut_coul_long_nonewton_NOMIC, Source/mod_force_nonbond.f90:300-400: the PME
  * real-space kernel with real*4 locals (each assignment rounds to float), erfc from the
  * A&S polynomial with gewald, double accumulators for force, energy and virial. */
+/* RDF_calc, Source/mod_force_nonbond.f90:708-775: over the Verlet list (1-based ids, stride
+ * neigh_alloc), every atom i of type itype_sel and neighbour j of type jtype_sel with
+ * dr2 < rcut2 (FP64 distance, real*8 minimum image) adds 1 to rdf[node],
+ * node = int(sqrt(dr2) / rdf_delta).  The reference has no bound on node; here nodes
+ * >= nbins are not counted.  numob_rdf is incremented once per call. */
+void orc_rdf_calc(int32_t np, const orc_ainfo *pos, const int32_t *nlist,
+                  const int32_t *numneigh, int32_t neigh_alloc, float box, float ibox,
+                  double rcut2, float rdf_delta, int32_t itype_sel, int32_t jtype_sel,
+                  int32_t nbins, float *rdf, int32_t *numob_rdf) {
+    *numob_rdf += 1;
+    for (int32_t i = 0; i < np; ++i) {
+        if (pos[i].type != itype_sel) continue;
+        const double x1 = pos[i].x, y1 = pos[i].y, z1 = pos[i].z;
+        const int32_t *row = nlist + (size_t)i * neigh_alloc;
+        for (int32_t l = 0; l < numneigh[i]; ++l) {
+            const orc_ainfo *pj = &pos[row[l] - 1];
+            if (pj->type != jtype_sel) continue;
+            const double dx = minimg_f64(x1 - (double)pj->x, box, ibox);
+            const double dy = minimg_f64(y1 - (double)pj->y, box, ibox);
+            const double dz = minimg_f64(z1 - (double)pj->z, box, ibox);
+            const double dr2 = dx * dx + dy * dy + dz * dz;
+            if (dr2 < rcut2) {
+                const int32_t node = (int32_t)(sqrt(dr2) / (double)rdf_delta);
+                if (node < nbins) rdf[node] += 1.0f;
+            }
+        }
+    }
+}
+
 void orc_coul_long(int32_t np, const orc_ainfo *pos, const float *q, const int32_t *nlist,
                    const int32_t *numneigh, int32_t neigh_alloc, const float *lj1,
                    const float *lj2, const float *lj3, const float *lj4, const orc_params *p,

@@ -67,6 +67,11 @@ void orc_dsf(int32_t np, const orc_ainfo *pos, const float *q, const int32_t *nl
              const int32_t *numneigh, int32_t neigh_alloc, const float *lj1, const float *lj2,
              const float *lj3, const float *lj4, const orc_params *p, orc_coords *ff,
              double *ff64, double *potential, double *e_coul, double *virial);
+/* RDF_calc, Source/mod_force_nonbond.f90:708-775 (bins past nbins are not counted) */
+void orc_rdf_calc(int32_t np, const orc_ainfo *pos, const int32_t *nlist,
+                  const int32_t *numneigh, int32_t neigh_alloc, float box, float ibox,
+                  double rcut2, float rdf_delta, int32_t itype_sel, int32_t jtype_sel,
+                  int32_t nbins, float *rdf, int32_t *numob_rdf);
 /* orc_dsf on FP64 positions (xyz: x, y, z per atom; type per atom): the same arithmetic
  * from the promotion of the coordinates on, so on real*4-representable positions it equals
  * orc_dsf bit for bit.  ff64 receives the per-atom sums. */

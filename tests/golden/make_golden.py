@@ -25,6 +25,7 @@ can tell a list change from a force change.
 from __future__ import annotations
 
 import hashlib
+import math
 import os
 import subprocess
 import sys
@@ -112,6 +113,10 @@ def main():
         g["long_f"], g["long_pot"], g["long_ecoul"], g["long_vir"] = R.coul_long(nl, nn, na)
         for kind, lst in ((12, g["bondlist"]), (13, g["anglelist"]), (14, g["dihedlist"])):
             g[f"corr{kind}_f"], g[f"corr{kind}_e"], g[f"corr{kind}_vir"] = R.correct(kind, lst)
+        # RDF_calc counts type-3 pairs (its hard-wired selection) into 0.05 A bins
+        g["rdf_delta"] = np.float32(0.05)
+        nbins = int(math.sqrt(d["rcut2"]) / 0.05) + 2
+        g["rdf"], g["rdf_numob"] = R.rdf(nl, nn, na, g["rdf_delta"], nbins, numob=4)
         g["p14_f"], g["p14_ecoul"], g["p14_pot"] = R.pair14(g["dihedlist"])
         g["n2_f"], g["n2_pot"], g["n2_ecoul"] = R.dsf_n2(g["num1bond"], g["num2bond"],
                                                        g["num3bond"], g["specbond"])

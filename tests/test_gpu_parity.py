@@ -449,3 +449,29 @@ def test_error_paths_are_reported_not_faulted():
         assert np.array_equal(now, good)
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("ti,tj", [(3, 3), (11, 12), (4, 11)])
+def test_rdf_calc_bit_exact(case, gpu_sys, ti, tj):
+    """sofia_rdf_calc (RDF_calc, Source/mod_force_nonbond.f90:708-775) on the device's list
+    equals the oracle's histogram on the oracle's list, bin for bin; a second call adds to
+    the bins and counts the call; a short histogram drops the bins past its end."""
+    import ctypes as C
+    import math
+    from molecular_simulation_code_amd._lib import check, lib, ptr
+    name, s, d, ref = case
+    g, _ = gpu_sys
+    delta = 0.05
+    nbins = int(math.sqrt(d["rcut2"]) / delta) + 2
+    want, _ = O.rdf_calc(ref["pos"], ref["types"], ref["nl"], ref["nn"], ref["na"], d["box"],
+                         d["ibox"], d["rcut2"], delta, ti, tj, nbins)
+    rdf = np.zeros(nbins, np.float32)
+    nob = C.c_int32(7)
+    check(lib().sofia_rdf_calc(g.ctx, ti, tj, C.c_float(delta), nbins, ptr(rdf), C.byref(nob)))
+    assert np.array_equal(rdf, want) and nob.value == 8
+    check(lib().sofia_rdf_calc(g.ctx, ti, tj, C.c_float(delta), nbins, ptr(rdf), C.byref(nob)))
+    assert np.array_equal(rdf, 2 * want) and nob.value == 9
+    short = np.zeros(40, np.float32)
+    check(lib().sofia_rdf_calc(g.ctx, ti, tj, C.c_float(delta), 40, ptr(short), None))
+    assert np.array_equal(short, want[:40])
+    say(f"[rdf {name}] types {ti}-{tj}: {int(want.sum())} pairs in {np.count_nonzero(want)} bins")

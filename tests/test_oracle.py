@@ -756,3 +756,59 @@ def test_molecule_centers_are_the_unwrapped_means():
     assert (com >= 0).all() and (com <= 30.0).all()
     com2 = O.molecule_centers(x, gid, mol, nmol, box)
     assert np.array_equal(com, com2)
+
+
+# ------------------------------------------------------------------ RDF_calc
+def _golden_list_1based(g, d):
+    nl, nn, na = _list(g, d)
+    return nl, nn, na
+
+
+def test_golden_rdf_bit_exact(golden):
+    """orc_rdf_calc on the golden list equals the reference's own RDF_calc (type 3 with
+    type 3, Source/mod_force_nonbond.f90:708-775) bin for bin, and counts the call."""
+    d = _derived(golden)
+    nl, nn, na = _golden_list_1based(golden, d)
+    ref = golden["rdf"]
+    out, nob = O.rdf_calc(golden["pos"], golden["types"], nl, nn, na, d["box"], d["ibox"],
+                          d["rcut2"], float(golden["rdf_delta"]), 3, 3, len(ref), numob=4)
+    assert nob == int(golden["rdf_numob"]) == 5
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("ti,tj", [(3, 3), (11, 12), (12, 11), (4, 11)])
+def test_rdf_equals_bruteforce_histogram(golden, ti, tj):
+    """The list-based histogram equals one over every ordered pair of the two types that
+    specbond does not exclude, closer than rcut (numpy, FP64 minimum image)."""
+    d = _derived(golden)
+    nl, nn, na = _golden_list_1based(golden, d)
+    pos = golden["pos"].astype(np.float64)
+    types = golden["types"]
+    box, ibox = float(d["box"]), float(np.float32(d["ibox"]))
+    delta = 0.05
+    nbins = int(math.sqrt(d["rcut2"]) / delta) + 2
+    out, _ = O.rdf_calc(golden["pos"], types, nl, nn, na, d["box"], d["ibox"], d["rcut2"],
+                        delta, ti, tj, nbins)
+    want = np.zeros(nbins, np.int64)
+    ii = np.nonzero(types == ti)[0]
+    jj = np.nonzero(types == tj)[0]
+    for i in ii:
+        dx = pos[i] - pos[jj]
+        dx = dx - box * np.rint(dx * ibox)
+        dr2 = (dx * dx).sum(axis=1)
+        ex = set(int(v) - 1 for v in golden["specbond"][i, :golden["num3bond"][i]])
+        keep = (dr2 < d["rcut2"]) & (jj != i) & np.array([j not in ex for j in jj])
+        node = (np.sqrt(dr2[keep]) / np.float64(np.float32(delta))).astype(np.int64)
+        np.add.at(want, node[node < nbins], 1)
+    assert want.sum() > 0 or ti == 3
+    assert np.array_equal(out.astype(np.int64), want)
+
+
+def test_rdf_bins_past_the_end_are_dropped(golden):
+    d = _derived(golden)
+    nl, nn, na = _golden_list_1based(golden, d)
+    full, _ = O.rdf_calc(golden["pos"], golden["types"], nl, nn, na, d["box"], d["ibox"],
+                         d["rcut2"], 0.05, 11, 12, 400)
+    cut, _ = O.rdf_calc(golden["pos"], golden["types"], nl, nn, na, d["box"], d["ibox"],
+                        d["rcut2"], 0.05, 11, 12, 30)
+    assert np.array_equal(cut, full[:30]) and full[30:].sum() > 0
