@@ -65,8 +65,9 @@ def traffic_bytes():
 
 
 def valu_counters():
-    """VALU busy % and the FP32 / FP64 / integer instruction mix of the pair kernel on the
-    batched workload, from the committed rocprofv3 counter summary, or None."""
+    """VALU issue rate (instructions over the one-per-CU-per-cycle ceiling) and the FP32 /
+    FP64 / integer instruction mix of the pair kernel on the batched workload, from the
+    committed rocprofv3 counter summary, or None."""
     return _profile_json("pair_kernel_batched64_valu.json")
 
 
