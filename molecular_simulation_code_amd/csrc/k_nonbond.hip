@@ -26,6 +26,26 @@
 namespace sofia {
 namespace {
 
+// exp(x) for the FP64 DSF Gaussian (x <= 0), table-driven (Tang's method): x = (64 n + k)
+// ln2/64 + r, |r| <= ln2/128, with ln2/64 split in two parts (the first with 12 trailing zero
+// bits, so n ln2/64 is exact for |x| < 44), and exp(x) = 2^n T[k] (1 + p(r)), T[k] = 2^(k/64)
+// from LDS and p the degree-6 Taylor polynomial of e^r - 1 (truncation below 1e-19).  Within
+// 1 ulp of the C library's exp over [-60, 0] (4.4e7 points checked on the host).
+__device__ inline double exp_tab(double x, const double *__restrict__ T) {
+    const double N = __builtin_rint(x * 92.33248261689366);   // 64 / ln 2
+    const int m = (int)N;
+    double r = __builtin_fma(N, -0.010830424696244734, x);
+    r = __builtin_fma(N, -4.411764150473684e-15, r);
+    double p = __builtin_fma(r, 1.0 / 720.0, 1.0 / 120.0);
+    p = __builtin_fma(r, p, 1.0 / 24.0);
+    p = __builtin_fma(r, p, 1.0 / 6.0);
+    p = __builtin_fma(r, p, 0.5);
+    p = __builtin_fma(r, p, 1.0);
+    p = p * r;
+    const double t = T[m & 63];
+    return __builtin_ldexp(__builtin_fma(t, p, t), m >> 6);
+}
+
 template <int L, int MODE, bool ENERGY>
 __global__ void __launch_bounds__(256)
 k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ q,
@@ -33,12 +53,24 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
           const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh, int32_t na,
           const float4 *__restrict__ ljtab, const float4 *__restrict__ tab, DevParams P,
           double *__restrict__ fnb, float *__restrict__ ff32, long long *__restrict__ eacc) {
-    // LDS: the LJ table, sized at launch
+    // LDS, sized at launch: the LJ table (FP64 mode: as doubles, then the 64 values 2^(k/64)
+    // of exp_tab)
     static_assert(MODE == 0 || MODE == 2, "mode 1 is k_dsf_mixed");
     extern __shared__ float4 s_dyn[];
     float4 *s_lj = s_dyn;
+    double2 *s_lj64 = (double2 *)s_dyn;                    // 2 per type pair: (lj1, lj2), (lj3, lj4)
     const int nt2 = P.ntypes * P.ntypes;
-    for (int t = threadIdx.x; t < nt2; t += blockDim.x) s_lj[t] = ljtab[t];
+    double *s_exp = (double *)(s_lj64 + 2 * nt2);
+    for (int t = threadIdx.x; t < nt2; t += blockDim.x) {
+        const float4 v = ljtab[t];
+        if (MODE == 0) {
+            s_lj64[2 * t] = make_double2(v.x, v.y);
+            s_lj64[2 * t + 1] = make_double2(v.z, v.w);
+        } else {
+            s_lj[t] = v;
+        }
+    }
+    if (MODE == 0 && threadIdx.x < 64) s_exp[threadIdx.x] = exp2((double)threadIdx.x / 64.0);
     __syncthreads();
 
     const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -54,6 +86,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
     const int nn = live ? min(numneigh[ii], na) : 0;   // a row never reads past its stride
     const int32_t *lst = nlist + (size_t)ii * na;
     const float4 *ljrow = s_lj + ti * P.ntypes - 1;   // index by jtype (1-based)
+    const double2 *ljrow64 = s_lj64 + 2 * (ti * P.ntypes - 1);
 
     double fxd = 0.0, fyd = 0.0, fzd = 0.0;
     double pot = 0.0, ecl = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
@@ -89,11 +122,12 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
             const double dr2i = dri * dri;
             double dr6i = dr2i * dr2i * dr2i;
             if (dr2 > P.rcut2) dr6i = 0.0;
-            const float4 lj = ljrow[__float_as_int(pj.w)];
-            const double flj = dr6i * ((double)lj.x * dr6i - (double)lj.y);
+            const int tj2 = 2 * __float_as_int(pj.w);
+            const double2 l12 = ljrow64[tj2];
+            const double flj = dr6i * (l12.x * dr6i - l12.y);
             double pref = qc * (double)qj * dri;
             if (dr2 > P.cut_coulsq) pref = 0.0;
-            const double erfcd = exp(-a2 * dr2);
+            const double erfcd = exp_tab(-a2 * dr2, s_exp);
             const double t = rcp_f64(1.0 + pa * r);
             const double erfcc =
                 t * (P.a1 + t * (P.a2 + t * (P.a3 + t * (P.a4 + t * P.a5)))) * erfcd;
@@ -103,7 +137,8 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
             fyd += dy * force;
             fzd += dz * force;
             if (ENERGY) {
-                pot += dr6i * (dr6i * (double)lj.z - (double)lj.w);
+                const double2 l34 = ljrow64[tj2 + 1];
+                pot += dr6i * (dr6i * l34.x - l34.y);
                 ecl += pref * (erfcc - r * P.e_shift - dr2 * P.f_shift);
                 vx += dx * dx * force;
                 vy += dy * dy * force;
@@ -198,7 +233,8 @@ void launch_t(sofia_ctx *c, bool energies, float *ff32) {
     const long long threads = (long long)c->N * L;
     const int grid = (int)((threads + block - 1) / block);
     const int nt2 = c->dp.ntypes * c->dp.ntypes;
-    const size_t lds = sizeof(float4) * nt2;
+    const size_t lds = MODE == 0 ? 2 * sizeof(double2) * nt2 + 64 * sizeof(double)
+                                 : sizeof(float4) * nt2;
     if (energies)
         hipLaunchKernelGGL((k_nonbond<L, MODE, true>), dim3(grid), dim3(block), lds, c->stream, c->N,
                            c->a.pos4, c->a.q, c->rep, c->reps_d, c->nlist, c->numneigh,
