@@ -184,7 +184,7 @@ __global__ void k_fixpos(int32_t n, const double *__restrict__ xd, const float4 
 
 // L lanes per atom; lane s of a group walks list entries s, s+L, ... with the loads of the
 // next entry in flight during this entry's math.  Blocks are dealt to the XCDs in runs.
-template <int L, bool ENERGY, bool QT, bool PADDED>
+template <int L, bool ENERGY, bool QT>
 __global__ void __launch_bounds__(256)
 k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
@@ -207,7 +207,6 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     for (int t = threadIdx.x; t < (ENERGY ? 2 : 1) * (tn + 1); t += blockDim.x) s_tab[t] = tab[t];
     __syncthreads();
     constexpr uint32_t RB = QT ? 12u : 16u;   // record bytes
-    static_assert(!PADDED || 3 * L <= LIST_PAD, "the loop evaluates 3 L entries past a padded row");
 
     const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = gt / L;
@@ -265,7 +264,7 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const uint32_t j0 = (PADDED || sub + u * L < nn) ? JJ[u] : (uint32_t)ii;
+        const uint32_t j0 = (sub + u * L < nn) ? JJ[u] : (uint32_t)ii;
         PJ[u] = QT ? ld96(prs, rec_off<QT>(j0)) : ld128(prs, rec_off<QT>(j0));
         JJ[u] = ld32(lrs, off + (u + 4) * step);
         __builtin_amdgcn_sched_barrier(0);
@@ -280,8 +279,6 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
         const float dx = sep_f(xi, (int)((uint32_t)pj.x & ~TYPE_BITS));
         const float dy = sep_f(pi.y, pj.y), dz = sep_f(pi.z, pj.z);
         const float dr2 = norm2_f(dx, dy, dz) * s2;
-        // padded rows: entries past the row are the atom itself, at zero distance
-        if (PADDED && !DIR) valid = dr2 > 0.0f;
         const float4 lj = ljrow[tj];
         const float qq = QT ? lj.z : qc * __int_as_float(pj.w);
         const PairOut o = pair_dsf<ENERGY, DIR>(dr2, qq, lj, s_tab, tinv, u0, tn, rcut2, D);
@@ -313,9 +310,9 @@ k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             entry(PJ[u], k + u * L < nn, std::false_type{});
-            const uint32_t jg = (PADDED || k + (u + 4) * L < nn) ? JJ[u] : (uint32_t)ii;
+            const uint32_t jg = (k + (u + 4) * L < nn) ? JJ[u] : (uint32_t)ii;
             PJ[u] = QT ? ld96(prs, rec_off<QT>(jg)) : ld128(prs, rec_off<QT>(jg));
-            JJ[u] = ld32(lrs, (PADDED || k + (u + 8) * L < nn) ? off + u * step : row0);
+            JJ[u] = ld32(lrs, (k + (u + 8) * L < nn) ? off + u * step : row0);
             // VALU, SALU and LDS work may cross; memory loads stay in slot order
             __builtin_amdgcn_sched_barrier(0x0086);
         }
@@ -501,7 +498,7 @@ k_dsf_compact(int32_t n, const int4 *__restrict__ pf,
     }
 }
 
-template <int L, bool PADDED>
+template <int L>
 void launch_lanes(sofia_ctx *c, bool energies, float *ff32, bool qt) {
     const int block = c->block;
     const long long threads = (long long)c->N * L;
@@ -509,8 +506,7 @@ void launch_lanes(sofia_ctx *c, bool energies, float *ff32, bool qt) {
     const int nt2 = c->dp.ntypes * c->dp.ntypes;
     const size_t lds = sizeof(float4) * ((qt ? 2 : 1) * nt2 + (energies ? 2 : 1) * (c->dp.tab_n + 1));
 #define DSF_LAUNCH(E, Q)                                                                          \
-    hipLaunchKernelGGL((k_dsf_mixed<L, E, Q, PADDED>), dim3(grid), dim3(block), lds, c->stream,     \
-                       c->N,                                                                     \
+    hipLaunchKernelGGL((k_dsf_mixed<L, E, Q>), dim3(grid), dim3(block), lds, c->stream, c->N,       \
                        c->pfix, c->rep, c->reps_d, c->nlist, c->numneigh, c->neigh_alloc,          \
                        c->ljtab, c->ptab, c->dsf_tab, c->dp, c->fnb, ff32, c->eacc)
     if (energies) {
@@ -545,25 +541,13 @@ void launch_dsf_mixed(sofia_ctx *c, bool energies, float *ff32, bool from_xd) {
     hipLaunchKernelGGL(k_fixpos, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, c->N,
                        from_xd ? c->a.xd : nullptr, c->a.pos4, qt ? nullptr : c->a.q, c->rep,
                        c->reps_d, c->pfix);
-    // engine lists (every replica on the half-width grid) carry LIST_PAD own-index entries
-    // after each row, enough for up to 16 lanes to prefetch without clamps
-    const bool padded = c->nrep > 0 && c->n_stencil5 == c->nrep;
     switch (c->lanes) {
         case 0: launch_compact(c, energies, ff32); break;
-        case 4:
-            if (padded) launch_lanes<4, true>(c, energies, ff32, qt);
-            else launch_lanes<4, false>(c, energies, ff32, qt);
-            break;
-        case 8:
-            if (padded) launch_lanes<8, true>(c, energies, ff32, qt);
-            else launch_lanes<8, false>(c, energies, ff32, qt);
-            break;
-        case 32: launch_lanes<32, false>(c, energies, ff32, qt); break;
-        case 64: launch_lanes<64, false>(c, energies, ff32, qt); break;
-        default:
-            if (padded) launch_lanes<16, true>(c, energies, ff32, qt);
-            else launch_lanes<16, false>(c, energies, ff32, qt);
-            break;
+        case 4: launch_lanes<4>(c, energies, ff32, qt); break;
+        case 8: launch_lanes<8>(c, energies, ff32, qt); break;
+        case 32: launch_lanes<32>(c, energies, ff32, qt); break;
+        case 64: launch_lanes<64>(c, energies, ff32, qt); break;
+        default: launch_lanes<16>(c, energies, ff32, qt); break;
     }
 }
 

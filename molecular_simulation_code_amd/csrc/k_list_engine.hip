@@ -194,11 +194,8 @@ k_build_engine(int32_t ncell, const float4 *__restrict__ pos4, const int32_t *__
                 }
                 if (lane == 0) {
                     numneigh[i] = cnt;
-                    if (g1 >= total && cnt + LIST_PAD > na) atomicMax(maxcnt, cnt + LIST_PAD);
+                    if (g1 >= total && cnt > na) atomicMax(maxcnt, cnt);
                 }
-                if (g1 >= total)   // the row's own-index padding (LIST_PAD)
-                    for (int p = lane; p < LIST_PAD; p += 64)
-                        if (cnt + p < na) out[cnt + p] = i;
             }
             if (g1 >= total) break;
         }

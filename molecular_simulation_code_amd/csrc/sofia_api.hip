@@ -158,12 +158,11 @@ int grow_alloc(int mx) {   // Source/mod_neighbor.f90:260-263
     return na;
 }
 
-// The engine's list stride for a largest count mx: headroom x mx plus the rows' LIST_PAD
-// padding, rounded up to 16 entries (mx + LIST_PAD at headroom 1, which leaves no room at
-// all), never below the floor a redo set.
+// The engine's list stride for a largest count mx: headroom x mx rounded up to 16 entries
+// (exactly mx at headroom 1, which leaves no room at all), never below the floor a redo set.
 int engine_stride(const sofia_ctx *c, int mx) {
-    int na = mx + LIST_PAD;   // rows carry LIST_PAD own-index entries after their neighbours
-    if (c->headroom > 1.0) na = (((int)std::ceil((double)mx * c->headroom)) + LIST_PAD + 15) / 16 * 16;
+    int na = mx;
+    if (c->headroom > 1.0) na = (((int)std::ceil((double)mx * c->headroom)) + 15) / 16 * 16;
     return std::max({64, na, c->stride_floor});
 }
 
@@ -522,8 +521,7 @@ int rebuild_now(sofia_ctx *c, bool engine, bool reference_growth) {
             c->have_list = true;
             return sync_check(c);
         }
-        // an engine row's count came back with its LIST_PAD padding added
-        int na = reference_growth ? grow_alloc(mx) : engine_stride(c, mx - LIST_PAD);
+        int na = reference_growth ? grow_alloc(mx) : engine_stride(c, mx);
         TRY(ensure_list(c, na));
     }
     return fail(SOFIA_E_OVERFLOW, "Verlet list kept overflowing");

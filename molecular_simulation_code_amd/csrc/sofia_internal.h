@@ -12,12 +12,6 @@
 namespace sofia {
 
 constexpr int SPEC = 12;                     // specbond slots per atom
-// Entries after each row of an engine list holding the atom's own index: the mixed pair
-// kernel (up to 16 lanes per atom) then prefetches past a row's end without clamps.  With 16
-// lanes it evaluates (and drops: zero distance) up to 3 x 16 entries past the end; the
-// records it gathers further out are never evaluated, and its buffer loads keep every read
-// inside the arrays.
-constexpr int LIST_PAD = 48;
 constexpr double FIX_SCALE = 4294967296.0;   // forces: 2^32 units per kcal/mol/A
 constexpr double FIX_INV = 1.0 / FIX_SCALE;
 constexpr double EFIX_SCALE = 16777216.0;    // energies and virial: 2^24 units per kcal/mol
