@@ -1,6 +1,6 @@
-"""The serving loop's host logic on the CPU: admission by HBM footprint, continuous batching
+"""Host logic on the CPU.  The serving loop: admission by HBM footprint, continuous batching
 (cohorts that form at segment boundaries, finish, re-batch their survivors and merge past
-max_cohorts), and complete results.  The GPU engine is replaced by a stand-in that records
+max_cohorts), and complete results; and the autotuner's cache of tunings.  The GPU engine is replaced by a stand-in that records
 what the server asks of it, so these tests need no device; the GPU tests
 (tests/test_gpu_engine.py, tests/test_gpu_configs.py) run the same loop on the real engine."""
 import numpy as np
@@ -131,3 +131,35 @@ def test_a_request_waits_until_the_resident_footprint_leaves_room(fake):
     assert len(res) == 2
     assert fake.peak == 10000                 # the second waited for the first to finish
     assert res[1].admitted >= res[0].finished
+
+
+def test_autotune_cache_survives_the_process_and_buckets_by_size_and_density(monkeypatch, tmp_path):
+    """The autotuner's cache: a bucket written by one process (SOFIA_TUNE_CACHE names a JSON
+    file) is read back by the next; the key separates replica count, size and density on a
+    coarse scale and is shared within a bucket."""
+    from molecular_simulation_code_amd import host as H
+    path = tmp_path / "tune.json"
+    monkeypatch.setenv("SOFIA_TUNE_CACHE", str(path))
+    monkeypatch.setattr(H, "_TUNE_CACHE", {})
+    H._tune_cache_put("k1", (1.6, 16, 2.4, 16, 1, 1))
+    monkeypatch.setattr(H, "_TUNE_CACHE", {})          # a fresh process
+    assert H._tune_cache_get("k1") == (1.6, 16, 2.4, 16, 1, 1)
+    assert H._tune_cache_get("k2") is None
+    path.write_text("not json")                         # a damaged file is ignored
+    monkeypatch.setattr(H, "_TUNE_CACHE", {})
+    assert H._tune_cache_get("k1") is None
+
+    class _P:
+        precision, coul_flag = 1, 0
+
+    def key(natoms, box, nrep=1):
+        e = H.Engine.__new__(H.Engine)
+        e.params = _P()
+        e.reps = [H.ReplicaSpec(type("Sys", (), {"box": box})()) for _ in range(nrep)]
+        e.N = natoms * nrep
+        return e.tune_key()
+
+    assert key(204000, 127.3) == key(205000, 127.5)     # same bucket
+    assert key(204000, 127.3) != key(1020000, 217.7)    # another size
+    assert key(204000, 127.3) != key(204000, 127.3, nrep=4)
+    assert key(204000, 127.3) != key(204000, 140.0)     # another density
