@@ -370,7 +370,7 @@ def profile_run(args, workers):
         if args.tuning:   # the timed run's tuning, given instead of autotuned
             knobs = [float(x) for x in args.tuning.split(",")]
             eng.set_skin(knobs[0])
-            eng.set_tuning(int(knobs[1]), 256, int(knobs[2]))
+            eng.set_tuning(int(knobs[1]), host.PAIR_BLOCK, int(knobs[2]))
             if len(knobs) > 3:
                 eng.set_bonded_mode(int(knobs[3]))
             if len(knobs) > 4:
@@ -570,7 +570,7 @@ def main():
             e1.set_steps([snap[2][k]])
             if tune_b is not None:
                 e1.set_skin(tune_b[0])
-                e1.set_tuning(tune_b[1], 256, tune_b[3])
+                e1.set_tuning(tune_b[1], host.PAIR_BLOCK, tune_b[3])
                 e1.set_bonded_mode(tune_b[4])
                 e1.set_graph_steps(tune_b[5])
             e1.resume([snap[1][k]])

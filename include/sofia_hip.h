@@ -357,7 +357,8 @@ int sofia_profile_step(sofia_ctx *ctx, int32_t nsteps, int32_t full_rebuild, dou
 /* Tuning knobs (autotuner): lanes per atom in the pair kernel (4..64, power of two; 0
  * selects the mixed-precision kernel that compacts the in-cutoff entries before the force
  * math, one atom per wavefront -- FP64 and Ewald modes then use 16), threads per block of
- * the pair kernel, waves per block in the list build. */
+ * the mixed pair kernel (64..1024 in steps of 64; the FP64 and Ewald kernels take at most
+ * 256 of them), waves per block in the list build. */
 int sofia_set_tuning(sofia_ctx *ctx, int32_t lanes_per_atom, int32_t block, int32_t list_waves);
 /* Engine Verlet skin in A (list radius rc + skin; default: the reference's pad).  The
  * engine's lists hold every pair inside the cutoff whatever the skin, so the skin changes

@@ -185,7 +185,7 @@ __global__ void k_fixpos(int32_t n, const double *__restrict__ xd, const float4 
 // L lanes per atom; lane s of a group walks list entries s, s+L, ... with the loads of the
 // next entry in flight during this entry's math.  Blocks are dealt to the XCDs in runs.
 template <int L, bool ENERGY, bool QT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_dsf_mixed(int32_t n, const int4 *__restrict__ pf,
             const int32_t *__restrict__ rep, const RepInfo *__restrict__ reps,
             const int32_t *__restrict__ nlist, const int32_t *__restrict__ numneigh, int32_t na,

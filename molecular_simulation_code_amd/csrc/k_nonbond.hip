@@ -229,7 +229,7 @@ k_nonbond(int32_t n, const float4 *__restrict__ pos4, const float *__restrict__ 
 
 template <int L, int MODE>
 void launch_t(sofia_ctx *c, bool energies, float *ff32) {
-    const int block = c->block;
+    const int block = std::min(c->block, 256);   // this kernel's launch bound
     const long long threads = (long long)c->N * L;
     const int grid = (int)((threads + block - 1) / block);
     const int nt2 = c->dp.ntypes * c->dp.ntypes;

@@ -2265,7 +2265,7 @@ int sofia_set_tuning(sofia_ctx *c, int32_t lanes, int32_t block, int32_t list_wa
     if (!c) return fail(SOFIA_E_ARG, "null context");
     if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
         return fail(SOFIA_E_ARG, "lanes_per_atom must be 0 (compacting kernel), 4, 8, 16, 32 or 64");
-    if (block < 64 || block > 256 || block % 64) return fail(SOFIA_E_ARG, "block must be 64, 128, 192 or 256");
+    if (block < 64 || block > 1024 || block % 64) return fail(SOFIA_E_ARG, "block must be a multiple of 64 up to 1024");
     if (list_waves != 2 && list_waves != 4 && list_waves != 8 && list_waves != 16)
         return fail(SOFIA_E_ARG, "list_waves must be 2, 4, 8 or 16");
     c->lanes = lanes;

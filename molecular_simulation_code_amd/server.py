@@ -34,7 +34,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import system as S
-from .host import PREC_MIXED, Engine, ReplicaSpec
+from .host import PAIR_BLOCK, PREC_MIXED, Engine, ReplicaSpec
 
 
 @dataclass
@@ -119,7 +119,7 @@ class SegmentServer:
         if self.tuning:
             skin, lanes, waves = self.tuning[:3]
             eng.set_skin(skin)
-            eng.set_tuning(lanes, 256, waves)
+            eng.set_tuning(lanes, PAIR_BLOCK, waves)
             if len(self.tuning) > 3:
                 eng.set_bonded_mode(self.tuning[3])
             if len(self.tuning) > 4:
